@@ -1,0 +1,99 @@
+"""ctypes binding of libdnspir.so (the HIP engine behind include/dnspir.h).
+
+The product path always runs through this library; there is no CPU fallback.
+Importing works without a GPU (the client functions are host code); device
+calls fail with DNSPIR_E_NOINIT when no GPU is present.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libdnspir.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "dnspir.h")
+
+N = 2048
+LANES = 8
+
+ERRORS = {
+    -1: "DNSPIR_E_SHORT_QUERY",
+    -2: "DNSPIR_E_CONJ",
+    -3: "DNSPIR_E_COUNT",
+    -4: "DNSPIR_E_SHAPE",
+    -5: "DNSPIR_E_ARG",
+    -6: "DNSPIR_E_ALIGN",
+    -7: "DNSPIR_E_OOM",
+    -8: "DNSPIR_E_HIP",
+    -9: "DNSPIR_E_DECODE",
+    -10: "DNSPIR_E_NOINIT",
+}
+
+
+class DnspirError(RuntimeError):
+    def __init__(self, code, where):
+        self.code = code
+        super().__init__(f"{where}: {ERRORS.get(code, code)} ({code})")
+
+
+class Timing(ctypes.Structure):
+    _fields_ = [("primary_us", ctypes.c_double), ("secondary_us", ctypes.c_double),
+                ("total_us", ctypes.c_double), ("bytes_streamed", ctypes.c_uint64)]
+
+
+vp = ctypes.c_void_p
+sz = ctypes.c_size_t
+ci = ctypes.c_int
+u8p = ctypes.POINTER(ctypes.c_uint8)
+u16p = ctypes.POINTER(ctypes.c_uint16)
+u64p = ctypes.POINTER(ctypes.c_uint64)
+ip = ctypes.POINTER(ctypes.c_int)
+
+SIGNATURES = {
+    "dnspir_init": (ci, [ci]),
+    "dnspir_strerror": (ctypes.c_char_p, [ci]),
+    "dnspir_required_bytes": (sz, [ci]),
+    "dnspir_shard_create": (ci, [ci, ci, vp, sz, ctypes.POINTER(vp)]),
+    "dnspir_shard_destroy": (ci, [vp]),
+    "dnspir_shard_group_order": (ci, [vp]),
+    "dnspir_shard_set_db": (ci, [vp, u16p, sz]),
+    "dnspir_shard_get_a": (ci, [vp, u64p]),
+    "dnspir_shard_perform_pir": (ci, [vp, u64p, u64p, u64p, u64p]),
+    "dnspir_shard_perform_pir_split": (ci, [vp, u64p, u64p, u64p, u64p]),
+    "dnspir_ctx_create": (ci, [vp, ctypes.POINTER(vp)]),
+    "dnspir_ctx_destroy": (ci, [vp]),
+    "dnspir_ctx_stream": (vp, [vp]),
+    "dnspir_double_pir": (ci, [vp, vp, sz, vp, sz, u64p, u64p, u64p, u64p, ctypes.POINTER(Timing)]),
+    "dnspir_process_queries": (ci, [vp, vp, sz, vp, sz, u8p, sz, u8p, sz, u8p, u8p, ctypes.POINTER(Timing)]),
+    "dnspir_process_queries_device": (ci, [vp, vp, sz, vp, sz, vp, sz, vp, sz, vp, vp, ctypes.POINTER(Timing)]),
+    "dnspir_database_shape": (None, [sz, ip, ip]),
+    "dnspir_num_databases": (sz, [sz]),
+    "dnspir_choose_index_group_secondary": (ci, [sz]),
+    "dnspir_query_bytes": (sz, [ci, ci]),
+    "dnspir_reply_bytes": (sz, []),
+    "dnspir_rng_new": (vp, [u8p]),
+    "dnspir_rng_free": (None, [vp]),
+    "dnspir_prepare_query": (ci, [vp, sz, sz, ci, ci, u8p, ctypes.POINTER(sz), u8p]),
+    "dnspir_process_reply": (ci, [u8p, sz, u8p, u16p]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libdnspir.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build the HIP engine first (__graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc, where):
+    if rc != 0:
+        raise DnspirError(rc, where)
+    return rc

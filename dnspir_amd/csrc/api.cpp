@@ -1,0 +1,797 @@
+// C ABI of the MI355X PIR response engine (include/dnspir.h): shard
+// lifecycle, query contexts, the two-level composition and the byte API,
+// plus the host-side client operations.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <map>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "../../include/dnspir.h"
+#include "dn_common.hpp"
+#include "host_math.hpp"
+#include "kernels.hpp"
+
+using namespace dn;
+
+namespace {
+
+constexpr int A_BITS = 17, B_BITS = 12, GK_BITS = 50, QRY_BITS = 34;
+constexpr size_t REPLY_BYTES = 3 * (N * A_BITS / 8 + N * B_BITS / 8);
+constexpr size_t EXTRA_BYTES = (size_t)LANES * N * 8 + (size_t)LANES * N * 4;  // a_coef + a17
+constexpr size_t PTR_SLOTS = 2 * 2 * 256 + 1;  // two fleets x (slabs, a17s) + one scratch slot
+
+#define HIP_OK(x)                                     \
+    do {                                              \
+        if ((x) != hipSuccess) return DNSPIR_E_HIP;   \
+    } while (0)
+
+int loop_len(int l) { return l < N ? l : N / 2; }
+int nstreams(int l) { return l == N ? 2 : 1; }
+bool valid_group(int l) {
+    if (l <= 0 || l > N || (N % l) != 0) return false;
+    return l == N || 2 * l <= N;  // base_pir.rs:150-154
+}
+size_t slab_bytes(int l) { return (size_t)6 * N * l * 32; }  // (GK_DIGITS+1)*2*N*l elements of 32 B
+
+// grow-only device buffer
+struct DevBuf {
+    void* p = nullptr;
+    size_t n = 0;
+    int ensure(size_t bytes) {
+        if (bytes <= n) return 0;
+        if (p) hipFree(p);
+        p = nullptr;
+        n = 0;
+        if (hipMalloc(&p, bytes) != hipSuccess) return DNSPIR_E_OOM;
+        n = bytes;
+        return 0;
+    }
+    template <class X>
+    X* as() const { return reinterpret_cast<X*>(p); }
+    ~DevBuf() {
+        if (p) hipFree(p);
+    }
+};
+
+struct Global {
+    std::mutex mu;
+    int device = -1;
+    DevTables* tables = nullptr;
+    hipStream_t pre_stream = nullptr;
+    DevBuf pre_entries, pre_M, pre_te, pre_ip, pre_aout, pre_idx;
+    dnspir_ctx* default_ctx = nullptr;
+};
+Global g;
+
+// ---- device tables ----------------------------------------------------------
+uint64_t find_root_2n(uint64_t p) {
+    for (uint64_t c = 3;; c += 2) {
+        uint64_t z = dnh::powmod(c, (p - 1) / M2N, p);
+        if (dnh::powmod(z, N, p) == p - 1) return z;
+    }
+}
+uint32_t shoup(uint64_t w, uint64_t p) { return (uint32_t)(((unsigned __int128)w << 32) / p); }
+int brv11(int v) {
+    int r = 0;
+    for (int b = 0; b < LOGN; ++b)
+        if (v & (1 << b)) r |= 1 << (LOGN - 1 - b);
+    return r;
+}
+// host replica of the device forward NTT (same butterflies)
+void host_ntt_fwd(std::vector<uint64_t>& a, uint64_t p, const uint32_t* psi) {
+    for (int s = 0; s < LOGN; ++s) {
+        int m = 1 << s, t = N >> (s + 1);
+        for (int i = 0; i < m; ++i)
+            for (int j = 2 * i * t; j < 2 * i * t + t; ++j) {
+                uint64_t U = a[j], V = a[j + t] * psi[m + i] % p;
+                a[j] = (U + V) % p;
+                a[j + t] = (U + p - V) % p;
+            }
+    }
+}
+
+int build_tables(DevTables* T) {
+    memset(T, 0, sizeof(*T));
+    for (int r = 0; r < 2; ++r) {
+        const uint64_t p = mod_p(r);
+        const uint64_t psi = find_root_2n(p);
+        const uint64_t ipsi = dnh::powmod(psi, p - 2, p);
+        for (int k = 0; k < N; ++k) {
+            uint64_t w = dnh::powmod(psi, brv11(k), p), wi = dnh::powmod(ipsi, brv11(k), p);
+            T->psi[r][k] = (uint32_t)w;
+            T->psiq[r][k] = shoup(w, p);
+            T->ipsi[r][k] = (uint32_t)wi;
+            T->ipsiq[r][k] = shoup(wi, p);
+        }
+        uint64_t ni = dnh::powmod(N, p - 2, p);
+        T->ninv[r] = (uint32_t)ni;
+        T->ninvq[r] = shoup(ni, p);
+        uint64_t x = 1;
+        for (int k = 0; k < M2N; ++k) {
+            T->psipow[r][k] = (uint32_t)x;
+            T->psipowq[r][k] = shoup(x, p);
+            x = x * psi % p;
+        }
+    }
+    // slot exponents: forward NTT of X gives psi^e_s in slot s
+    {
+        std::vector<uint64_t> a(N, 0);
+        a[1] = 1;
+        host_ntt_fwd(a, P1, T->psi[0]);
+        std::map<uint64_t, int> dlog;
+        for (int k = 0; k < M2N; ++k) dlog[T->psipow[0][k]] = k;
+        for (int k = 0; k < M2N; ++k) T->exp_slot[k] = 0xFFFF;
+        for (int s = 0; s < N; ++s) {
+            auto it = dlog.find(a[s]);
+            if (it == dlog.end() || (it->second & 1) == 0) return DNSPIR_E_HIP;
+            T->slot_exp[s] = (uint16_t)it->second;
+            T->exp_slot[it->second] = (uint16_t)s;
+        }
+        // the same exponents mod p2 (both moduli share the slot order)
+        std::vector<uint64_t> b(N, 0);
+        b[1] = 1;
+        host_ntt_fwd(b, P2, T->psi[1]);
+        for (int s = 0; s < N; ++s)
+            if (b[s] != T->psipow[1][T->slot_exp[s]]) return DNSPIR_E_HIP;
+    }
+    uint64_t f = 1, fi = 1;
+    const uint64_t inv5 = [] {
+        for (uint64_t x = 1; x < (uint64_t)M2N; x += 2)
+            if ((5 * x) % M2N == 1) return x;
+        return (uint64_t)0;
+    }();
+    for (int i = 0; i < HALFN; ++i) {
+        T->pow5[i] = (uint16_t)f;
+        T->inv5pow[i] = (uint16_t)fi;
+        f = f * 5 % M2N;
+        fi = fi * inv5 % M2N;
+    }
+    for (int s = 0; s < N; ++s) T->sigma5[s] = T->exp_slot[(5 * T->slot_exp[s]) % M2N];
+    for (int gp = 0; gp < N; ++gp) {
+        int e = gp < HALFN ? T->pow5[gp] : (M2N - T->pow5[gp - HALFN]) % M2N;
+        T->gpos_slot[gp] = T->exp_slot[e];
+        T->slot_gpos[T->exp_slot[e]] = (uint16_t)gp;
+    }
+    std::vector<uint64_t> c(N), v(N);
+    for (int sd = 0; sd < 6; ++sd) {
+        dnh::expand(dnh::SEEDS[sd], c.data());
+        for (int r = 0; r < 2; ++r) {
+            const uint64_t p = mod_p(r);
+            for (int i = 0; i < N; ++i) v[i] = c[i] % p;
+            host_ntt_fwd(v, p, T->psi[r]);
+            uint32_t* dst = sd < 2 ? T->gka[sd][r] : T->qa[sd - 2][r];
+            for (int i = 0; i < N; ++i) dst[i] = (uint32_t)v[i];
+        }
+    }
+    return 0;
+}
+
+}  // namespace
+
+// ---- shard and context types --------------------------------------------------
+struct dnspir_shard {
+    int l, L, nst, seeds;
+    uint32_t* slab;
+    uint64_t* a_coef;  // [8][N] mod q, the get_a result
+    uint32_t* a17;     // [8][N] a mod-switched to 17 bits
+    bool has_db;
+};
+
+struct dnspir_ctx {
+    hipStream_t stream = nullptr;
+    bool own = false;
+    DevBuf se, M, te, aout, cout, qcoef, qs, qp, acc, acoef, bcoef, a17o, b12o, ptrs, idx, qbytes, rbytes;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    std::vector<const dnspir_shard*> cached[2];
+    std::vector<uint64_t> stage;
+    int h_idx[18];  // host staging of the lane lists (outlives the async copy)
+};
+
+namespace {
+
+int ctx_init(dnspir_ctx* c, hipStream_t s) {
+    if (s) c->stream = s;
+    else {
+        HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        c->own = true;
+    }
+    for (auto& e : c->ev) HIP_OK(hipEventCreate(&e));
+    return 0;
+}
+
+int ensure_init() {
+    if (g.device < 0 || !g.tables) return DNSPIR_E_NOINIT;
+    return hipSetDevice(g.device) == hipSuccess ? 0 : DNSPIR_E_HIP;
+}
+
+dnspir_ctx* default_ctx() { return g.default_ctx; }
+
+// rows per K1 block: enough blocks to fill the chip, few atomics per row
+int k1_rows(int L, int nsh, int nst) {
+    const int per_chunk_blocks = (N / 128) * nsh * 2 * nst;
+    int chunks = (2048 + per_chunk_blocks - 1) / per_chunk_blocks;
+    chunks = std::max(1, std::min(chunks, L / 16 > 0 ? L / 16 : 1));
+    return (L + chunks - 1) / chunks;
+}
+
+// uploads the fleet's slab and a17 pointer lists (cached per context)
+int upload_fleet(dnspir_ctx* c, int slot, const dnspir_shard* const* fleet, size_t len, const uint32_t* const** slabs_dev,
+                 const uint32_t* const** a17_dev) {
+    int rc = c->ptrs.ensure(PTR_SLOTS * sizeof(void*));
+    if (rc) return rc;
+    const uint32_t** base = c->ptrs.as<const uint32_t*>() + slot * 512;
+    *slabs_dev = base;
+    *a17_dev = base + 256;
+    std::vector<const dnspir_shard*> cur(fleet, fleet + len);
+    if (cur != c->cached[slot]) {
+        std::vector<const uint32_t*> h(512, nullptr);
+        for (size_t i = 0; i < len; ++i) {
+            h[i] = fleet[i]->slab;
+            h[256 + i] = fleet[i]->a17;
+        }
+        HIP_OK(hipMemcpyAsync(base, h.data(), 512 * sizeof(void*), hipMemcpyHostToDevice, c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
+        c->cached[slot] = cur;
+    }
+    return 0;
+}
+
+int check_fleet(const dnspir_shard* const* fleet, size_t len, int l_p) {
+    if (!fleet || len == 0 || len > (size_t)(N / LANES)) return DNSPIR_E_SHAPE;
+    for (size_t i = 0; i < len; ++i) {
+        if (!fleet[i] || !fleet[i]->has_db || fleet[i]->seeds != 0) return DNSPIR_E_ARG;
+        if (fleet[i]->l != l_p) return DNSPIR_E_CONJ;  // double_pir.rs:160 (same shape throughout)
+    }
+    return 0;
+}
+
+// Runs perform_batched_double_pir on ring-element queries already in
+// c->qcoef ([2][6][N]); leaves the reply in c->acoef/bcoef/a17o/b12o with
+// lane index li = (#present slots before s)*3 + part.
+int run_double(dnspir_ctx* c, const dnspir_shard* const* fleets[2], const size_t lens[2], const bool present[2],
+               dnspir_timing* tm) {
+    const hipStream_t s = c->stream;
+    const int l_p = fleets[0][0]->l;
+    const int L_p = loop_len(l_p), nst_p = nstreams(l_p);
+    const int l_s = dnspir_choose_index_group_secondary(lens[0]);
+    const int L_s = loop_len(l_s), nst_s = nstreams(l_s);
+    int rc;
+    size_t maxlen = std::max(present[0] ? lens[0] : 0, present[1] ? lens[1] : 0);
+    if ((rc = c->qs.ensure((size_t)2 * 6 * 2 * N * 4))) return rc;
+    if ((rc = c->qp.ensure((size_t)2 * 4 * 2 * N * 4))) return rc;
+    if ((rc = c->acc.ensure(std::max<size_t>(1, maxlen) * 2 * N * LANES * 8))) return rc;
+    if ((rc = c->se.ensure((size_t)l_s * LANES * N * 2))) return rc;
+    int nlanes = 0;
+    int h_lanes[6], h_ncols[6], h_slot[6];
+    for (int sl = 0; sl < 2; ++sl)
+        if (present[sl])
+            for (int p = 0; p < 3; ++p) {
+                h_lanes[nlanes] = 3 * sl + p;
+                h_ncols[nlanes] = (int)(LANES * lens[sl]);
+                h_slot[nlanes] = sl;
+                ++nlanes;
+            }
+    if ((rc = c->M.ensure((size_t)nlanes * l_s * N * 4))) return rc;
+    if ((rc = c->te.ensure((size_t)nst_s * nlanes * L_s * 2 * N * 4))) return rc;
+    if ((rc = c->aout.ensure((size_t)nst_s * nlanes * 2 * N * 4))) return rc;
+    if ((rc = c->cout.ensure((size_t)nst_s * nlanes * 2 * N * 4))) return rc;
+    if ((rc = c->acoef.ensure((size_t)6 * N * 8))) return rc;
+    if ((rc = c->bcoef.ensure((size_t)6 * N * 8))) return rc;
+    if ((rc = c->a17o.ensure((size_t)6 * N * 4))) return rc;
+    if ((rc = c->b12o.ensure((size_t)6 * N * 4))) return rc;
+    if ((rc = c->idx.ensure(64 * sizeof(int)))) return rc;
+    int* h_idx = c->h_idx;
+    for (int i = 0; i < 6; ++i) {
+        h_idx[i] = i < nlanes ? h_lanes[i] : 0;
+        h_idx[6 + i] = i < nlanes ? h_ncols[i] : 0;
+        h_idx[12 + i] = i < nlanes ? h_slot[i] : 0;
+    }
+    HIP_OK(hipMemcpyAsync(c->idx.p, h_idx, sizeof(c->h_idx), hipMemcpyHostToDevice, s));
+    const int* d_lanes = c->idx.as<int>();
+    const int* d_ncols = d_lanes + 6;
+    const int* d_slot = d_lanes + 12;
+    const uint32_t* const* slabs[2] = {nullptr, nullptr};
+    const uint32_t* const* a17s[2] = {nullptr, nullptr};
+    for (int sl = 0; sl < 2; ++sl)
+        if (present[sl] && (rc = upload_fleet(c, sl, fleets[sl], lens[sl], &slabs[sl], &a17s[sl]))) return rc;
+
+    HIP_OK(hipEventRecord(c->ev[0], s));
+    launch_qprep(c->qcoef.as<uint64_t>(), c->qs.as<uint32_t>(), c->qp.as<uint32_t>(), 2, g.tables, s);
+    uint64_t streamed = 0;
+    for (int sl = 0; sl < 2; ++sl) {
+        if (!present[sl]) continue;
+        const int len = (int)lens[sl];
+        unsigned long long* acc = c->acc.as<unsigned long long>();
+        launch_fill_u64(acc, (size_t)len * 2 * N * LANES, 0, s);
+        launch_k1(slabs[sl], len, c->qp.as<uint32_t>() + (size_t)sl * 4 * 2 * N, acc, L_p, nst_p, k1_rows(L_p, len, nst_p), s);
+        launch_epi_primary(acc, len, a17s[sl], c->se.as<uint16_t>(), nullptr, sl, g.tables, s);
+        streamed += (uint64_t)len * slab_bytes(l_p);
+    }
+    HIP_OK(hipEventRecord(c->ev[1], s));
+    launch_k5(c->se.as<uint16_t>(), c->M.as<uint32_t>(), c->te.as<uint32_t>(), l_s, d_ncols, d_lanes, nlanes, g.tables, s);
+    launch_chain_sec(c->te.as<uint32_t>(), c->aout.as<uint32_t>(), c->cout.as<uint32_t>(), c->qs.as<uint32_t>(), d_slot, L_s,
+                     nst_s, nlanes, g.tables, s);
+    launch_chain_out(c->aout.as<uint32_t>(), c->cout.as<uint32_t>(), nst_s, nlanes, c->acoef.as<uint64_t>(),
+                     c->bcoef.as<uint64_t>(), c->a17o.as<uint32_t>(), c->b12o.as<uint32_t>(), g.tables, s);
+    HIP_OK(hipEventRecord(c->ev[2], s));
+    if (tm) tm->bytes_streamed = streamed;
+    return 0;
+}
+
+void fill_timing(dnspir_ctx* c, dnspir_timing* tm, double wall_us) {
+    if (!tm) return;
+    float a = 0, b = 0;
+    hipEventElapsedTime(&a, c->ev[0], c->ev[1]);
+    hipEventElapsedTime(&b, c->ev[1], c->ev[2]);
+    tm->primary_us = a * 1000.0;
+    tm->secondary_us = b * 1000.0;
+    tm->total_us = wall_us;
+}
+
+double now_us() {
+    return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* dnspir_strerror(int code) {
+    switch (code) {
+        case DNSPIR_OK: return "ok";
+        case DNSPIR_E_SHORT_QUERY: return "not enough data";
+        case DNSPIR_E_CONJ: return "conjugate query presence does not match the index group";
+        case DNSPIR_E_COUNT: return "entry count does not match the index group";
+        case DNSPIR_E_SHAPE: return "invalid index group or fleet shape";
+        case DNSPIR_E_ARG: return "invalid argument";
+        case DNSPIR_E_ALIGN: return "device memory misaligned or too small";
+        case DNSPIR_E_OOM: return "device allocation failed";
+        case DNSPIR_E_HIP: return "HIP runtime error";
+        case DNSPIR_E_DECODE: return "reply digits out of range";
+        case DNSPIR_E_NOINIT: return "dnspir_init has not been called";
+        default: return "unknown error";
+    }
+}
+
+int dnspir_init(int device) {
+    std::lock_guard<std::mutex> lk(g.mu);
+    if (g.tables && g.device == device) return 0;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= device || device < 0) return DNSPIR_E_NOINIT;
+    HIP_OK(hipSetDevice(device));
+    DevTables* h = new (std::nothrow) DevTables;
+    if (!h) return DNSPIR_E_OOM;
+    int rc = build_tables(h);
+    if (rc) {
+        delete h;
+        return rc;
+    }
+    DevTables* d = nullptr;
+    if (hipMalloc(&d, sizeof(DevTables)) != hipSuccess) {
+        delete h;
+        return DNSPIR_E_OOM;
+    }
+    if (hipMemcpy(d, h, sizeof(DevTables), hipMemcpyHostToDevice) != hipSuccess) {
+        delete h;
+        return DNSPIR_E_HIP;
+    }
+    delete h;
+    g.tables = d;
+    g.device = device;
+    HIP_OK(hipStreamCreateWithFlags(&g.pre_stream, hipStreamNonBlocking));
+    g.default_ctx = new dnspir_ctx;
+    return ctx_init(g.default_ctx, nullptr);
+}
+
+size_t dnspir_required_bytes(int group_order) {
+    if (!valid_group(group_order)) return 0;
+    return slab_bytes(group_order) + EXTRA_BYTES;
+}
+
+int dnspir_shard_create(int group_order, int used_seeds, void* dev_mem, size_t bytes, dnspir_shard** out) {
+    if (!out || !dev_mem) return DNSPIR_E_ARG;
+    if (!valid_group(group_order)) return DNSPIR_E_SHAPE;
+    if (used_seeds != 0 && used_seeds != 1) return DNSPIR_E_ARG;
+    if (((uintptr_t)dev_mem & 255) != 0 || bytes < dnspir_required_bytes(group_order)) return DNSPIR_E_ALIGN;
+    dnspir_shard* s = new (std::nothrow) dnspir_shard;
+    if (!s) return DNSPIR_E_OOM;
+    s->l = group_order;
+    s->L = loop_len(group_order);
+    s->nst = nstreams(group_order);
+    s->seeds = used_seeds;
+    s->slab = (uint32_t*)dev_mem;
+    s->a_coef = (uint64_t*)((char*)dev_mem + slab_bytes(group_order));
+    s->a17 = (uint32_t*)((char*)s->a_coef + (size_t)LANES * N * 8);
+    s->has_db = false;
+    *out = s;
+    return 0;
+}
+
+int dnspir_shard_destroy(dnspir_shard* s) {
+    delete s;
+    return 0;
+}
+
+int dnspir_shard_group_order(const dnspir_shard* s) { return s ? s->l : 0; }
+
+int dnspir_shard_set_db(dnspir_shard* sh, const uint16_t* entries, size_t n_entries) {
+    if (!sh || !entries) return DNSPIR_E_ARG;
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (n_entries != (size_t)LANES * sh->l) return DNSPIR_E_COUNT;
+    std::lock_guard<std::mutex> lk(g.mu);
+    const int l = sh->l, L = sh->L, nst = sh->nst;
+    const hipStream_t s = g.pre_stream;
+    if ((rc = g.pre_entries.ensure(n_entries * N * 2))) return rc;
+    if ((rc = g.pre_M.ensure((size_t)LANES * l * N * 4))) return rc;
+    if ((rc = g.pre_te.ensure((size_t)nst * LANES * L * 2 * N * 4))) return rc;
+    if ((rc = g.pre_ip.ensure((size_t)nst * LANES * L * 4 * N * 4))) return rc;
+    if ((rc = g.pre_aout.ensure((size_t)nst * LANES * 2 * N * 4))) return rc;
+    if ((rc = g.pre_idx.ensure(16 * sizeof(int)))) return rc;
+    int h_idx[16];
+    for (int i = 0; i < 8; ++i) {
+        h_idx[i] = i;
+        h_idx[8 + i] = l;
+    }
+    HIP_OK(hipMemcpyAsync(g.pre_idx.p, h_idx, sizeof(h_idx), hipMemcpyHostToDevice, s));
+    HIP_OK(hipMemcpyAsync(g.pre_entries.p, entries, n_entries * N * 2, hipMemcpyHostToDevice, s));
+    launch_k5(g.pre_entries.as<uint16_t>(), g.pre_M.as<uint32_t>(), g.pre_te.as<uint32_t>(), l, g.pre_idx.as<int>() + 8,
+              g.pre_idx.as<int>(), LANES, g.tables, s);
+    launch_chain_pre(g.pre_te.as<uint32_t>(), g.pre_ip.as<uint32_t>(), g.pre_aout.as<uint32_t>(), L, nst,
+                     sh->seeds == 0 ? 0 : 2, g.tables, s);
+    launch_relayout(g.pre_te.as<uint32_t>(), g.pre_ip.as<uint32_t>(), sh->slab, L, nst, g.tables, s);
+    launch_chain_out(g.pre_aout.as<uint32_t>(), nullptr, nst, LANES, sh->a_coef, nullptr, sh->a17, nullptr, g.tables, s);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(s));
+    sh->has_db = true;
+    return 0;
+}
+
+int dnspir_shard_get_a(const dnspir_shard* sh, uint64_t* out) {
+    if (!sh || !out) return DNSPIR_E_ARG;
+    int rc = ensure_init();
+    if (rc) return rc;
+    HIP_OK(hipMemcpy(out, sh->a_coef, (size_t)LANES * N * 8, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+static int perform_common(const dnspir_shard* sh, const uint64_t* qb, const uint64_t* qbc, const uint64_t* gkb,
+                          uint64_t* out, bool split) {
+    if (!sh || !qb || !gkb || !out) return DNSPIR_E_ARG;
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (!sh->has_db) return DNSPIR_E_ARG;
+    if (sh->nst == 2 && !qbc) return DNSPIR_E_CONJ;  // base_pir.rs:965
+    std::lock_guard<std::mutex> lk(g.mu);
+    dnspir_ctx* c = default_ctx();
+    const hipStream_t s = c->stream;
+    const int nq = split ? LANES : 1;
+    if ((rc = c->qcoef.ensure((size_t)LANES * 6 * N * 8))) return rc;
+    if ((rc = c->qs.ensure((size_t)LANES * 6 * 2 * N * 4))) return rc;
+    if ((rc = c->qp.ensure((size_t)LANES * 4 * 2 * N * 4))) return rc;
+    if ((rc = c->acc.ensure((size_t)2 * N * LANES * 8))) return rc;
+    if ((rc = c->bcoef.ensure((size_t)LANES * N * 8))) return rc;
+    if ((rc = c->ptrs.ensure(PTR_SLOTS * sizeof(void*)))) return rc;
+    c->stage.assign((size_t)nq * 6 * N, 0);
+    for (int q = 0; q < nq; ++q) {
+        uint64_t* blk = c->stage.data() + (size_t)q * 6 * N;
+        memcpy(blk, gkb + (size_t)q * 2 * N, 2 * N * 8);
+        memcpy(blk + 2 * N, qb + (size_t)q * N, N * 8);
+        if (sh->nst == 2) memcpy(blk + 3 * N, qbc + (size_t)q * N, N * 8);
+    }
+    HIP_OK(hipMemcpyAsync(c->qcoef.p, c->stage.data(), c->stage.size() * 8, hipMemcpyHostToDevice, s));
+    const uint32_t* slab = sh->slab;
+    const uint32_t** pslab = c->ptrs.as<const uint32_t*>() + (PTR_SLOTS - 1);  // scratch slot outside the fleet cache
+    HIP_OK(hipMemcpyAsync((void*)pslab, &slab, sizeof(slab), hipMemcpyHostToDevice, s));
+    launch_qprep(c->qcoef.as<uint64_t>(), c->qs.as<uint32_t>(), c->qp.as<uint32_t>(), nq, g.tables, s);
+    unsigned long long* acc = c->acc.as<unsigned long long>();
+    launch_fill_u64(acc, (size_t)2 * N * LANES, 0, s);
+    if (split) launch_k1_split(pslab, 1, c->qp.as<uint32_t>(), acc, sh->L, sh->nst, k1_rows(sh->L, 1, sh->nst), s);
+    else launch_k1(pslab, 1, c->qp.as<uint32_t>(), acc, sh->L, sh->nst, k1_rows(sh->L, 1, sh->nst), s);
+    launch_epi_primary(acc, 1, nullptr, nullptr, c->bcoef.as<uint64_t>(), 0, g.tables, s);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(out, c->bcoef.p, (size_t)LANES * N * 8, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    return 0;
+}
+
+int dnspir_shard_perform_pir(const dnspir_shard* sh, const uint64_t* qb, const uint64_t* qbc, const uint64_t* gkb,
+                             uint64_t* out) {
+    return perform_common(sh, qb, qbc, gkb, out, false);
+}
+
+int dnspir_shard_perform_pir_split(const dnspir_shard* sh, const uint64_t* qb, const uint64_t* qbc, const uint64_t* gkb,
+                                   uint64_t* out) {
+    return perform_common(sh, qb, qbc, gkb, out, true);
+}
+
+int dnspir_ctx_create(void* stream, dnspir_ctx** out) {
+    if (!out) return DNSPIR_E_ARG;
+    int rc = ensure_init();
+    if (rc) return rc;
+    dnspir_ctx* c = new (std::nothrow) dnspir_ctx;
+    if (!c) return DNSPIR_E_OOM;
+    rc = ctx_init(c, (hipStream_t)stream);
+    if (rc) {
+        delete c;
+        return rc;
+    }
+    *out = c;
+    return 0;
+}
+
+int dnspir_ctx_destroy(dnspir_ctx* c) {
+    if (!c) return DNSPIR_E_ARG;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (auto& e : c->ev)
+        if (e) hipEventDestroy(e);
+    if (c->own && c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+void* dnspir_ctx_stream(dnspir_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int dnspir_choose_index_group_secondary(size_t num_shards) {
+    int log2l = 0;
+    while (((size_t)1 << log2l) < num_shards) ++log2l;
+    return LANES << log2l;
+}
+
+void dnspir_database_shape(size_t n, int* lp, int* ls) {
+    const int l_p = n <= (size_t)N * N / 16 ? N / 2 : N;  // pir_wrapper.rs:188, 223-227
+    const size_t shards = (n + (size_t)LANES * l_p - 1) / ((size_t)LANES * l_p);
+    if (lp) *lp = l_p;
+    if (ls) *ls = dnspir_choose_index_group_secondary(shards);
+}
+
+size_t dnspir_num_databases(size_t n) {
+    int lp = 0;
+    dnspir_database_shape(n, &lp, nullptr);
+    return (n + (size_t)LANES * lp - 1) / ((size_t)LANES * lp);
+}
+
+size_t dnspir_query_bytes(int lp, int ls) {
+    size_t b = 2 * (N / 8) * GK_BITS + 2 * (N / 8) * QRY_BITS;
+    if (lp == N) b += (N / 8) * QRY_BITS;
+    if (ls == N) b += (N / 8) * QRY_BITS;
+    return b;
+}
+
+size_t dnspir_reply_bytes(void) { return REPLY_BYTES; }
+
+int dnspir_double_pir(dnspir_ctx* c, const dnspir_shard* const* f0, size_t len0, const dnspir_shard* const* f1,
+                      size_t len1, const uint64_t* q0, const uint64_t* q1, uint64_t* out0, uint64_t* out1,
+                      dnspir_timing* tm) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (!c) c = default_ctx();
+    if (!f0 || len0 == 0) return DNSPIR_E_SHAPE;
+    const int l_p = f0[0] ? f0[0]->l : 0;
+    const bool present[2] = {q0 != nullptr, q1 != nullptr};
+    const dnspir_shard* const* fleets[2] = {f0, f1};
+    const size_t lens[2] = {len0, len1};
+    if (len0 < len1) return DNSPIR_E_SHAPE;  // pir_wrapper.rs:362
+    for (int sl = 0; sl < 2; ++sl)
+        if (present[sl] && (rc = check_fleet(fleets[sl], lens[sl], l_p))) return rc;
+    std::lock_guard<std::mutex> lk(g.mu);
+    const double t0 = now_us();
+    if ((rc = c->qcoef.ensure((size_t)LANES * 6 * N * 8))) return rc;
+    const uint64_t* qs[2] = {q0, q1};
+    for (int sl = 0; sl < 2; ++sl)
+        if (present[sl])
+            HIP_OK(hipMemcpyAsync(c->qcoef.as<uint64_t>() + (size_t)sl * 6 * N, qs[sl], (size_t)6 * N * 8,
+                                  hipMemcpyHostToDevice, c->stream));
+    if ((rc = run_double(c, fleets, lens, present, tm))) return rc;
+    HIP_OK(hipGetLastError());
+    std::vector<uint64_t> a(6 * N), b(6 * N);
+    HIP_OK(hipMemcpyAsync(a.data(), c->acoef.p, a.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipMemcpyAsync(b.data(), c->bcoef.p, b.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    uint64_t* outs[2] = {out0, out1};
+    int li = 0;
+    for (int sl = 0; sl < 2; ++sl) {
+        if (!present[sl]) continue;
+        for (int p = 0; p < 3; ++p, ++li) {
+            if (!outs[sl]) continue;
+            memcpy(outs[sl] + ((size_t)p * 2 + 0) * N, a.data() + (size_t)li * N, N * 8);
+            memcpy(outs[sl] + ((size_t)p * 2 + 1) * N, b.data() + (size_t)li * N, N * 8);
+        }
+    }
+    fill_timing(c, tm, now_us() - t0);
+    return 0;
+}
+
+static int process_device_impl(dnspir_ctx* c, const dnspir_shard* const* f0, size_t len0, const dnspir_shard* const* f1,
+                               size_t len1, const uint8_t* q0, size_t ql0, const uint8_t* q1, size_t ql1, uint8_t* r0,
+                               uint8_t* r1, dnspir_timing* tm, double t0) {
+    int rc;
+    if (!f0 || len0 == 0 || !f0[0]) return DNSPIR_E_SHAPE;
+    if (len0 < len1) return DNSPIR_E_SHAPE;
+    const int l_p = f0[0]->l;
+    const int l_s = dnspir_choose_index_group_secondary(len0);
+    const bool present[2] = {q0 != nullptr, q1 != nullptr};
+    const dnspir_shard* const* fleets[2] = {f0, f1};
+    const size_t lens[2] = {len0, len1};
+    const uint8_t* qb[2] = {q0, q1};
+    const size_t ql[2] = {ql0, ql1};
+    for (int sl = 0; sl < 2; ++sl)
+        if (present[sl] && (rc = check_fleet(fleets[sl], lens[sl], l_p))) return rc;
+    if ((rc = c->qcoef.ensure((size_t)LANES * 6 * N * 8))) return rc;
+    const int conj_p = l_p == N, conj_s = l_s == N;
+    for (int sl = 0; sl < 2; ++sl) {
+        if (!present[sl]) continue;
+        // pir_wrapper.rs:370-384: gk0 gk1 (50 bits) || primary (34) [|| conj] || secondary (34) [|| conj]
+        int offs[6], bits[6] = {GK_BITS, GK_BITS, QRY_BITS, QRY_BITS, QRY_BITS, QRY_BITS};
+        int pres[6] = {1, 1, 1, conj_p, 1, conj_s};
+        size_t off = 0;
+        for (int v = 0; v < 6; ++v) {
+            if (!pres[v]) {
+                offs[v] = -1;
+                continue;
+            }
+            offs[v] = (int)off;
+            off += dnh::condensed_size(bits[v], N);
+        }
+        if (ql[sl] < off) return DNSPIR_E_SHORT_QUERY;
+        launch_decode_query(qb[sl], offs, bits, c->qcoef.as<uint64_t>() + (size_t)sl * 6 * N, c->stream);
+    }
+    if ((rc = run_double(c, fleets, lens, present, tm))) return rc;
+    uint8_t* rs[2] = {r0, r1};
+    int li = 0;
+    for (int sl = 0; sl < 2; ++sl) {
+        if (!present[sl]) continue;
+        if (rs[sl]) launch_encode_reply(c->a17o.as<uint32_t>(), c->b12o.as<uint32_t>(), li, rs[sl], c->stream);
+        li += 3;
+    }
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(c->stream));
+    fill_timing(c, tm, now_us() - t0);
+    return 0;
+}
+
+int dnspir_process_queries_device(dnspir_ctx* c, const dnspir_shard* const* f0, size_t len0,
+                                  const dnspir_shard* const* f1, size_t len1, const uint8_t* q0, size_t ql0,
+                                  const uint8_t* q1, size_t ql1, uint8_t* r0, uint8_t* r1, dnspir_timing* tm) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (!c) c = default_ctx();
+    std::lock_guard<std::mutex> lk(g.mu);
+    return process_device_impl(c, f0, len0, f1, len1, q0, ql0, q1, ql1, r0, r1, tm, now_us());
+}
+
+int dnspir_process_queries(dnspir_ctx* c, const dnspir_shard* const* f0, size_t len0, const dnspir_shard* const* f1,
+                           size_t len1, const uint8_t* q0, size_t ql0, const uint8_t* q1, size_t ql1, uint8_t* r0,
+                           uint8_t* r1, dnspir_timing* tm) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (!c) c = default_ctx();
+    std::lock_guard<std::mutex> lk(g.mu);
+    const double t0 = now_us();
+    const size_t qcap = 80000;
+    if ((rc = c->qbytes.ensure(2 * qcap))) return rc;
+    if ((rc = c->rbytes.ensure(2 * REPLY_BYTES))) return rc;
+    const uint8_t* qh[2] = {q0, q1};
+    const size_t ql[2] = {ql0, ql1};
+    const uint8_t* qd[2] = {nullptr, nullptr};
+    for (int sl = 0; sl < 2; ++sl) {
+        if (!qh[sl]) continue;
+        const size_t n = std::min(ql[sl], qcap);
+        uint8_t* dst = c->qbytes.as<uint8_t>() + sl * qcap;
+        if (n) HIP_OK(hipMemcpyAsync(dst, qh[sl], n, hipMemcpyHostToDevice, c->stream));
+        qd[sl] = dst;
+    }
+    uint8_t* rd[2] = {r0 ? c->rbytes.as<uint8_t>() : nullptr, r1 ? c->rbytes.as<uint8_t>() + REPLY_BYTES : nullptr};
+    rc = process_device_impl(c, f0, len0, f1, len1, qd[0], std::min(ql0, qcap), qd[1], std::min(ql1, qcap), rd[0], rd[1],
+                             nullptr, t0);
+    if (rc) return rc;
+    if (r0 && q0) HIP_OK(hipMemcpy(r0, rd[0], REPLY_BYTES, hipMemcpyDeviceToHost));
+    if (r1 && q1) HIP_OK(hipMemcpy(r1, rd[1], REPLY_BYTES, hipMemcpyDeviceToHost));
+    fill_timing(c, tm, now_us() - t0);
+    if (tm) {
+        int lp = f0[0]->l;
+        tm->bytes_streamed = (uint64_t)((q0 ? len0 : 0) + (q1 ? len1 : 0)) * slab_bytes(lp);
+    }
+    return 0;
+}
+
+// ---- client ------------------------------------------------------------------
+struct dnspir_rng {
+    dnh::ChaCha12 r;
+    explicit dnspir_rng(const uint8_t* s) : r(s) {}
+};
+
+dnspir_rng* dnspir_rng_new(const uint8_t seed[32]) { return seed ? new (std::nothrow) dnspir_rng(seed) : nullptr; }
+void dnspir_rng_free(dnspir_rng* r) { delete r; }
+
+int dnspir_prepare_query(dnspir_rng* rng, size_t pidx, size_t sidx, int lp, int ls, uint8_t* query, size_t* qlen,
+                         uint8_t sk_seed[32]) {
+    if (!rng || !query || !qlen || !sk_seed) return DNSPIR_E_ARG;
+    if (!valid_group(lp) || !valid_group(ls)) return DNSPIR_E_SHAPE;
+    if (pidx >= (size_t)lp * LANES || sidx >= (size_t)ls / LANES) return DNSPIR_E_ARG;
+    for (int i = 0; i < 32; ++i) sk_seed[i] = (uint8_t)(rng->r.next_u32() & 0xFFFF);
+    std::vector<int64_t> sk(N);
+    {
+        dnh::ChaCha12 skr(sk_seed);
+        dnh::gen_sk(skr, 128, sk.data());
+    }
+    const int kp = N / 2 / loop_len(lp), ks = N / 2 / loop_len(ls);
+    const int64_t i = (int64_t)(pidx / LANES);
+    const int64_t j = (int64_t)(sidx * LANES + pidx % LANES);
+    std::vector<uint64_t> gk(2 * N), ct(N), tmp(N);
+    std::vector<uint16_t> mono(N);
+    auto set_mono = [&](int64_t e) {
+        std::fill(mono.begin(), mono.end(), 0);
+        e %= 2 * N;
+        if (e < 0) e += 2 * N;
+        if (e < N) mono[e] = 1;
+        else mono[e - N] = (uint16_t)(dnh::T - 1);
+    };
+    dnh::gen_gk_b(rng->r, sk.data(), gk.data());
+    size_t n = 0;
+    auto emit = [&](const uint64_t* x, int bits) {
+        for (int k = 0; k < N; ++k) tmp[k] = dnh::mod_switch_encode(x[k], bits);
+        n += dnh::condense(bits, tmp.data(), N, query + n);
+    };
+    emit(gk.data(), GK_BITS);
+    emit(gk.data() + N, GK_BITS);
+    set_mono(i * kp);
+    dnh::enc_sym_b(dnh::SEEDS[2], rng->r, sk.data(), mono.data(), ct.data());
+    std::vector<uint64_t> pq = ct, pqc, sq, sqc;
+    if (lp == N) {
+        set_mono(2 * N - i * kp);
+        dnh::enc_sym_b(dnh::SEEDS[3], rng->r, sk.data(), mono.data(), ct.data());
+        pqc = ct;
+    }
+    set_mono(j * ks);
+    dnh::enc_sym_b(dnh::SEEDS[4], rng->r, sk.data(), mono.data(), ct.data());
+    sq = ct;
+    if (ls == N) {
+        set_mono(2 * N - j * ks);
+        dnh::enc_sym_b(dnh::SEEDS[5], rng->r, sk.data(), mono.data(), ct.data());
+        sqc = ct;
+    }
+    emit(pq.data(), QRY_BITS);
+    if (lp == N) emit(pqc.data(), QRY_BITS);
+    emit(sq.data(), QRY_BITS);
+    if (ls == N) emit(sqc.data(), QRY_BITS);
+    *qlen = n;
+    return 0;
+}
+
+int dnspir_process_reply(const uint8_t* reply, size_t rlen, const uint8_t sk_seed[32], uint16_t* out) {
+    if (!reply || !sk_seed || !out) return DNSPIR_E_ARG;
+    std::vector<int64_t> sk(N);
+    {
+        dnh::ChaCha12 skr(sk_seed);
+        dnh::gen_sk(skr, 128, sk.data());
+    }
+    std::vector<uint64_t> tmp(N), a(N), b(N);
+    std::vector<uint16_t> parts(3 * N);
+    size_t off = 0;
+    for (int p = 0; p < 3; ++p) {
+        if (!dnh::uncondense(A_BITS, N, reply + off, rlen - off, tmp.data())) return DNSPIR_E_SHORT_QUERY;
+        off += dnh::condensed_size(A_BITS, N);
+        for (int k = 0; k < N; ++k) a[k] = dnh::mod_switch_decode(tmp[k], A_BITS);
+        if (!dnh::uncondense(B_BITS, N, reply + off, rlen - off, tmp.data())) return DNSPIR_E_SHORT_QUERY;
+        off += dnh::condensed_size(B_BITS, N);
+        for (int k = 0; k < N; ++k) b[k] = dnh::mod_switch_decode(tmp[k], B_BITS);
+        dnh::dec(sk.data(), a.data(), b.data(), parts.data() + (size_t)p * N);
+    }
+    std::vector<uint64_t> av(N), bv(N);
+    for (int k = 0; k < N; ++k) {
+        uint64_t v = 0;
+        for (int p = 2; p >= 0; --p) v = v * 1025 + parts[(size_t)p * N + k];
+        if ((v >> B_BITS) >= (1ull << A_BITS)) return DNSPIR_E_DECODE;  // double_pir.rs:88
+        av[k] = dnh::mod_switch_decode(v >> B_BITS, A_BITS);
+        bv[k] = dnh::mod_switch_decode(v & ((1ull << B_BITS) - 1), B_BITS);
+    }
+    dnh::dec(sk.data(), av.data(), bv.data(), out);
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,694 @@
+// HIP kernels of the PIR response engine (gfx950).
+//
+// Data layouts (all residues u32 in [0, p_r)):
+//   slot order   my negacyclic NTT's output order; slot s holds f(psi^e_s)
+//   Galois order position g = sgn*N/2 + a holds exponent (-1)^sgn 5^a, so
+//                sigma_5 is a cyclic shift by one inside each half
+//   shard slab   [r][stream][row i][array (TE, IPGK0, IPGK1)][gpos][lane];
+//                row i is stored pre-rotated by L-1-i inside each half so the
+//                streaming inner product reads it at the output position
+//                (the Galois scatter of base_pir.rs:1085-1247 made static)
+//   lane-major temporaries [stream][lane][row][...][r][slot] during encoding
+#include "dn_common.hpp"
+#include "kernels.hpp"
+
+namespace dn {
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// block-level transforms over LDS (radix 2, `nt` cooperating threads with
+// index `tid`; every thread of the block runs the same barrier sequence)
+// ---------------------------------------------------------------------------
+
+// forward negacyclic NTT (Cooley-Tukey, psi powers in bit-reversed order):
+// natural coefficients -> slot order
+__device__ void ntt_fwd(uint32_t* a, const DevTables* __restrict__ T, int r, int tid, int nt) {
+    const uint32_t p = mod_p(r);
+    const uint32_t* psi = T->psi[r];
+    const uint32_t* psiq = T->psiq[r];
+#pragma unroll 1
+    for (int s = 0; s < LOGN; ++s) {
+        const int m = 1 << s;
+        const int lt = LOGN - 1 - s;
+        const int t = 1 << lt;
+        for (int b = tid; b < N / 2; b += nt) {
+            const int i = b >> lt;
+            const int j = (i << (lt + 1)) + (b & (t - 1));
+            const uint32_t U = a[j];
+            const uint32_t V = mul_shoup(a[j + t], psi[m + i], psiq[m + i], p);
+            a[j] = add_mod(U, V, p);
+            a[j + t] = sub_mod(U, V, p);
+        }
+        __syncthreads();
+    }
+}
+
+// inverse negacyclic NTT (Gentleman-Sande): slot order -> coefficients
+__device__ void ntt_inv(uint32_t* a, const DevTables* __restrict__ T, int r, int tid, int nt) {
+    const uint32_t p = mod_p(r);
+    const uint32_t* ipsi = T->ipsi[r];
+    const uint32_t* ipsiq = T->ipsiq[r];
+#pragma unroll 1
+    for (int s = LOGN - 1; s >= 0; --s) {
+        const int m = 1 << s;
+        const int lt = LOGN - 1 - s;
+        const int t = 1 << lt;
+        for (int b = tid; b < N / 2; b += nt) {
+            const int i = b >> lt;
+            const int j = (i << (lt + 1)) + (b & (t - 1));
+            const uint32_t U = a[j];
+            const uint32_t V = a[j + t];
+            a[j] = add_mod(U, V, p);
+            a[j + t] = mul_shoup(sub_mod(U, V, p), ipsi[m + i], ipsiq[m + i], p);
+        }
+        __syncthreads();
+    }
+    const uint32_t ni = T->ninv[r], niq = T->ninvq[r];
+    for (int j = tid; j < N; j += nt) a[j] = mul_shoup(a[j], ni, niq, p);
+    __syncthreads();
+}
+
+// `nd` interleaved cyclic DFTs of length n = 2^logn (element j of DFT d at
+// a[j*nd + d]) with root omega = psi_1^(-2N/n), modulus p1, decimation in
+// frequency: natural order in, bit-reversed order out.
+__device__ void dft_dif_p1(uint32_t* a, int logn, int nd, const DevTables* __restrict__ T, int tid, int nt) {
+    const uint32_t p = P1;
+    const int n = 1 << logn;
+#pragma unroll 1
+    for (int lh = logn - 1; lh >= 0; --lh) {
+        const int half = 1 << lh;
+        const int tw_step = M2N >> (lh + 1);  // 2N / len
+        const int work = nd * (n / 2);
+        for (int w = tid; w < work; w += nt) {
+            const int d = w % nd;
+            const int b = w / nd;
+            const int blk = b >> lh;
+            const int j = b & (half - 1);
+            const int i0 = (blk << (lh + 1)) + j;
+            const uint32_t u = a[i0 * nd + d];
+            const uint32_t v = a[(i0 + half) * nd + d];
+            const int e = (M2N - ((tw_step * j) & (M2N - 1))) & (M2N - 1);
+            a[i0 * nd + d] = add_mod(u, v, p);
+            a[(i0 + half) * nd + d] = mul_shoup(sub_mod(u, v, p), T->psipow[0][e], T->psipowq[0][e], p);
+        }
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ int brv_bits(int v, int bits) {
+    return bits == 0 ? 0 : (int)(__brev((unsigned)v) >> (32 - bits));
+}
+
+// ---------------------------------------------------------------------------
+// K5: database encoding (base_pir.rs:662-772), modulus p1 for steps 1-5
+// ---------------------------------------------------------------------------
+
+// steps 1-2: column j, lane -> NTT of the entry's coefficient vector.
+// entries [j][8][N] u16; M [lane_idx][j][slot]
+__global__ __launch_bounds__(256) void k5_col_ntt_kernel(const uint16_t* __restrict__ entries, uint32_t* __restrict__ M,
+                                                         int l, const int* __restrict__ ncols_nonzero,
+                                                         const int* __restrict__ lanes, const DevTables* __restrict__ T) {
+    __shared__ uint32_t a[N];
+    const int j = blockIdx.x, li = blockIdx.y, lane = lanes[li];
+    uint32_t* out = M + ((size_t)li * l + j) * N;
+    if (j >= ncols_nonzero[li]) {
+        for (int s = threadIdx.x; s < N; s += blockDim.x) out[s] = 0;
+        return;
+    }
+    const uint16_t* src = entries + ((size_t)j * LANES + lane) * N;
+    for (int s = threadIdx.x; s < N; s += blockDim.x) a[s] = src[s];
+    __syncthreads();
+    ntt_fwd(a, T, 0, threadIdx.x, blockDim.x);
+    for (int s = threadIdx.x; s < N; s += blockDim.x) out[s] = a[s];
+}
+
+// step 3 (coset_ntt, base_pir.rs:397-444) for a tile of TS slots of one lane,
+// in place in M. Row s: twist by psi^(-coset j), DFT over j, then
+// new[i] = evaluation at psi^(-z 5^i) (main half) / psi^(z 5^(i-N/2)) (conj
+// half), z = e_s * k mod 2N.
+constexpr int K5_TS = 8;
+__global__ __launch_bounds__(256) void k5_rows_kernel(uint32_t* __restrict__ M, int l, int logl, int kq, int conj,
+                                                      const DevTables* __restrict__ T) {
+    extern __shared__ uint32_t tile[];  // [l][TS]
+    const int s0 = blockIdx.x * K5_TS, li = blockIdx.y;
+    uint32_t* base = M + (size_t)li * l * N;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int step = M2N / l;
+    for (int idx = tid; idx < l * K5_TS; idx += nt) {
+        const int j = idx / K5_TS, d = idx % K5_TS;
+        const int s = s0 + d;
+        const int z = (T->slot_exp[s] * kq) & (M2N - 1);
+        const int coset = z % step;
+        const int e = (M2N - ((coset * j) & (M2N - 1))) & (M2N - 1);
+        tile[idx] = mul_shoup(base[(size_t)j * N + s], T->psipow[0][e], T->psipowq[0][e], P1);
+    }
+    __syncthreads();
+    dft_dif_p1(tile, logl, K5_TS, T, tid, nt);
+    for (int idx = tid; idx < l * K5_TS; idx += nt) {
+        const int i = idx / K5_TS, d = idx % K5_TS;
+        const int s = s0 + d;
+        const int z = (T->slot_exp[s] * kq) & (M2N - 1);
+        const int coset = z % step;
+        int y;
+        if (conj && i >= N / 2) y = (z * T->pow5[i - N / 2]) & (M2N - 1);
+        else y = ((M2N - z) * T->pow5[i]) & (M2N - 1);
+        const int u = (((M2N - y) & (M2N - 1)) - coset) / step;  // natural DFT index
+        base[(size_t)i * N + s] = tile[brv_bits(u, logl) * K5_TS + d];
+    }
+}
+
+// steps 4-6: final column c takes column j = rev(c) (step 5), gathers its
+// slots by sigma_{5^-j} (step 4), then inverse NTT, centered lift, times
+// l^-1 mod t, centered lift, embedding into p1 and p2 and forward NTTs.
+// te_tmp [st][lane_idx][row][r][slot]
+__global__ __launch_bounds__(256) void k5_cols_final_kernel(const uint32_t* __restrict__ M, uint32_t* __restrict__ te_tmp,
+                                                            int l, int L, int conj, int nlanes, int scale,
+                                                            const DevTables* __restrict__ T) {
+    __shared__ uint32_t a[2][N];
+    const int c = blockIdx.x, li = blockIdx.y;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    int j;
+    if (conj) j = c < N / 2 ? N / 2 - 1 - c : N / 2 + (N - 1 - c);
+    else j = l - 1 - c;
+    const uint32_t* col = M + ((size_t)li * l + j) * N;
+    const int au = T->inv5pow[j & (HALFN - 1)];
+    for (int s = tid; s < N; s += nt) a[0][s] = col[T->exp_slot[(au * T->slot_exp[s]) & (M2N - 1)]];
+    __syncthreads();
+    ntt_inv(a[0], T, 0, tid, nt);
+    for (int s = tid; s < N; s += nt) {
+        int32_t x = (int32_t)a[0][s];
+        if (x > (int32_t)(P1 / 2)) x -= (int32_t)P1;
+        int32_t xt = x % (int32_t)T_PLAIN;
+        if (xt < 0) xt += T_PLAIN;
+        int32_t y = (int32_t)(((int64_t)xt * scale) % T_PLAIN);
+        if (y > (int32_t)(T_PLAIN / 2)) y -= (int32_t)T_PLAIN;
+        a[0][s] = y < 0 ? (uint32_t)(y + (int32_t)P1) : (uint32_t)y;
+        a[1][s] = y < 0 ? (uint32_t)(y + (int32_t)P2) : (uint32_t)y;
+    }
+    __syncthreads();
+    const int half = tid >= nt / 2;
+    ntt_fwd(a[half], T, half, tid - half * (nt / 2), nt / 2);
+    const int st = c < L ? 0 : 1;
+    const int row = c < L ? c : c - L;
+    uint32_t* dst = te_tmp + (((size_t)(st * nlanes + li) * L + row) * 2) * N;
+    for (int s = tid; s < N; s += nt) {
+        dst[s] = a[0][s];
+        dst[N + s] = a[1][s];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K4/K6: the sequential a-chain (preprocess_loop, base_pir.rs:819-862, with
+// partial_gal :496-594). One block per (lane, stream). SEC additionally runs
+// the secondary b-recurrence c <- sigma_5(c) + D0 gkb0 + D1 gkb1 + TE qb
+// (main_pir_loop in gathered form) in the same step, so the digits never go
+// to memory. PRE stores the digits as the IPGK rows.
+// ---------------------------------------------------------------------------
+template <bool SEC>
+__global__ __launch_bounds__(512) void chain_kernel(const uint32_t* __restrict__ te_tmp, uint32_t* __restrict__ ip_tmp,
+                                                    uint32_t* __restrict__ a_out, uint32_t* __restrict__ c_out,
+                                                    const uint32_t* __restrict__ qs, const int* __restrict__ lane_slot,
+                                                    int L, int nlanes, int qa_base, const DevTables* __restrict__ T) {
+    __shared__ uint32_t sa[2][N];
+    __shared__ uint32_t bufA[2][N];
+    __shared__ uint32_t bufB[2][N];
+    __shared__ uint32_t sd[2][2][N];
+    const int li = blockIdx.x, st = blockIdx.y;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    uint32_t(*sw)[N] = bufA;
+    uint32_t(*sc)[N] = bufB;
+    for (int s = tid; s < N; s += nt) {
+        sa[0][s] = 0; sa[1][s] = 0;
+        sc[0][s] = 0; sc[1][s] = 0;
+    }
+    __syncthreads();
+    const uint32_t* te_base = te_tmp + ((size_t)(st * nlanes + li) * L) * 2 * N;
+    const uint32_t* qv = nullptr;
+    if (SEC) qv = qs + (size_t)lane_slot[li] * 6 * 2 * N;  // [v][r][slot]
+    const uint32_t* qa = T->qa[qa_base + st][0];
+#pragma unroll 1
+    for (int i = 0; i < L; ++i) {
+        for (int s = tid; s < N; s += nt) {
+            const int g = T->sigma5[s];
+            sw[0][s] = sa[0][g];
+            sw[1][s] = sa[1][g];
+        }
+        __syncthreads();
+        {
+            const int half = tid >= nt / 2;
+            ntt_inv(sw[half], T, half, tid - half * (nt / 2), nt / 2);
+        }
+        for (int s = tid; s < N; s += nt) {
+            const int64_t x = center_q(crt(sw[0][s], sw[1][s]));
+            const int64_t hi = (x + (1ll << (LOG2_B - 1))) >> LOG2_B;
+            const int64_t lo = x - (hi << LOG2_B);
+            const int64_t hi2 = (hi + (1ll << (LOG2_B - 1))) >> LOG2_B;
+            const int64_t lo2 = hi - (hi2 << LOG2_B);
+            sd[0][0][s] = from_i64(lo, P1, 26);
+            sd[0][1][s] = from_i64(lo, P2, 25);
+            sd[1][0][s] = from_i64(lo2, P1, 26);
+            sd[1][1][s] = from_i64(lo2, P2, 25);
+        }
+        __syncthreads();
+        {
+            const int q4 = tid / (nt / 4);
+            ntt_fwd(sd[q4 >> 1][q4 & 1], T, q4 & 1, tid - q4 * (nt / 4), nt / 4);
+        }
+        const uint32_t* te_row = te_base + (size_t)i * 2 * N;
+        for (int s = tid; s < N; s += nt) {
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                const uint32_t p = mod_p(r);
+                const int K = mod_k(r);
+                const uint32_t d0 = sd[0][r][s], d1 = sd[1][r][s];
+                const uint32_t te = te_row[r * N + s];
+                const uint64_t av = (uint64_t)d0 * T->gka[0][r][s] + (uint64_t)d1 * T->gka[1][r][s] +
+                                    (uint64_t)te * qa[r * N + s];
+                sa[r][s] = red64(av, p, K);
+                if (SEC) {
+                    const uint64_t cv = (uint64_t)sc[r][T->sigma5[s]] + (uint64_t)d0 * qv[(0 * 2 + r) * N + s] +
+                                        (uint64_t)d1 * qv[(1 * 2 + r) * N + s] +
+                                        (uint64_t)te * qv[((4 + st) * 2 + r) * N + s];
+                    sw[r][s] = red64(cv, p, K);
+                } else {
+                    uint32_t* dst = ip_tmp + (((size_t)(st * nlanes + li) * L + i) * 4) * N;
+                    dst[(0 * 2 + r) * N + s] = d0;
+                    dst[(1 * 2 + r) * N + s] = d1;
+                }
+            }
+        }
+        __syncthreads();
+        if (SEC) {
+            uint32_t(*t)[N] = sw;
+            sw = sc;
+            sc = t;
+        }
+    }
+    uint32_t* ao = a_out + ((size_t)(st * nlanes + li) * 2) * N;
+    for (int s = tid; s < N; s += nt) {
+        ao[s] = sa[0][s];
+        ao[N + s] = sa[1][s];
+    }
+    if (SEC) {
+        uint32_t* co = c_out + ((size_t)(st * nlanes + li) * 2) * N;
+        for (int s = tid; s < N; s += nt) {
+            co[s] = sc[0][s];
+            co[N + s] = sc[1][s];
+        }
+    }
+}
+
+// lane-major temporaries -> the streaming slab layout (rotated Galois order,
+// 8 lanes interleaved). One block per (row, array, r*nst + st).
+__global__ __launch_bounds__(256) void relayout_kernel(const uint32_t* __restrict__ te_tmp, const uint32_t* __restrict__ ip_tmp,
+                                                       uint32_t* __restrict__ slab, int L, int nst,
+                                                       const DevTables* __restrict__ T) {
+    __shared__ uint32_t rows[LANES][N];
+    const int i = blockIdx.x, arr = blockIdx.y;
+    const int r = blockIdx.z / nst, st = blockIdx.z % nst;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int idx = tid; idx < LANES * N; idx += nt) {
+        const int lane = idx / N, s = idx % N;
+        uint32_t v;
+        if (arr == 0) v = te_tmp[(((size_t)(st * LANES + lane) * L + i) * 2 + r) * N + s];
+        else v = ip_tmp[(((size_t)(st * LANES + lane) * L + i) * 4 + (arr - 1) * 2 + r) * N + s];
+        rows[lane][s] = v;
+    }
+    __syncthreads();
+    uint32_t* out = slab + ((((size_t)(r * nst + st) * L + i) * 3 + arr) * N) * LANES;
+    for (int o = tid; o < N * LANES; o += nt) {
+        const int g = o >> 3, lane = o & 7;
+        const int hb = g & ~(HALFN - 1);
+        const int rot = hb + (((g & (HALFN - 1)) + L - 1 - i) & (HALFN - 1));
+        out[o] = rows[lane][T->gpos_slot[rot]];
+    }
+}
+
+// sum of the chains' a (and c) over the streams -> inverse NTT -> CRT ->
+// coefficients mod q and their 17-bit (a) / 12-bit (b) mod switch.
+// One block per lane index. a_out/c_out [st][li][r][slot].
+__global__ __launch_bounds__(256) void chain_out_kernel(const uint32_t* __restrict__ a_out, const uint32_t* __restrict__ c_out,
+                                                        int nst, int nlanes, uint64_t* __restrict__ a_coef,
+                                                        uint64_t* __restrict__ b_coef, uint32_t* __restrict__ a17,
+                                                        uint32_t* __restrict__ b12, const DevTables* __restrict__ T) {
+    __shared__ uint32_t w[4][N];
+    const int li = blockIdx.x;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int s = tid; s < N; s += nt) {
+        for (int r = 0; r < 2; ++r) {
+            const uint32_t p = mod_p(r);
+            uint32_t va = 0, vc = 0;
+            for (int st = 0; st < nst; ++st) {
+                va = add_mod(va, a_out[((size_t)(st * nlanes + li) * 2 + r) * N + s], p);
+                if (c_out) vc = add_mod(vc, c_out[((size_t)(st * nlanes + li) * 2 + r) * N + s], p);
+            }
+            w[r][s] = va;
+            w[2 + r][s] = vc;
+        }
+    }
+    __syncthreads();
+    {
+        const int q4 = tid / (nt / 4);
+        ntt_inv(w[q4], T, q4 & 1, tid - q4 * (nt / 4), nt / 4);
+    }
+    for (int s = tid; s < N; s += nt) {
+        const uint64_t a = crt(w[0][s], w[1][s]);
+        if (a_coef) a_coef[(size_t)li * N + s] = a;
+        if (a17) a17[(size_t)li * N + s] = mod_switch_enc(a, 17);
+        if (c_out) {
+            const uint64_t b = crt(w[2][s], w[3][s]);
+            if (b_coef) b_coef[(size_t)li * N + s] = b;
+            if (b12) b12[(size_t)li * N + s] = mod_switch_enc(b, 12);
+        }
+    }
+}
+
+// query ring elements (coefficients mod q) -> NTT vectors mod p1 and p2.
+// qcoef [nq][6][N] (gk0, gk1, primary, primary conj, secondary, secondary conj)
+// qs [nq][6][2][slot]; qp [nq][4][2][gpos]
+__global__ __launch_bounds__(256) void qprep_kernel(const uint64_t* __restrict__ qcoef, uint32_t* __restrict__ qs,
+                                                    uint32_t* __restrict__ qp, const DevTables* __restrict__ T) {
+    __shared__ uint32_t a[N];
+    const int v = blockIdx.x, r = blockIdx.y, q = blockIdx.z;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const uint32_t p = mod_p(r);
+    const int K = mod_k(r);
+    const uint64_t* src = qcoef + ((size_t)q * 6 + v) * N;
+    for (int s = tid; s < N; s += nt) a[s] = red64(src[s], p, K);
+    __syncthreads();
+    ntt_fwd(a, T, r, tid, nt);
+    uint32_t* o = qs + (((size_t)q * 6 + v) * 2 + r) * N;
+    for (int s = tid; s < N; s += nt) o[s] = a[s];
+    if (v < 4) {
+        uint32_t* g = qp + (((size_t)q * 4 + v) * 2 + r) * N;
+        for (int s = tid; s < N; s += nt) g[s] = a[T->gpos_slot[s]];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K1: the primary inner product (main_pir_loop, base_pir.rs:1085-1247) in
+// closed form  c[g] = sum_i sum_arr DBrot_arr[i][g] * Q_arr[rot_i(g)].
+// Block: 128 Galois positions x 2 lane halves, a chunk of rows of one
+// (shard, r, stream). Partial sums are reduced mod p and added atomically
+// into acc[shard][r][gpos][lane] (u64, at most 2 * 1024 partials < 2^37).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k1_kernel(const uint32_t* const* __restrict__ slabs, const uint32_t* __restrict__ qp,
+                                                 unsigned long long* __restrict__ acc, int L, int nst, int rows_per_block) {
+    __shared__ uint32_t qv[3][N];
+    const int z = blockIdx.z;
+    const int st = z % nst;
+    const int r = (z / nst) & 1;
+    const int sh = z / (2 * nst);
+    const int tid = threadIdx.x;
+    for (int s = tid; s < N; s += 256) {
+        qv[0][s] = qp[((2 + st) * 2 + r) * N + s];
+        qv[1][s] = qp[(0 * 2 + r) * N + s];
+        qv[2][s] = qp[(1 * 2 + r) * N + s];
+    }
+    __syncthreads();
+    const int g = blockIdx.x * 128 + (tid >> 1);
+    const int half = tid & 1;
+    const int hb = g & ~(HALFN - 1);
+    const int ga = g & (HALFN - 1);
+    const uint32_t* base = slabs[sh] + ((size_t)(r * nst + st) * L) * 3 * N * LANES;
+    const int i0 = blockIdx.y * rows_per_block;
+    const int i1 = min(L, i0 + rows_per_block);
+    unsigned long long a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+#pragma unroll 4
+    for (int i = i0; i < i1; ++i) {
+        const int src = hb + ((ga + L - 1 - i) & (HALFN - 1));
+        const uint32_t qt = qv[0][src], g0 = qv[1][src], g1 = qv[2][src];
+        const u32x4* rowp = reinterpret_cast<const u32x4*>(base + (size_t)i * 3 * N * LANES);
+        const u32x4 te = __builtin_nontemporal_load(rowp + g * 2 + half);
+        const u32x4 x0 = __builtin_nontemporal_load(rowp + (N + g) * 2 + half);
+        const u32x4 x1 = __builtin_nontemporal_load(rowp + (2 * N + g) * 2 + half);
+        a0 += (unsigned long long)te.x * qt + (unsigned long long)x0.x * g0 + (unsigned long long)x1.x * g1;
+        a1 += (unsigned long long)te.y * qt + (unsigned long long)x0.y * g0 + (unsigned long long)x1.y * g1;
+        a2 += (unsigned long long)te.z * qt + (unsigned long long)x0.z * g0 + (unsigned long long)x1.z * g1;
+        a3 += (unsigned long long)te.w * qt + (unsigned long long)x0.w * g0 + (unsigned long long)x1.w * g1;
+    }
+    const uint32_t p = mod_p(r);
+    const int K = mod_k(r);
+    unsigned long long* o = acc + (((size_t)sh * 2 + r) * N + g) * LANES + half * 4;
+    atomicAdd(o + 0, (unsigned long long)red64(a0, p, K));
+    atomicAdd(o + 1, (unsigned long long)red64(a1, p, K));
+    atomicAdd(o + 2, (unsigned long long)red64(a2, p, K));
+    atomicAdd(o + 3, (unsigned long long)red64(a3, p, K));
+}
+
+// Per-lane query vectors (perform_pir_split): same as k1_kernel but Q is read
+// per lane from global memory: qpl [lane][v][r][gpos].
+__global__ __launch_bounds__(256) void k1_split_kernel(const uint32_t* const* __restrict__ slabs, const uint32_t* __restrict__ qpl,
+                                                       unsigned long long* __restrict__ acc, int L, int nst, int rows_per_block) {
+    const int z = blockIdx.z;
+    const int st = z % nst;
+    const int r = (z / nst) & 1;
+    const int sh = z / (2 * nst);
+    const int tid = threadIdx.x;
+    const int g = blockIdx.x * 128 + (tid >> 1);
+    const int half = tid & 1;
+    const int hb = g & ~(HALFN - 1);
+    const int ga = g & (HALFN - 1);
+    const uint32_t* base = slabs[sh] + ((size_t)(r * nst + st) * L) * 3 * N * LANES;
+    const int i0 = blockIdx.y * rows_per_block;
+    const int i1 = min(L, i0 + rows_per_block);
+    unsigned long long a[4] = {0, 0, 0, 0};
+    for (int i = i0; i < i1; ++i) {
+        const int src = hb + ((ga + L - 1 - i) & (HALFN - 1));
+        const uint32_t* rowp = base + (size_t)i * 3 * N * LANES;
+        for (int k = 0; k < 4; ++k) {
+            const int lane = half * 4 + k;
+            const uint32_t* ql = qpl + (size_t)lane * 4 * 2 * N;
+            a[k] += (unsigned long long)rowp[g * LANES + lane] * ql[((2 + st) * 2 + r) * N + src] +
+                    (unsigned long long)rowp[(N + g) * LANES + lane] * ql[(0 * 2 + r) * N + src] +
+                    (unsigned long long)rowp[(2 * N + g) * LANES + lane] * ql[(1 * 2 + r) * N + src];
+        }
+    }
+    const uint32_t p = mod_p(r);
+    const int K = mod_k(r);
+    unsigned long long* o = acc + (((size_t)sh * 2 + r) * N + g) * LANES + half * 4;
+    for (int k = 0; k < 4; ++k) atomicAdd(o + k, (unsigned long long)red64(a[k], p, K));
+}
+
+// primary epilogue per (shard, lane): reduce, Galois order -> slot order,
+// inverse NTT, CRT (from_simd_element, base_pir.rs:905-935); then either the
+// b coefficients (perform_pir) or, with the shard's 17-bit a-part, the three
+// base-1025 digits of a*2^12 + b written as secondary entries
+// (double_pir.rs:157-177, 70-81): column shard*8 + lane, lane 3*slot + part.
+__global__ __launch_bounds__(256) void epi_primary_kernel(const unsigned long long* __restrict__ acc,
+                                                          const uint32_t* const* __restrict__ a17s, uint16_t* __restrict__ se,
+                                                          uint64_t* __restrict__ bcoef, int slot,
+                                                          const DevTables* __restrict__ T) {
+    __shared__ uint32_t w[2][N];
+    const int sh = blockIdx.x, lane = blockIdx.y;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int s = tid; s < N; s += nt) {
+        const int g = T->slot_gpos[s];
+        w[0][s] = red64(acc[(((size_t)sh * 2 + 0) * N + g) * LANES + lane], P1, 26);
+        w[1][s] = red64(acc[(((size_t)sh * 2 + 1) * N + g) * LANES + lane], P2, 25);
+    }
+    __syncthreads();
+    {
+        const int half = tid >= nt / 2;
+        ntt_inv(w[half], T, half, tid - half * (nt / 2), nt / 2);
+    }
+    for (int s = tid; s < N; s += nt) {
+        const uint64_t b = crt(w[0][s], w[1][s]);
+        if (bcoef) bcoef[((size_t)sh * LANES + lane) * N + s] = b;
+        if (se) {
+            const uint32_t b12 = mod_switch_enc(b, 12);
+            const uint32_t a17 = a17s[sh][(size_t)lane * N + s];
+            uint32_t v = (a17 << 12) + b12;
+            const int col = sh * LANES + lane;
+            uint16_t* dst = se + ((size_t)col * LANES + 3 * slot) * N + s;
+            dst[0] = (uint16_t)(v % T_PLAIN);
+            v /= T_PLAIN;
+            dst[N] = (uint16_t)(v % T_PLAIN);
+            dst[2 * N] = (uint16_t)(v / T_PLAIN);
+        }
+    }
+}
+
+__global__ void fill_u64_kernel(unsigned long long* p, size_t n, unsigned long long v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void fill_u32_kernel(uint32_t* p, size_t n, uint32_t v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+void launch_fill_u64(unsigned long long* p, size_t n, unsigned long long v, hipStream_t s) {
+    if (!n) return;
+    int blocks = (int)std::min<size_t>(4096, (n + 255) / 256);
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(blocks), dim3(256), 0, s, p, n, v);
+}
+void launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s) {
+    if (!n) return;
+    int blocks = (int)std::min<size_t>(4096, (n + 255) / 256);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks), dim3(256), 0, s, p, n, v);
+}
+
+void launch_k5(const uint16_t* entries, uint32_t* M, uint32_t* te_tmp, int l, const int* ncols_nonzero, const int* lanes,
+               int nlanes, const DevTables* T, hipStream_t s) {
+    const int L = l < N ? l : N / 2;
+    const int conj = l == N;
+    const int kq = N / 2 / L;
+    int logl = 0;
+    while ((1 << logl) < l) ++logl;
+    int scale = 1;
+    for (int x = 1; x < (int)T_PLAIN; ++x)
+        if ((x * l) % (int)T_PLAIN == 1) { scale = x; break; }
+    hipLaunchKernelGGL(k5_col_ntt_kernel, dim3(l, nlanes), dim3(256), 0, s, entries, M, l, ncols_nonzero, lanes, T);
+    hipLaunchKernelGGL(k5_rows_kernel, dim3(N / K5_TS, nlanes), dim3(256), (size_t)l * K5_TS * 4, s, M, l, logl, kq, conj, T);
+    hipLaunchKernelGGL(k5_cols_final_kernel, dim3(l, nlanes), dim3(256), 0, s, M, te_tmp, l, L, conj, nlanes, scale, T);
+}
+
+void launch_chain_pre(const uint32_t* te_tmp, uint32_t* ip_tmp, uint32_t* a_out, int L, int nst, int qa_base,
+                      const DevTables* T, hipStream_t s) {
+    hipLaunchKernelGGL(chain_kernel<false>, dim3(LANES, nst), dim3(512), 0, s, te_tmp, ip_tmp, a_out,
+                       (uint32_t*)nullptr, (const uint32_t*)nullptr, (const int*)nullptr, L, LANES, qa_base, T);
+}
+
+void launch_chain_sec(const uint32_t* te_tmp, uint32_t* a_out, uint32_t* c_out, const uint32_t* qs, const int* lane_slot,
+                      int L, int nst, int nlanes, const DevTables* T, hipStream_t s) {
+    hipLaunchKernelGGL(chain_kernel<true>, dim3(nlanes, nst), dim3(512), 0, s, te_tmp, (uint32_t*)nullptr, a_out, c_out,
+                       qs, lane_slot, L, nlanes, 2, T);
+}
+
+void launch_relayout(const uint32_t* te_tmp, const uint32_t* ip_tmp, uint32_t* slab, int L, int nst, const DevTables* T,
+                     hipStream_t s) {
+    hipLaunchKernelGGL(relayout_kernel, dim3(L, 3, 2 * nst), dim3(256), 0, s, te_tmp, ip_tmp, slab, L, nst, T);
+}
+
+void launch_chain_out(const uint32_t* a_out, const uint32_t* c_out, int nst, int nlanes, uint64_t* a_coef, uint64_t* b_coef,
+                      uint32_t* a17, uint32_t* b12, const DevTables* T, hipStream_t s) {
+    hipLaunchKernelGGL(chain_out_kernel, dim3(nlanes), dim3(256), 0, s, a_out, c_out, nst, nlanes, a_coef, b_coef, a17, b12, T);
+}
+
+void launch_qprep(const uint64_t* qcoef, uint32_t* qs, uint32_t* qp, int nq, const DevTables* T, hipStream_t s) {
+    hipLaunchKernelGGL(qprep_kernel, dim3(6, 2, nq), dim3(256), 0, s, qcoef, qs, qp, T);
+}
+
+void launch_k1(const uint32_t* const* slabs, int nshards, const uint32_t* qp, unsigned long long* acc, int L, int nst,
+               int rows_per_block, hipStream_t s) {
+    const int chunks = (L + rows_per_block - 1) / rows_per_block;
+    hipLaunchKernelGGL(k1_kernel, dim3(N / 128, chunks, nshards * 2 * nst), dim3(256), 0, s, slabs, qp, acc, L, nst,
+                       rows_per_block);
+}
+
+void launch_k1_split(const uint32_t* const* slabs, int nshards, const uint32_t* qpl, unsigned long long* acc, int L, int nst,
+                     int rows_per_block, hipStream_t s) {
+    const int chunks = (L + rows_per_block - 1) / rows_per_block;
+    hipLaunchKernelGGL(k1_split_kernel, dim3(N / 128, chunks, nshards * 2 * nst), dim3(256), 0, s, slabs, qpl, acc, L, nst,
+                       rows_per_block);
+}
+
+void launch_epi_primary(const unsigned long long* acc, int nshards, const uint32_t* const* a17s, uint16_t* se,
+                        uint64_t* bcoef, int slot, const DevTables* T, hipStream_t s) {
+    hipLaunchKernelGGL(epi_primary_kernel, dim3(nshards, LANES), dim3(256), 0, s, acc, a17s, se, bcoef, slot, T);
+}
+
+// ---------------------------------------------------------------------------
+// wire codec on the device (pir_wrapper.rs:84-158 bit planes, bfv.rs:51-69)
+// ---------------------------------------------------------------------------
+// value j of a `bits`-bit stream of N values condensed at d
+__device__ uint64_t uncondense_one(const uint8_t* __restrict__ d, int bits, int j) {
+    uint64_t v = 0;
+    int off = 0;
+    for (int i = 0; i < bits / 8; ++i) {
+        v |= (uint64_t)d[off + j] << (8 * i);
+        off += N;
+    }
+    if (bits % 8 >= 4) {
+        const uint8_t b = d[off + j / 2];
+        v |= (uint64_t)((j & 1) ? (b & 0xF) : (b >> 4)) << ((bits / 8) * 8);
+        off += N / 2;
+    }
+    if (bits % 4 >= 2) {
+        const uint8_t b = d[off + j / 4];
+        v |= (uint64_t)((b >> (6 - 2 * (j & 3))) & 3) << ((bits / 4) * 4);
+        off += N / 4;
+    }
+    if (bits % 2 == 1) {
+        const uint8_t b = d[off + j / 8];
+        v |= (uint64_t)((b >> (7 - (j & 7))) & 1) << ((bits / 2) * 2);
+    }
+    return v;
+}
+
+// byte k of the condensed stream of N `bits`-bit values x[]
+__device__ uint8_t condense_byte(const uint32_t* __restrict__ x, int bits, int k) {
+    const int nbp = bits / 8;
+    if (k < nbp * N) return (uint8_t)(x[k % N] >> (8 * (k / N)));
+    k -= nbp * N;
+    if (bits % 8 >= 4) {
+        if (k < N / 2) {
+            const int sh = nbp * 8;
+            return (uint8_t)((((x[2 * k] >> sh) & 0xF) << 4) | ((x[2 * k + 1] >> sh) & 0xF));
+        }
+        k -= N / 2;
+    }
+    if (bits % 4 >= 2) {
+        if (k < N / 4) {
+            const int sh = (bits / 4) * 4;
+            uint8_t v = 0;
+            for (int c = 0; c < 4; ++c) v |= (uint8_t)(((x[4 * k + c] >> sh) & 3) << (6 - 2 * c));
+            return v;
+        }
+        k -= N / 4;
+    }
+    const int sh = (bits / 2) * 2;
+    uint8_t v = 0;
+    for (int c = 0; c < 8; ++c) v |= (uint8_t)(((x[8 * k + c] >> sh) & 1) << (7 - c));
+    return v;
+}
+
+// query bytes -> ring elements [6][N] (mod_switch_decode: floor(x q / 2^B)).
+// offs[v] < 0 marks an absent element (written as zero).
+struct QueryLayout {
+    int offs[6];
+    int bits[6];
+};
+__global__ __launch_bounds__(256) void decode_query_kernel(const uint8_t* __restrict__ qbytes, QueryLayout lay,
+                                                           uint64_t* __restrict__ qcoef) {
+    const int v = blockIdx.y;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= N) return;
+    uint64_t out = 0;
+    if (lay.offs[v] >= 0) {
+        const uint64_t x = uncondense_one(qbytes + lay.offs[v], lay.bits[v], j);
+        out = (uint64_t)(((unsigned __int128)x * Q) >> lay.bits[v]);
+    }
+    qcoef[(size_t)v * N + j] = out;
+}
+
+// reply bytes: for part p: condense<17>(a17[lane0 + p]) || condense<12>(b12[lane0 + p])
+__global__ __launch_bounds__(256) void encode_reply_kernel(const uint32_t* __restrict__ a17, const uint32_t* __restrict__ b12,
+                                                           int lane0, uint8_t* __restrict__ reply) {
+    constexpr int AB = N * 17 / 8, BB = N * 12 / 8;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= 3 * (AB + BB)) return;
+    const int p = k / (AB + BB);
+    const int o = k % (AB + BB);
+    const int li = lane0 + p;
+    reply[k] = o < AB ? condense_byte(a17 + (size_t)li * N, 17, o) : condense_byte(b12 + (size_t)li * N, 12, o - AB);
+}
+
+void launch_decode_query(const uint8_t* qbytes, const int offs[6], const int bits[6], uint64_t* qcoef, hipStream_t s) {
+    QueryLayout lay;
+    for (int i = 0; i < 6; ++i) {
+        lay.offs[i] = offs[i];
+        lay.bits[i] = bits[i];
+    }
+    hipLaunchKernelGGL(decode_query_kernel, dim3(N / 256, 6), dim3(256), 0, s, qbytes, lay, qcoef);
+}
+
+void launch_encode_reply(const uint32_t* a17, const uint32_t* b12, int lane0, uint8_t* reply, hipStream_t s) {
+    const int total = 3 * (N * 17 / 8 + N * 12 / 8);
+    hipLaunchKernelGGL(encode_reply_kernel, dim3((total + 255) / 256), dim3(256), 0, s, a17, b12, lane0, reply);
+}
+
+}  // namespace dn

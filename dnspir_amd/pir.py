@@ -1,0 +1,230 @@
+"""Host-side mirror of the `dnspir-pir` crate API over the HIP engine.
+
+Names and argument meanings follow crates/pir/src/{base_pir,double_pir,
+pir_wrapper}.rs: `PIRDatabase` (create / set_db / get_a / perform_pir /
+perform_pir_split), `perform_batched_double_pir`, `process_queries`,
+`prepare_query` / `process_reply`, `get_database_shape`, `num_databases`.
+Errors the reference reports by panicking raise `DnspirError`.
+
+Device memory for shards is caller-owned, as in the reference (the crate
+borrows a slab, base_pir.rs:280-300); here a torch uint8 tensor on the GPU
+holds it.
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import LANES, N, DnspirError, Timing, check, lib
+
+_ptr = lambda a, t: a.ctypes.data_as(t)  # noqa: E731
+u8p = ctypes.POINTER(ctypes.c_uint8)
+u16p = ctypes.POINTER(ctypes.c_uint16)
+u64p = ctypes.POINTER(ctypes.c_uint64)
+
+_initialised = None
+
+
+def init(device=0):
+    """dnspir_init: builds the device tables (idempotent)."""
+    global _initialised
+    if _initialised != device:
+        import torch  # device memory and streams only
+        torch.cuda.init()
+        check(lib().dnspir_init(device), "dnspir_init")
+        _initialised = device
+    return True
+
+
+def required_bytes(group_order):
+    return lib().dnspir_required_bytes(group_order)
+
+
+class PIRDatabase:
+    """One preprocessed shard (base_pir.rs:225-1247) resident in HBM."""
+
+    def __init__(self, group_order=N, used_seeds=0, device=0, memory=None):
+        import torch
+        init(device)
+        self.group_order = group_order
+        nbytes = required_bytes(group_order)
+        if nbytes == 0:
+            raise DnspirError(-4, "PIRDatabase")
+        if memory is None:
+            memory = torch.empty(nbytes + 256, dtype=torch.uint8, device=f"cuda:{device}")
+        self.memory = memory  # keeps the slab alive
+        base = memory.data_ptr()
+        aligned = (base + 255) & ~255
+        h = ctypes.c_void_p()
+        check(lib().dnspir_shard_create(group_order, used_seeds, ctypes.c_void_p(aligned),
+                                        nbytes + 256 - (aligned - base), ctypes.byref(h)), "dnspir_shard_create")
+        self.h = h
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value:
+            lib().dnspir_shard_destroy(h)
+            self.h = None
+
+    @property
+    def requires_conj(self):
+        return self.group_order == N
+
+    def set_db(self, entries):
+        e = np.ascontiguousarray(entries, dtype=np.uint16)
+        if e.ndim != 2 or e.shape[1] != N:
+            raise DnspirError(-5, "set_db: entries must be [n][2048]")
+        check(lib().dnspir_shard_set_db(self.h, _ptr(e, u16p), e.shape[0]), "set_db")
+
+    def get_a(self):
+        out = np.zeros((LANES, N), np.uint64)
+        check(lib().dnspir_shard_get_a(self.h, _ptr(out, u64p)), "get_a")
+        return out
+
+    def perform_pir(self, qry_b, qry_conj_b, gk_b):
+        qb = np.ascontiguousarray(qry_b, np.uint64)
+        gk = np.ascontiguousarray(gk_b, np.uint64)
+        qc = _ptr(np.ascontiguousarray(qry_conj_b, np.uint64), u64p) if qry_conj_b is not None else None
+        out = np.zeros((LANES, N), np.uint64)
+        check(lib().dnspir_shard_perform_pir(self.h, _ptr(qb, u64p), qc, _ptr(gk, u64p), _ptr(out, u64p)), "perform_pir")
+        return out
+
+    def perform_pir_split(self, qry_b, qry_conj_b, gk_b):
+        qb = np.ascontiguousarray(qry_b, np.uint64)
+        gk = np.ascontiguousarray(gk_b, np.uint64)
+        qc = _ptr(np.ascontiguousarray(qry_conj_b, np.uint64), u64p) if qry_conj_b is not None else None
+        out = np.zeros((LANES, N), np.uint64)
+        check(lib().dnspir_shard_perform_pir_split(self.h, _ptr(qb, u64p), qc, _ptr(gk, u64p), _ptr(out, u64p)),
+              "perform_pir_split")
+        return out
+
+
+class Context:
+    """A query context: per-stream secondary scratch (double_pir.rs:43-56)."""
+
+    def __init__(self, stream=None):
+        init()
+        h = ctypes.c_void_p()
+        check(lib().dnspir_ctx_create(ctypes.c_void_p(stream) if stream else None, ctypes.byref(h)), "dnspir_ctx_create")
+        self.h = h
+
+    @property
+    def stream(self):
+        return lib().dnspir_ctx_stream(self.h)
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value:
+            lib().dnspir_ctx_destroy(h)
+            self.h = None
+
+
+def _fleet(shards):
+    if not shards:
+        return None, 0
+    arr = (ctypes.c_void_p * len(shards))(*[s.h.value for s in shards])
+    return arr, len(shards)
+
+
+def perform_batched_double_pir(fleet0, fleet1, q0, q1, ctx=None):
+    """double_pir.rs:126-228. q0/q1: [6][N] query blocks or None; returns
+    ([3][2][N] or None per slot, Timing)."""
+    init()
+    f0, l0 = _fleet(fleet0)
+    f1, l1 = _fleet(fleet1)
+    qs = [np.ascontiguousarray(q, np.uint64) if q is not None else None for q in (q0, q1)]
+    outs = [np.zeros((3, 2, N), np.uint64) if q is not None else None for q in qs]
+    tm = Timing()
+    check(lib().dnspir_double_pir(ctx.h if ctx else None, f0, l0, f1, l1,
+                                  *[_ptr(q, u64p) if q is not None else None for q in qs],
+                                  *[_ptr(o, u64p) if o is not None else None for o in outs],
+                                  ctypes.byref(tm)), "double_pir")
+    return outs, tm
+
+
+def process_queries(fleet0, fleet1, query0, query1=None, ctx=None):
+    """pir_wrapper.rs:349-409 (host bytes in, host bytes out)."""
+    init()
+    f0, l0 = _fleet(fleet0)
+    f1, l1 = _fleet(fleet1)
+    rb = lib().dnspir_reply_bytes()
+    bufs = [np.frombuffer(bytes(q), np.uint8).copy() if q is not None else None for q in (query0, query1)]
+    reps = [np.zeros(rb, np.uint8) if q is not None else None for q in (query0, query1)]
+    tm = Timing()
+    check(lib().dnspir_process_queries(ctx.h if ctx else None, f0, l0, f1, l1,
+                                       _ptr(bufs[0], u8p) if bufs[0] is not None else None, len(bufs[0]) if bufs[0] is not None else 0,
+                                       _ptr(bufs[1], u8p) if bufs[1] is not None else None, len(bufs[1]) if bufs[1] is not None else 0,
+                                       _ptr(reps[0], u8p) if reps[0] is not None else None,
+                                       _ptr(reps[1], u8p) if reps[1] is not None else None,
+                                       ctypes.byref(tm)), "process_queries")
+    return [bytes(r) if r is not None else None for r in reps], tm
+
+
+def process_queries_device(fleet0, fleet1, query0_dev, query1_dev, reply0_dev, reply1_dev, ctx=None):
+    """Same with query / reply bytes in device memory (torch uint8 tensors)."""
+    init()
+    f0, l0 = _fleet(fleet0)
+    f1, l1 = _fleet(fleet1)
+    tm = Timing()
+    dp = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None  # noqa: E731
+    check(lib().dnspir_process_queries_device(ctx.h if ctx else None, f0, l0, f1, l1,
+                                              dp(query0_dev), query0_dev.numel() if query0_dev is not None else 0,
+                                              dp(query1_dev), query1_dev.numel() if query1_dev is not None else 0,
+                                              dp(reply0_dev), dp(reply1_dev), ctypes.byref(tm)),
+          "process_queries_device")
+    return tm
+
+
+def get_database_shape(num_entries):
+    lp, ls = ctypes.c_int(), ctypes.c_int()
+    lib().dnspir_database_shape(num_entries, ctypes.byref(lp), ctypes.byref(ls))
+    return lp.value, ls.value
+
+
+def num_databases(num_entries):
+    return lib().dnspir_num_databases(num_entries)
+
+
+def choose_index_group_secondary(num_shards):
+    return lib().dnspir_choose_index_group_secondary(num_shards)
+
+
+def query_bytes(lp, ls):
+    return lib().dnspir_query_bytes(lp, ls)
+
+
+def reply_bytes():
+    return lib().dnspir_reply_bytes()
+
+
+class Rng:
+    """StdRng stand-in (ChaCha12) for the client side."""
+
+    def __init__(self, seed):
+        s = np.frombuffer(bytes(seed), np.uint8).copy()
+        assert s.size == 32
+        self.h = lib().dnspir_rng_new(_ptr(s, u8p))
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h:
+            lib().dnspir_rng_free(h)
+            self.h = None
+
+
+def prepare_query(rng, primary_idx, secondary_idx, l_primary, l_secondary):
+    """prepare_query_general (pir_wrapper.rs:276-317) -> (query bytes, sk seed)."""
+    out = np.zeros(query_bytes(l_primary, l_secondary) + 16, np.uint8)
+    n = ctypes.c_size_t()
+    sk = np.zeros(32, np.uint8)
+    check(lib().dnspir_prepare_query(rng.h, primary_idx, secondary_idx, l_primary, l_secondary, _ptr(out, u8p),
+                                     ctypes.byref(n), _ptr(sk, u8p)), "prepare_query")
+    return bytes(out[:n.value]), bytes(sk)
+
+
+def process_reply(reply, sk_seed):
+    """pir_wrapper.rs:420-440 -> N coefficients mod 1025."""
+    r = np.frombuffer(bytes(reply), np.uint8).copy()
+    s = np.frombuffer(bytes(sk_seed), np.uint8).copy()
+    out = np.zeros(N, np.uint16)
+    check(lib().dnspir_process_reply(_ptr(r, u8p), len(reply), _ptr(s, u8p), _ptr(out, u16p)), "process_reply")
+    return out

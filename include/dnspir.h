@@ -1,0 +1,136 @@
+/*
+ * dnspir.h -- C ABI of the MI355X PIR response engine.
+ *
+ * Mirrors the `dnspir-pir` crate API (crates/pir/src/lib.rs:35-41) so that the
+ * reference's Rust host (or any C/C++/Python caller) can bind it over FFI:
+ * each entry point names the reference item it replaces. All values are
+ * exact: ring elements cross the boundary as uint64_t[N] canonical
+ * coefficients in [0, q), q = 67104769 * 33550337, N = 2048; database entries
+ * as uint16_t[N] coefficients in [0, 1025). Every call returns an int status
+ * (0 = OK, negative = DNSPIR_E_*) where the reference panics.
+ */
+#ifndef DNSPIR_H
+#define DNSPIR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DNSPIR_N 2048
+#define DNSPIR_LANES 8
+
+#define DNSPIR_OK 0
+#define DNSPIR_E_SHORT_QUERY (-1) /* "not enough data", pir_wrapper.rs:124 */
+#define DNSPIR_E_CONJ (-2)        /* conj presence mismatch, base_pir.rs:965, double_pir.rs:160 */
+#define DNSPIR_E_COUNT (-3)       /* entry count != 8 * l, base_pir.rs:693 */
+#define DNSPIR_E_SHAPE (-4)       /* index group / fleet shape, base_pir.rs:150-154, double_pir.rs:148-152 */
+#define DNSPIR_E_ARG (-5)         /* null pointer or bad argument */
+#define DNSPIR_E_ALIGN (-6)       /* slab misaligned or too small, base_pir.rs:299-300 */
+#define DNSPIR_E_OOM (-7)         /* device allocation failed */
+#define DNSPIR_E_HIP (-8)         /* a HIP runtime call failed */
+#define DNSPIR_E_DECODE (-9)      /* reply digits out of range, double_pir.rs:88 */
+#define DNSPIR_E_NOINIT (-10)     /* dnspir_init not called / no device */
+
+typedef struct dnspir_shard dnspir_shard;
+typedef struct dnspir_ctx dnspir_ctx;
+typedef struct dnspir_rng dnspir_rng;
+
+/* per-call phase timing; replaces the racy globals PRIMARY_PHASE_MICROS /
+ * SECONDARY_PHASE_MICROS / MEM_LOAD_DATA (double_pir.rs:229-243, base_pir.rs:1251) */
+typedef struct {
+    double primary_us;       /* device time of the primary phase */
+    double secondary_us;     /* device time of the secondary phase */
+    double total_us;         /* wall time of the call */
+    uint64_t bytes_streamed; /* bytes read by the primary inner product */
+} dnspir_timing;
+
+/* Initialise the library on HIP device `device` (builds the NTT / Galois
+ * tables). Must precede every other device call. */
+int dnspir_init(int device);
+const char* dnspir_strerror(int code);
+
+/* ---- one shard: PIRDatabase (base_pir.rs:225-1247) ------------------------ */
+
+/* Bytes of caller-owned device memory a shard of index-group order
+ * `group_order` needs: PIRDatabase::required_memory_general * 32 B
+ * (base_pir.rs:369-371) plus 192 KiB for the cached a-part. */
+size_t dnspir_required_bytes(int group_order);
+/* PIRDatabase::create (base_pir.rs:280-349). used_seeds: 0 = FirstSet,
+ * 1 = SecondSet. dev_mem must be 256-byte aligned device memory of at least
+ * dnspir_required_bytes(group_order) bytes; the shard borrows it. */
+int dnspir_shard_create(int group_order, int used_seeds, void* dev_mem, size_t bytes, dnspir_shard** out);
+/* frees the host handle only; the caller owns dev_mem */
+int dnspir_shard_destroy(dnspir_shard* shard);
+int dnspir_shard_group_order(const dnspir_shard* shard);
+/* PIRDatabase::set_db (base_pir.rs:632-637): exactly 8 * group_order entries,
+ * host memory [n_entries][N]. */
+int dnspir_shard_set_db(dnspir_shard* shard, const uint16_t* entries, size_t n_entries);
+/* PIRDatabase::get_a (base_pir.rs:897-903): out [8][N] host */
+int dnspir_shard_get_a(const dnspir_shard* shard, uint64_t* out);
+/* PIRDatabase::perform_pir (base_pir.rs:945-952): qb [N], qb_conj [N] (NULL
+ * iff the index group needs no conjugate), gkb [2][N]; out_b [8][N] host. */
+int dnspir_shard_perform_pir(const dnspir_shard* shard, const uint64_t* qb, const uint64_t* qb_conj,
+                             const uint64_t* gkb, uint64_t* out_b);
+/* PIRDatabase::perform_pir_split (base_pir.rs:958-1078): query k on lane k;
+ * qb [8][N], qb_conj [8][N] or NULL, gkb [8][2][N]; out_b [8][N] host. */
+int dnspir_shard_perform_pir_split(const dnspir_shard* shard, const uint64_t* qb, const uint64_t* qb_conj,
+                                   const uint64_t* gkb, uint64_t* out_b);
+
+/* ---- query contexts: one per stream / serving slot ------------------------ */
+/* Device scratch of the per-query secondary database (the thread-local
+ * SECONDARY_DB_MEMORY of double_pir.rs:43-56), grow-only. `stream` is a
+ * hipStream_t (NULL = a new stream owned by the context). */
+int dnspir_ctx_create(void* stream, dnspir_ctx** out);
+int dnspir_ctx_destroy(dnspir_ctx* ctx);
+void* dnspir_ctx_stream(dnspir_ctx* ctx);
+
+/* ---- two-level composition: perform_batched_double_pir (double_pir.rs:126-228)
+ * fleets: random-access shard lists (shards may repeat). q0/q1: query blocks
+ * [6][N] = gk0, gk1, primary, primary_conj, secondary, secondary_conj (conj
+ * entries ignored when not required), NULL = empty batch slot.
+ * out0/out1: [3][2][N] = (a, b) of the three reply ciphertexts. ctx NULL =
+ * the library's default context. */
+int dnspir_double_pir(dnspir_ctx* ctx, const dnspir_shard* const* fleet0, size_t len0,
+                      const dnspir_shard* const* fleet1, size_t len1, const uint64_t* q0, const uint64_t* q1,
+                      uint64_t* out0, uint64_t* out1, dnspir_timing* timing);
+
+/* ---- byte API: pir_wrapper.rs ---------------------------------------------- */
+/* process_queries (pir_wrapper.rs:349-409): query bytes in, reply bytes out
+ * (dnspir_reply_bytes() each), host buffers; query1 NULL = empty slot. */
+int dnspir_process_queries(dnspir_ctx* ctx, const dnspir_shard* const* fleet0, size_t len0,
+                           const dnspir_shard* const* fleet1, size_t len1, const uint8_t* query0, size_t qlen0,
+                           const uint8_t* query1, size_t qlen1, uint8_t* reply0, uint8_t* reply1,
+                           dnspir_timing* timing);
+/* Same with query and reply bytes in device memory on the context's stream
+ * (no host copies; the call returns after the stream has finished). */
+int dnspir_process_queries_device(dnspir_ctx* ctx, const dnspir_shard* const* fleet0, size_t len0,
+                                  const dnspir_shard* const* fleet1, size_t len1, const uint8_t* query0_dev,
+                                  size_t qlen0, const uint8_t* query1_dev, size_t qlen1, uint8_t* reply0_dev,
+                                  uint8_t* reply1_dev, dnspir_timing* timing);
+/* get_database_shape (pir_wrapper.rs:219-231): index-group orders */
+void dnspir_database_shape(size_t num_entries, int* l_primary, int* l_secondary);
+/* num_databases (pir_wrapper.rs:238-241) */
+size_t dnspir_num_databases(size_t num_entries);
+/* choose_index_group_secondary (pir_wrapper.rs:166-175) */
+int dnspir_choose_index_group_secondary(size_t num_shards);
+size_t dnspir_query_bytes(int l_primary, int l_secondary);
+size_t dnspir_reply_bytes(void);
+
+/* ---- client side (host only): prepare_query / process_reply --------------- */
+/* StdRng stand-in: ChaCha12 in rand_chacha 0.3 word order */
+dnspir_rng* dnspir_rng_new(const uint8_t seed[32]);
+void dnspir_rng_free(dnspir_rng* rng);
+/* prepare_query_general (pir_wrapper.rs:276-317); query buffer of
+ * dnspir_query_bytes(l_primary, l_secondary) bytes */
+int dnspir_prepare_query(dnspir_rng* rng, size_t primary_idx, size_t secondary_idx, int l_primary, int l_secondary,
+                         uint8_t* query, size_t* qlen, uint8_t sk_seed[32]);
+/* process_reply (pir_wrapper.rs:420-440): out [N] coefficients mod 1025 */
+int dnspir_process_reply(const uint8_t* reply, size_t rlen, const uint8_t sk_seed[32], uint16_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DNSPIR_H */

@@ -1,0 +1,185 @@
+"""GPU parity: the HIP engine against the CPU oracle on identical seeded
+inputs, bit-exact (every value is an integer mod q). Runs through the C ABI
+(libdnspir.so) only."""
+import sys
+
+import numpy as np
+import pytest
+
+import pyoracle as orc
+
+pytestmark = pytest.mark.gpu
+
+Q = orc.Q
+N = 2048
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def pir():
+    from dnspir_amd import pir as p
+    p.init(0)
+    return p
+
+
+def _entries(n, seed, hi=1025):
+    return np.random.default_rng(seed).integers(0, hi, (n, N)).astype(np.uint16)
+
+
+def _pair(pir, l, seed, used_seeds=0, entries=None):
+    e = _entries(8 * l, seed) if entries is None else entries
+    g = pir.PIRDatabase(l, used_seeds)
+    g.set_db(e)
+    o = orc.Shard(11, l, used_seeds=used_seeds)
+    o.set_db(e)
+    return g, o, e
+
+
+_cache = {}
+
+
+def _shard(pir, l, seed=1, used_seeds=0):
+    key = (l, seed, used_seeds)
+    if key not in _cache:
+        _say(f"[parity] preprocessing l={l} seeds={used_seeds} seed={seed}")
+        _cache[key] = _pair(pir, l, seed, used_seeds)
+    return _cache[key]
+
+
+def _rand_q(seed, n=1):
+    r = np.random.default_rng(seed)
+    return r.integers(0, Q, (n, N), dtype=np.uint64)
+
+
+@pytest.mark.parametrize("l,used_seeds", [(1024, 0), (2048, 0), (8, 1), (64, 1), (256, 0), (2048, 1)])
+def test_get_a_and_perform_pir(pir, l, used_seeds):
+    g, o, _ = _shard(pir, l, 1, used_seeds)
+    _say(f"[parity] get_a l={l}")
+    assert np.array_equal(g.get_a(), o.get_a())
+    conj = l == N
+    for seed in (10, 11):
+        qb = _rand_q(seed)[0]
+        qbc = _rand_q(seed + 100)[0] if conj else None
+        gk = _rand_q(seed + 200, 2)
+        _say(f"[parity] perform_pir l={l} seed={seed}")
+        assert np.array_equal(g.perform_pir(qb, qbc, gk), o.perform_pir(qb, qbc, gk))
+
+
+@pytest.mark.parametrize("l", [1024, 2048, 32])
+def test_perform_pir_split(pir, l):
+    g, o, _ = _shard(pir, l, 1, 0)
+    qb = _rand_q(30, 8)
+    qbc = _rand_q(31, 8) if l == N else None
+    gk = _rand_q(32, 16).reshape(8, 2, N)
+    _say(f"[parity] perform_pir_split l={l}")
+    assert np.array_equal(g.perform_pir_split(qb, qbc, gk), o.perform_pir_split(qb, qbc, gk))
+
+
+def test_edge_entries_all_zero_and_all_max(pir):
+    for val in (0, 1024):
+        e = np.full((8 * 1024, N), val, np.uint16)
+        g, o, _ = _pair(pir, 1024, 0, entries=e)
+        qb = _rand_q(40)[0]
+        gk = _rand_q(41, 2)
+        _say(f"[parity] constant entries {val}")
+        assert np.array_equal(g.get_a(), o.get_a())
+        assert np.array_equal(g.perform_pir(qb, None, gk), o.perform_pir(qb, None, gk))
+
+
+def _block(seed, conj_p, conj_s):
+    q = _rand_q(seed, 6)
+    if not conj_p:
+        q[3] = 0
+    if not conj_s:
+        q[5] = 0
+    return q
+
+
+@pytest.mark.parametrize("l,lens", [(1024, (1, 0)), (1024, (3, 2)), (2048, (2, 2)), (64, (5, 1))])
+def test_double_pir(pir, l, lens):
+    g, o, _ = _shard(pir, l, 1, 0)
+    g2, o2, _ = _shard(pir, l, 2, 0)
+    fg = [g, g2, g][:lens[0]] if lens[0] <= 3 else [g, g2] * (lens[0] // 2) + [g] * (lens[0] % 2)
+    fo = [o, o2, o][:lens[0]] if lens[0] <= 3 else [o, o2] * (lens[0] // 2) + [o] * (lens[0] % 2)
+    ls = orc.choose_index_group_secondary(lens[0])
+    conj_p, conj_s = l == N, ls == N
+    q0 = _block(50, conj_p, conj_s)
+    q1 = _block(51, conj_p, conj_s) if lens[1] else None
+    _say(f"[parity] double_pir l={l} lens={lens}")
+    go, tm = pir.perform_batched_double_pir(fg, fg[:lens[1]] if lens[1] else None, q0, q1)
+    oo, _, _ = orc.double_pir(fo, fo[:lens[1]] if lens[1] else None, q0, q1, ls)
+    assert np.array_equal(go[0], oo[0])
+    if lens[1]:
+        assert np.array_equal(go[1], oo[1])
+        # the same query alone gives the same bytes as in the mix
+        alone, _ = pir.perform_batched_double_pir(fg, None, q0, None)
+        assert np.array_equal(alone[0], go[0])
+
+
+def test_double_pir_full_group_secondary(pir):
+    """More than N/16 shards: full-group secondary with conjugated query
+    (the 2^22 shape), fleet of 130 repeated half shards."""
+    g, o, _ = _shard(pir, 1024, 1, 0)
+    g2, o2, _ = _shard(pir, 1024, 2, 0)
+    fg = [g, g2] * 65
+    fo = [o, o2] * 65
+    ls = orc.choose_index_group_secondary(len(fo))
+    assert ls == N
+    q0 = _block(60, False, True)
+    _say("[parity] double_pir full-group secondary, 130 shards")
+    go, _ = pir.perform_batched_double_pir(fg, None, q0, None)
+    oo, _, _ = orc.double_pir(fo, None, q0, None, ls)
+    assert np.array_equal(go[0], oo[0])
+
+
+@pytest.mark.parametrize("l", [1024, 2048])
+def test_process_queries_bytes_and_decrypt(pir, l):
+    g, o, e = _shard(pir, l, 1, 0)
+    g2, o2, e2 = _shard(pir, l, 2, 0)
+    fg, fo = [g, g2], [o, o2]
+    lp = l
+    ls = orc.choose_index_group_secondary(2)
+    rng = pir.Rng(bytes([5] * 32))
+    for (pi, si) in [(0, 0), (8 * l - 1, 1), (2742 % (8 * l), 1)]:
+        q, sk = pir.prepare_query(rng, pi, si, lp, ls)
+        _say(f"[parity] process_queries l={l} idx=({pi},{si})")
+        reps, tm = pir.process_queries(fg, None, q, None)
+        oreps, _, _ = orc.process_queries(fo, None, q, None)
+        assert reps[0] == oreps[0]
+        want = (e, e2)[si][pi]
+        assert np.array_equal(pir.process_reply(reps[0], sk), want)
+
+
+def test_process_queries_two_slots(pir):
+    g, o, e = _shard(pir, 1024, 1, 0)
+    rng = pir.Rng(bytes([6] * 32))
+    q0, sk0 = pir.prepare_query(rng, 77, 0, 1024, 8)
+    q1, sk1 = pir.prepare_query(rng, 4000, 0, 1024, 8)
+    _say("[parity] process_queries two slots")
+    reps, _ = pir.process_queries([g], [g], q0, q1)
+    oreps, _, _ = orc.process_queries([o], [o], q0, q1)
+    assert reps == oreps
+    assert np.array_equal(pir.process_reply(reps[0], sk0), e[77])
+    assert np.array_equal(pir.process_reply(reps[1], sk1), e[4000])
+
+
+def test_error_paths(pir):
+    from dnspir_amd import DnspirError
+    g, _, e = _shard(pir, 1024, 1, 0)
+    bad = pir.PIRDatabase(1024)
+    with pytest.raises(DnspirError) as ei:
+        bad.set_db(e[:-1])
+    assert ei.value.code == -3
+    full, _, _ = _shard(pir, 2048, 1, 0)
+    with pytest.raises(DnspirError) as ei:
+        full.perform_pir(_rand_q(1)[0], None, _rand_q(2, 2))
+    assert ei.value.code == -2
+    rng = pir.Rng(bytes(32))
+    q, _ = pir.prepare_query(rng, 1, 0, 1024, 8)
+    with pytest.raises(DnspirError) as ei:
+        pir.process_queries([g], None, q[:-1], None)
+    assert ei.value.code == -1

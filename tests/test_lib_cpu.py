@@ -1,0 +1,84 @@
+"""CPU-side checks of the C ABI library: it loads, exports every symbol that
+include/dnspir.h declares, its host-only functions (shape rule, wire sizes,
+client) agree with the oracle, and device calls fail cleanly without a GPU."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import pyoracle as orc
+from dnspir_amd import _lib
+from dnspir_amd import pir
+
+
+def _declared_functions():
+    text = open(_lib.HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(dnspir_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.lib()
+    names = _declared_functions()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(_lib.SIGNATURES), set(names) ^ set(_lib.SIGNATURES)
+
+
+@pytest.mark.parametrize("n", [1, 8192, 16384, 32768, 262144, 262145, 1 << 20, 3 << 20, 1 << 22])
+def test_shape_rule_matches_oracle(n):
+    assert pir.get_database_shape(n) == orc.database_shape(n)
+    assert pir.num_databases(n) == orc.num_databases(n)
+    lp, ls = pir.get_database_shape(n)
+    assert pir.query_bytes(lp, ls) == orc.query_bytes(lp, ls)
+
+
+def test_reply_bytes_and_required_bytes():
+    assert pir.reply_bytes() == 22272
+    assert _lib.lib().dnspir_required_bytes(2048) == 805306368 + 196608
+    assert _lib.lib().dnspir_required_bytes(1024) == 402653184 + 196608
+    assert _lib.lib().dnspir_required_bytes(1536) == 0  # not a divisor of N
+    assert _lib.lib().dnspir_required_bytes(2048 * 3 // 4) == 0
+
+
+@pytest.mark.parametrize("shape", [(1024, 32), (2048, 512), (2048, 2048), (1024, 8)])
+def test_client_query_bytes_match_oracle(shape):
+    """prepare_query of the product and of the oracle are the same restatement:
+    identical bytes and key seeds for the same RNG stream."""
+    lp, ls = shape
+    ro = orc.Rng(bytes([7] * 32))
+    rp = pir.Rng(bytes([7] * 32))
+    for (pi, si) in [(0, 0), (lp * 8 - 1, ls // 8 - 1), (2742, 0)]:
+        pi %= lp * 8
+        qo, sko = orc.prepare_query(ro, pi, si, lp, ls)
+        qp, skp = pir.prepare_query(rp, pi, si, lp, ls)
+        assert sko == skp
+        assert qo == qp
+        assert len(qp) == pir.query_bytes(lp, ls)
+
+
+def test_client_process_reply_decrypts_oracle_reply():
+    """The product's process_reply decodes a reply the oracle computed."""
+    n_entries = 8192
+    lp, ls = orc.database_shape(n_entries)
+    data = np.random.default_rng(3).integers(0, 1024, (8 * lp, 2048)).astype(np.uint16)
+    db = orc.Shard(11, lp)
+    db.set_db(data)
+    rng = pir.Rng(bytes(32))
+    q, sk = pir.prepare_query(rng, 4321, 0, lp, ls)
+    reps, _, _ = orc.process_queries([db], None, q, None)
+    assert np.array_equal(pir.process_reply(reps[0], sk), data[4321])
+    with pytest.raises(_lib.DnspirError):
+        pir.process_reply(reps[0][:-1], sk)
+
+
+def test_device_calls_without_init_fail_cleanly():
+    if os.environ.get("HIP_VISIBLE_DEVICES") is None:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is present")
+    assert _lib.lib().dnspir_init(0) == -10
+    h = __import__("ctypes").c_void_p()
+    assert _lib.lib().dnspir_ctx_create(None, __import__("ctypes").byref(h)) == -10

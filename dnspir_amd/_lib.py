@@ -36,7 +36,8 @@ class DnspirError(RuntimeError):
 
 class Timing(ctypes.Structure):
     _fields_ = [("primary_us", ctypes.c_double), ("secondary_us", ctypes.c_double),
-                ("total_us", ctypes.c_double), ("bytes_streamed", ctypes.c_uint64)]
+                ("total_us", ctypes.c_double), ("bytes_streamed", ctypes.c_uint64),
+                ("k1_us", ctypes.c_double), ("k1_launches", ctypes.c_int)]
 
 
 vp = ctypes.c_void_p
@@ -55,6 +56,7 @@ SIGNATURES = {
     "dnspir_shard_destroy": (ci, [vp]),
     "dnspir_shard_group_order": (ci, [vp]),
     "dnspir_shard_set_db": (ci, [vp, u16p, sz]),
+    "dnspir_shard_set_db_device": (ci, [vp, vp, sz]),
     "dnspir_shard_get_a": (ci, [vp, u64p]),
     "dnspir_shard_perform_pir": (ci, [vp, u64p, u64p, u64p, u64p]),
     "dnspir_shard_perform_pir_split": (ci, [vp, u64p, u64p, u64p, u64p]),

@@ -43,7 +43,7 @@ struct DevBuf {
     size_t n = 0;
     int ensure(size_t bytes) {
         if (bytes <= n) return 0;
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
         p = nullptr;
         n = 0;
         if (hipMalloc(&p, bytes) != hipSuccess) return DNSPIR_E_OOM;
@@ -53,7 +53,7 @@ struct DevBuf {
     template <class X>
     X* as() const { return reinterpret_cast<X*>(p); }
     ~DevBuf() {
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     }
 };
 
@@ -185,7 +185,8 @@ struct dnspir_ctx {
     hipStream_t stream = nullptr;
     bool own = false;
     DevBuf se, M, te, aout, cout, qcoef, qs, qp, acc, acoef, bcoef, a17o, b12o, ptrs, idx, qbytes, rbytes;
-    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // start, primary end, end, k1 start, k1 end
+    int k1_launches = 0;
     std::vector<const dnspir_shard*> cached[2];
     std::vector<uint64_t> stage;
     int h_idx[18];  // host staging of the lane lists (outlives the async copy)
@@ -307,7 +308,11 @@ int run_double(dnspir_ctx* c, const dnspir_shard* const* fleets[2], const size_t
         const int len = (int)lens[sl];
         unsigned long long* acc = c->acc.as<unsigned long long>();
         launch_fill_u64(acc, (size_t)len * 2 * N * LANES, 0, s);
+        if (sl == 0 || !present[0]) HIP_OK(hipEventRecord(c->ev[3], s));
         launch_k1(slabs[sl], len, c->qp.as<uint32_t>() + (size_t)sl * 4 * 2 * N, acc, L_p, nst_p, k1_rows(L_p, len, nst_p), s);
+        if (sl == 0 && present[1]) c->k1_launches = 2;
+        else if (sl == 0 || !present[0]) c->k1_launches = 1;
+        HIP_OK(hipEventRecord(c->ev[4], s));
         launch_epi_primary(acc, len, a17s[sl], c->se.as<uint16_t>(), nullptr, sl, g.tables, s);
         streamed += (uint64_t)len * slab_bytes(l_p);
     }
@@ -324,12 +329,16 @@ int run_double(dnspir_ctx* c, const dnspir_shard* const* fleets[2], const size_t
 
 void fill_timing(dnspir_ctx* c, dnspir_timing* tm, double wall_us) {
     if (!tm) return;
-    float a = 0, b = 0;
-    hipEventElapsedTime(&a, c->ev[0], c->ev[1]);
-    hipEventElapsedTime(&b, c->ev[1], c->ev[2]);
+    float a = 0, b = 0, k = 0;
+    (void)hipEventElapsedTime(&a, c->ev[0], c->ev[1]);
+    (void)hipEventElapsedTime(&b, c->ev[1], c->ev[2]);
+    (void)hipEventElapsedTime(&k, c->ev[3], c->ev[4]);
     tm->primary_us = a * 1000.0;
     tm->secondary_us = b * 1000.0;
     tm->total_us = wall_us;
+    // with two slots the span also holds the first slot's epilogue
+    tm->k1_us = k * 1000.0;
+    tm->k1_launches = c->k1_launches;
 }
 
 double now_us() {
@@ -418,7 +427,7 @@ int dnspir_shard_destroy(dnspir_shard* s) {
 
 int dnspir_shard_group_order(const dnspir_shard* s) { return s ? s->l : 0; }
 
-int dnspir_shard_set_db(dnspir_shard* sh, const uint16_t* entries, size_t n_entries) {
+static int set_db_impl(dnspir_shard* sh, const uint16_t* entries, size_t n_entries, bool on_device) {
     if (!sh || !entries) return DNSPIR_E_ARG;
     int rc = ensure_init();
     if (rc) return rc;
@@ -426,7 +435,7 @@ int dnspir_shard_set_db(dnspir_shard* sh, const uint16_t* entries, size_t n_entr
     std::lock_guard<std::mutex> lk(g.mu);
     const int l = sh->l, L = sh->L, nst = sh->nst;
     const hipStream_t s = g.pre_stream;
-    if ((rc = g.pre_entries.ensure(n_entries * N * 2))) return rc;
+    if (!on_device && (rc = g.pre_entries.ensure(n_entries * N * 2))) return rc;
     if ((rc = g.pre_M.ensure((size_t)LANES * l * N * 4))) return rc;
     if ((rc = g.pre_te.ensure((size_t)nst * LANES * L * 2 * N * 4))) return rc;
     if ((rc = g.pre_ip.ensure((size_t)nst * LANES * L * 4 * N * 4))) return rc;
@@ -438,8 +447,12 @@ int dnspir_shard_set_db(dnspir_shard* sh, const uint16_t* entries, size_t n_entr
         h_idx[8 + i] = l;
     }
     HIP_OK(hipMemcpyAsync(g.pre_idx.p, h_idx, sizeof(h_idx), hipMemcpyHostToDevice, s));
-    HIP_OK(hipMemcpyAsync(g.pre_entries.p, entries, n_entries * N * 2, hipMemcpyHostToDevice, s));
-    launch_k5(g.pre_entries.as<uint16_t>(), g.pre_M.as<uint32_t>(), g.pre_te.as<uint32_t>(), l, g.pre_idx.as<int>() + 8,
+    const uint16_t* ent = entries;
+    if (!on_device) {
+        HIP_OK(hipMemcpyAsync(g.pre_entries.p, entries, n_entries * N * 2, hipMemcpyHostToDevice, s));
+        ent = g.pre_entries.as<uint16_t>();
+    }
+    launch_k5(ent, g.pre_M.as<uint32_t>(), g.pre_te.as<uint32_t>(), l, g.pre_idx.as<int>() + 8,
               g.pre_idx.as<int>(), LANES, g.tables, s);
     launch_chain_pre(g.pre_te.as<uint32_t>(), g.pre_ip.as<uint32_t>(), g.pre_aout.as<uint32_t>(), L, nst,
                      sh->seeds == 0 ? 0 : 2, g.tables, s);
@@ -449,6 +462,14 @@ int dnspir_shard_set_db(dnspir_shard* sh, const uint16_t* entries, size_t n_entr
     HIP_OK(hipStreamSynchronize(s));
     sh->has_db = true;
     return 0;
+}
+
+int dnspir_shard_set_db(dnspir_shard* sh, const uint16_t* entries, size_t n_entries) {
+    return set_db_impl(sh, entries, n_entries, false);
+}
+
+int dnspir_shard_set_db_device(dnspir_shard* sh, const uint16_t* entries_dev, size_t n_entries) {
+    return set_db_impl(sh, entries_dev, n_entries, true);
 }
 
 int dnspir_shard_get_a(const dnspir_shard* sh, uint64_t* out) {
@@ -526,10 +547,10 @@ int dnspir_ctx_create(void* stream, dnspir_ctx** out) {
 
 int dnspir_ctx_destroy(dnspir_ctx* c) {
     if (!c) return DNSPIR_E_ARG;
-    if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (auto& e : c->ev)
-        if (e) hipEventDestroy(e);
-    if (c->own && c->stream) hipStreamDestroy(c->stream);
+        if (e) (void)hipEventDestroy(e);
+    if (c->own && c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return 0;
 }

@@ -200,101 +200,264 @@ __global__ __launch_bounds__(256) void k5_cols_final_kernel(const uint32_t* __re
 
 // ---------------------------------------------------------------------------
 // K4/K6: the sequential a-chain (preprocess_loop, base_pir.rs:819-862, with
-// partial_gal :496-594). One block per (lane, stream). SEC additionally runs
-// the secondary b-recurrence c <- sigma_5(c) + D0 gkb0 + D1 gkb1 + TE qb
-// (main_pir_loop in gathered form) in the same step, so the digits never go
-// to memory. PRE stores the digits as the IPGK rows.
+// partial_gal :496-594). One block of 1024 threads per (lane, stream). Step i:
+//   A  gather sigma_5(a) and inverse NTT in p1 and p2     (threads 0..511)
+//   B  CRT, centered lift, balanced base-2^26 digits D0, D1 (all threads)
+//   C  forward NTTs of D0, D1 in p1 and p2                  (256 threads each)
+//   D  a <- D0 gka0 + D1 gka1 + qa TE_i (per slot)
+// SEC additionally runs the secondary b-recurrence in gathered form,
+// c <- sigma_5(c) + D0 gkb0 + D1 gkb1 + TE_i qb (main_pir_loop,
+// base_pir.rs:1085-1247), so the digits never leave the chip; PRE stores the
+// digits as the shard's IPGK rows.
+// NTT passes are radix-8 (radix-4 for the last two stages) over registers with
+// LDS exchanges; values stay in [0, 2p) between butterflies (Harvey's lazy
+// reduction with Shoup multiplications); twiddles live in LDS.
 // ---------------------------------------------------------------------------
-template <bool SEC>
-__global__ __launch_bounds__(512) void chain_kernel(const uint32_t* __restrict__ te_tmp, uint32_t* __restrict__ ip_tmp,
-                                                    uint32_t* __restrict__ a_out, uint32_t* __restrict__ c_out,
-                                                    const uint32_t* __restrict__ qs, const int* __restrict__ lane_slot,
-                                                    int L, int nlanes, int qa_base, const DevTables* __restrict__ T) {
-    __shared__ uint32_t sa[2][N];
-    __shared__ uint32_t bufA[2][N];
-    __shared__ uint32_t bufB[2][N];
-    __shared__ uint32_t sd[2][2][N];
-    const int li = blockIdx.x, st = blockIdx.y;
-    const int tid = threadIdx.x, nt = blockDim.x;
-    uint32_t(*sw)[N] = bufA;
-    uint32_t(*sc)[N] = bufB;
-    for (int s = tid; s < N; s += nt) {
-        sa[0][s] = 0; sa[1][s] = 0;
-        sc[0][s] = 0; sc[1][s] = 0;
+__device__ __forceinline__ uint32_t red2p(uint32_t a, uint32_t p2) {  // [0, 4p) -> [0, 2p)
+    const uint32_t b = a - p2;
+    return a < p2 ? a : b;
+}
+__device__ __forceinline__ uint32_t shoup_lazy(uint32_t a, uint32_t w, uint32_t wq, uint32_t p) {  // -> [0, 2p)
+    return a * w - __umulhi(a, wq) * p;
+}
+// forward CT butterfly, inputs/outputs in [0, 2p)
+__device__ __forceinline__ void bf_ct(uint32_t& x, uint32_t& y, uint32_t w, uint32_t wq, uint32_t p) {
+    const uint32_t t = shoup_lazy(y, w, wq, p);
+    const uint32_t p2 = 2 * p;
+    y = red2p(x - t + p2, p2);
+    x = red2p(x + t, p2);
+}
+// inverse GS butterfly, inputs/outputs in [0, 2p)
+__device__ __forceinline__ void bf_gs(uint32_t& x, uint32_t& y, uint32_t w, uint32_t wq, uint32_t p) {
+    const uint32_t p2 = 2 * p;
+    const uint32_t u = x, v = y;
+    x = red2p(u + v, p2);
+    y = shoup_lazy(u - v + p2, w, wq, p);
+}
+
+// one radix-8 forward pass (stages 3P..3P+2) for group g of 256
+__device__ __forceinline__ void fwd_pass8(uint32_t* buf, const uint32_t* tw, const uint32_t* twq, uint32_t p, int P, int g) {
+    const int m = 1 << (3 * P);
+    const int q = N >> (3 * P + 3);
+    const int i = g / q, r = g % q;
+    const int j0 = i * 8 * q + r;
+    uint32_t v[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) v[c] = buf[j0 + c * q];
+    {
+        const uint32_t w = tw[m + i], wq = twq[m + i];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) bf_ct(v[c], v[c + 4], w, wq, p);
     }
-    __syncthreads();
-    const uint32_t* te_base = te_tmp + ((size_t)(st * nlanes + li) * L) * 2 * N;
-    const uint32_t* qv = nullptr;
-    if (SEC) qv = qs + (size_t)lane_slot[li] * 6 * 2 * N;  // [v][r][slot]
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const uint32_t w = tw[2 * m + 2 * i + h], wq = twq[2 * m + 2 * i + h];
+        bf_ct(v[4 * h + 0], v[4 * h + 2], w, wq, p);
+        bf_ct(v[4 * h + 1], v[4 * h + 3], w, wq, p);
+    }
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+        const uint32_t w = tw[4 * m + 4 * i + h], wq = twq[4 * m + 4 * i + h];
+        bf_ct(v[2 * h], v[2 * h + 1], w, wq, p);
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c) buf[j0 + c * q] = v[c];
+}
+// last forward pass: stages 9 and 10 for the two radix-4 groups g, g + 256
+__device__ __forceinline__ void fwd_pass4(uint32_t* buf, const uint32_t* tw, const uint32_t* twq, uint32_t p, int g) {
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int i = g + 256 * k;
+        uint32_t v0 = buf[4 * i], v1 = buf[4 * i + 1], v2 = buf[4 * i + 2], v3 = buf[4 * i + 3];
+        const uint32_t w = tw[512 + i], wq = twq[512 + i];
+        bf_ct(v0, v2, w, wq, p);
+        bf_ct(v1, v3, w, wq, p);
+        bf_ct(v0, v1, tw[1024 + 2 * i], twq[1024 + 2 * i], p);
+        bf_ct(v2, v3, tw[1025 + 2 * i], twq[1025 + 2 * i], p);
+        buf[4 * i] = v0; buf[4 * i + 1] = v1; buf[4 * i + 2] = v2; buf[4 * i + 3] = v3;
+    }
+}
+// one radix-8 inverse pass (stages 3P+2..3P)
+__device__ __forceinline__ void inv_pass8(uint32_t* buf, const uint32_t* tw, const uint32_t* twq, uint32_t p, int P, int g) {
+    const int m = 1 << (3 * P);
+    const int q = N >> (3 * P + 3);
+    const int i = g / q, r = g % q;
+    const int j0 = i * 8 * q + r;
+    uint32_t v[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) v[c] = buf[j0 + c * q];
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+        const uint32_t w = tw[4 * m + 4 * i + h], wq = twq[4 * m + 4 * i + h];
+        bf_gs(v[2 * h], v[2 * h + 1], w, wq, p);
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const uint32_t w = tw[2 * m + 2 * i + h], wq = twq[2 * m + 2 * i + h];
+        bf_gs(v[4 * h + 0], v[4 * h + 2], w, wq, p);
+        bf_gs(v[4 * h + 1], v[4 * h + 3], w, wq, p);
+    }
+    {
+        const uint32_t w = tw[m + i], wq = twq[m + i];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) bf_gs(v[c], v[c + 4], w, wq, p);
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c) buf[j0 + c * q] = v[c];
+}
+
+constexpr uint32_t NINV_Q2INV_P1 = 67055624u;  // N^-1 p2^-1 mod p1
+constexpr uint32_t NINV_Q1INV_P2 = 8189u;      // N^-1 p1^-1 mod p2
+
+template <bool SEC>
+__global__ __launch_bounds__(1024) void chain_kernel(const uint32_t* __restrict__ te_tmp, uint32_t* __restrict__ ip_tmp,
+                                                     uint32_t* __restrict__ a_out, uint32_t* __restrict__ c_out,
+                                                     const uint32_t* __restrict__ qs, const int* __restrict__ lane_slot,
+                                                     int L, int nlanes, int qa_base, const DevTables* __restrict__ T) {
+    __shared__ uint32_t tw[2][2][N], twq[2][2][N];  // [fwd/inv][r][k]
+    __shared__ uint32_t sa[2][N];
+    __shared__ uint32_t sc[2][2][N];                // [buffer][r][slot]
+    __shared__ uint32_t xb[4][N];
+    const int li = blockIdx.x, st = blockIdx.y;
+    const int tid = threadIdx.x;
+    for (int k = tid; k < N; k += 1024) {
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            tw[0][r][k] = T->psi[r][k]; twq[0][r][k] = T->psiq[r][k];
+            tw[1][r][k] = T->ipsi[r][k]; twq[1][r][k] = T->ipsiq[r][k];
+            sa[r][k] = 0;
+            sc[0][r][k] = 0;
+        }
+    }
+    // per-thread constants for the update phase: slots tid, tid + 1024, both moduli
+    uint32_t cg0[4], cg1[4], cqa[4], cb0[4], cb1[4], cqb[4];
+    int sig_upd[2];
     const uint32_t* qa = T->qa[qa_base + st][0];
+    const uint32_t* qv = SEC ? qs + (size_t)lane_slot[li] * 6 * 2 * N : nullptr;  // [v][r][slot]
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int s = tid + 1024 * (k >> 1), r = k & 1;
+        cg0[k] = T->gka[0][r][s];
+        cg1[k] = T->gka[1][r][s];
+        cqa[k] = qa[r * N + s];
+        if (SEC) {
+            cb0[k] = qv[(0 * 2 + r) * N + s];
+            cb1[k] = qv[(1 * 2 + r) * N + s];
+            cqb[k] = qv[((4 + st) * 2 + r) * N + s];
+        }
+    }
+    sig_upd[0] = T->sigma5[tid];
+    sig_upd[1] = T->sigma5[tid + 1024];
+    // phase A input gather indices: radix-4 groups g, g + 256 of modulus rA
+    const int rA = (tid >> 8) & 1, gA = tid & 255;
+    int sigA[8];
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) sigA[4 * k + c] = T->sigma5[4 * (gA + 256 * k) + c];
+    const int q4 = tid >> 8, rC = q4 & 1, gC = tid & 255;
+    __syncthreads();
+
+    const uint32_t* te_base = te_tmp + ((size_t)(st * nlanes + li) * L) * 2 * N;
+    int cur = 0;
 #pragma unroll 1
     for (int i = 0; i < L; ++i) {
-        for (int s = tid; s < N; s += nt) {
-            const int g = T->sigma5[s];
-            sw[0][s] = sa[0][g];
-            sw[1][s] = sa[1][g];
-        }
-        __syncthreads();
-        {
-            const int half = tid >= nt / 2;
-            ntt_inv(sw[half], T, half, tid - half * (nt / 2), nt / 2);
-        }
-        for (int s = tid; s < N; s += nt) {
-            const int64_t x = center_q(crt(sw[0][s], sw[1][s]));
-            const int64_t hi = (x + (1ll << (LOG2_B - 1))) >> LOG2_B;
-            const int64_t lo = x - (hi << LOG2_B);
-            const int64_t hi2 = (hi + (1ll << (LOG2_B - 1))) >> LOG2_B;
-            const int64_t lo2 = hi - (hi2 << LOG2_B);
-            sd[0][0][s] = from_i64(lo, P1, 26);
-            sd[0][1][s] = from_i64(lo, P2, 25);
-            sd[1][0][s] = from_i64(lo2, P1, 26);
-            sd[1][1][s] = from_i64(lo2, P2, 25);
-        }
-        __syncthreads();
-        {
-            const int q4 = tid / (nt / 4);
-            ntt_fwd(sd[q4 >> 1][q4 & 1], T, q4 & 1, tid - q4 * (nt / 4), nt / 4);
-        }
+        // TE row of this step (used in phase D)
+        uint32_t te[4];
         const uint32_t* te_row = te_base + (size_t)i * 2 * N;
-        for (int s = tid; s < N; s += nt) {
 #pragma unroll
-            for (int r = 0; r < 2; ++r) {
-                const uint32_t p = mod_p(r);
-                const int K = mod_k(r);
-                const uint32_t d0 = sd[0][r][s], d1 = sd[1][r][s];
-                const uint32_t te = te_row[r * N + s];
-                const uint64_t av = (uint64_t)d0 * T->gka[0][r][s] + (uint64_t)d1 * T->gka[1][r][s] +
-                                    (uint64_t)te * qa[r * N + s];
-                sa[r][s] = red64(av, p, K);
-                if (SEC) {
-                    const uint64_t cv = (uint64_t)sc[r][T->sigma5[s]] + (uint64_t)d0 * qv[(0 * 2 + r) * N + s] +
-                                        (uint64_t)d1 * qv[(1 * 2 + r) * N + s] +
-                                        (uint64_t)te * qv[((4 + st) * 2 + r) * N + s];
-                    sw[r][s] = red64(cv, p, K);
-                } else {
-                    uint32_t* dst = ip_tmp + (((size_t)(st * nlanes + li) * L + i) * 4) * N;
-                    dst[(0 * 2 + r) * N + s] = d0;
-                    dst[(1 * 2 + r) * N + s] = d1;
-                }
+        for (int k = 0; k < 4; ++k) te[k] = te_row[(k & 1) * N + tid + 1024 * (k >> 1)];
+
+        // ---- A: inverse NTTs of sigma_5(a) ----
+        if (tid < 512) {
+            const uint32_t p = mod_p(rA);
+            const uint32_t* itw = tw[1][rA];
+            const uint32_t* itwq = twq[1][rA];
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const int ii = gA + 256 * k;
+                uint32_t v0 = sa[rA][sigA[4 * k]], v1 = sa[rA][sigA[4 * k + 1]];
+                uint32_t v2 = sa[rA][sigA[4 * k + 2]], v3 = sa[rA][sigA[4 * k + 3]];
+                bf_gs(v0, v1, itw[1024 + 2 * ii], itwq[1024 + 2 * ii], p);
+                bf_gs(v2, v3, itw[1025 + 2 * ii], itwq[1025 + 2 * ii], p);
+                const uint32_t w = itw[512 + ii], wq = itwq[512 + ii];
+                bf_gs(v0, v2, w, wq, p);
+                bf_gs(v1, v3, w, wq, p);
+                uint32_t* b = xb[rA];
+                b[4 * ii] = v0; b[4 * ii + 1] = v1; b[4 * ii + 2] = v2; b[4 * ii + 3] = v3;
             }
         }
         __syncthreads();
-        if (SEC) {
-            uint32_t(*t)[N] = sw;
-            sw = sc;
-            sc = t;
+#pragma unroll 1
+        for (int P = 2; P >= 0; --P) {
+            if (tid < 512) inv_pass8(xb[rA], tw[1][rA], twq[1][rA], mod_p(rA), P, gA);
+            __syncthreads();
         }
+        // ---- B: CRT, centered lift, digits (base_pir.rs:544-570) ----
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int j = tid + 1024 * k;
+            uint32_t x1 = xb[0][j], x2 = xb[1][j];
+            x1 = x1 >= P1 ? x1 - P1 : x1;
+            x2 = x2 >= P2 ? x2 - P2 : x2;
+            const uint32_t e1 = mul_mod(x1, NINV_Q2INV_P1, P1, 26);
+            const uint32_t e2 = mul_mod(x2, NINV_Q1INV_P2, P2, 25);
+            uint64_t v = (uint64_t)e1 * P2 + (uint64_t)e2 * P1;
+            v = v >= Q ? v - Q : v;
+            const int64_t x = center_q(v);
+            const int64_t hi = (x + (1ll << (LOG2_B - 1))) >> LOG2_B;
+            const int32_t lo = (int32_t)(x - (hi << LOG2_B));               // [-2^25, 2^25)
+            const int32_t hi2 = (int32_t)((hi + (1ll << (LOG2_B - 1))) >> LOG2_B);
+            const int32_t lo2 = (int32_t)(hi - ((int64_t)hi2 << LOG2_B));   // |lo2| <= 2^24
+            uint32_t t;
+            t = (uint32_t)(lo + (int32_t)(2 * P1)); t = t >= 2 * P1 ? t - 2 * P1 : t; xb[0][j] = t >= P1 ? t - P1 : t;
+            t = (uint32_t)(lo + (int32_t)(2 * P2)); t = t >= 2 * P2 ? t - 2 * P2 : t; xb[1][j] = t >= P2 ? t - P2 : t;
+            t = (uint32_t)(lo2 + (int32_t)P1); xb[2][j] = t >= P1 ? t - P1 : t;
+            t = (uint32_t)(lo2 + (int32_t)P2); xb[3][j] = t >= P2 ? t - P2 : t;
+        }
+        __syncthreads();
+        // ---- C: forward NTTs of the four digit vectors ----
+        {
+            const uint32_t p = mod_p(rC);
+#pragma unroll 1
+            for (int P = 0; P < 3; ++P) {
+                fwd_pass8(xb[q4], tw[0][rC], twq[0][rC], p, P, gC);
+                __syncthreads();
+            }
+            fwd_pass4(xb[q4], tw[0][rC], twq[0][rC], p, gC);
+        }
+        __syncthreads();
+        // ---- D: update a (and c) per slot ----
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int s = tid + 1024 * (k >> 1), r = k & 1;
+            const uint32_t p = mod_p(r);
+            const int K = mod_k(r);
+            const uint32_t d0 = xb[r][s], d1 = xb[2 + r][s];  // [0, 2p)
+            sa[r][s] = red64((uint64_t)d0 * cg0[k] + (uint64_t)d1 * cg1[k] + (uint64_t)te[k] * cqa[k], p, K);
+            if (SEC) {
+                const uint64_t cv = (uint64_t)sc[cur][r][sig_upd[k >> 1]] + (uint64_t)d0 * cb0[k] + (uint64_t)d1 * cb1[k] +
+                                    (uint64_t)te[k] * cqb[k];
+                sc[cur ^ 1][r][s] = red64(cv, p, K);
+            } else {
+                uint32_t* dst = ip_tmp + (((size_t)(st * nlanes + li) * L + i) * 4) * N;
+                dst[(0 * 2 + r) * N + s] = d0 >= p ? d0 - p : d0;
+                dst[(1 * 2 + r) * N + s] = d1 >= p ? d1 - p : d1;
+            }
+        }
+        cur ^= 1;
+        __syncthreads();
     }
     uint32_t* ao = a_out + ((size_t)(st * nlanes + li) * 2) * N;
-    for (int s = tid; s < N; s += nt) {
+    for (int s = tid; s < N; s += 1024) {
         ao[s] = sa[0][s];
         ao[N + s] = sa[1][s];
     }
     if (SEC) {
         uint32_t* co = c_out + ((size_t)(st * nlanes + li) * 2) * N;
-        for (int s = tid; s < N; s += nt) {
-            co[s] = sc[0][s];
-            co[N + s] = sc[1][s];
+        for (int s = tid; s < N; s += 1024) {
+            co[s] = sc[cur][0][s];
+            co[N + s] = sc[cur][1][s];
         }
     }
 }
@@ -548,13 +711,13 @@ void launch_k5(const uint16_t* entries, uint32_t* M, uint32_t* te_tmp, int l, co
 
 void launch_chain_pre(const uint32_t* te_tmp, uint32_t* ip_tmp, uint32_t* a_out, int L, int nst, int qa_base,
                       const DevTables* T, hipStream_t s) {
-    hipLaunchKernelGGL(chain_kernel<false>, dim3(LANES, nst), dim3(512), 0, s, te_tmp, ip_tmp, a_out,
+    hipLaunchKernelGGL(chain_kernel<false>, dim3(LANES, nst), dim3(1024), 0, s, te_tmp, ip_tmp, a_out,
                        (uint32_t*)nullptr, (const uint32_t*)nullptr, (const int*)nullptr, L, LANES, qa_base, T);
 }
 
 void launch_chain_sec(const uint32_t* te_tmp, uint32_t* a_out, uint32_t* c_out, const uint32_t* qs, const int* lane_slot,
                       int L, int nst, int nlanes, const DevTables* T, hipStream_t s) {
-    hipLaunchKernelGGL(chain_kernel<true>, dim3(nlanes, nst), dim3(512), 0, s, te_tmp, (uint32_t*)nullptr, a_out, c_out,
+    hipLaunchKernelGGL(chain_kernel<true>, dim3(nlanes, nst), dim3(1024), 0, s, te_tmp, (uint32_t*)nullptr, a_out, c_out,
                        qs, lane_slot, L, nlanes, 2, T);
 }
 

@@ -75,6 +75,16 @@ class PIRDatabase:
             raise DnspirError(-5, "set_db: entries must be [n][2048]")
         check(lib().dnspir_shard_set_db(self.h, _ptr(e, u16p), e.shape[0]), "set_db")
 
+    def set_db_device(self, entries_dev):
+        """entries_dev: a [n][2048] 16-bit torch tensor on this GPU."""
+        if entries_dev.dim() != 2 or entries_dev.shape[1] != N or entries_dev.element_size() != 2:
+            raise DnspirError(-5, "set_db_device: entries must be a [n][2048] 16-bit tensor")
+        import torch
+        e = entries_dev.contiguous()
+        # the library works on its own streams: finish the producer's work first
+        torch.cuda.current_stream().synchronize()
+        check(lib().dnspir_shard_set_db_device(self.h, ctypes.c_void_p(e.data_ptr()), e.shape[0]), "set_db_device")
+
     def get_a(self):
         out = np.zeros((LANES, N), np.uint64)
         check(lib().dnspir_shard_get_a(self.h, _ptr(out, u64p)), "get_a")
@@ -161,10 +171,13 @@ def process_queries(fleet0, fleet1, query0, query1=None, ctx=None):
 
 def process_queries_device(fleet0, fleet1, query0_dev, query1_dev, reply0_dev, reply1_dev, ctx=None):
     """Same with query / reply bytes in device memory (torch uint8 tensors)."""
+    import torch
     init()
     f0, l0 = _fleet(fleet0)
     f1, l1 = _fleet(fleet1)
     tm = Timing()
+    # query bytes may come from torch's stream; the engine runs on the context's stream
+    torch.cuda.current_stream().synchronize()
     dp = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None  # noqa: E731
     check(lib().dnspir_process_queries_device(ctx.h if ctx else None, f0, l0, f1, l1,
                                               dp(query0_dev), query0_dev.numel() if query0_dev is not None else 0,

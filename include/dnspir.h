@@ -45,6 +45,8 @@ typedef struct {
     double secondary_us;     /* device time of the secondary phase */
     double total_us;         /* wall time of the call */
     uint64_t bytes_streamed; /* bytes read by the primary inner product */
+    double k1_us;            /* device time of the primary inner-product launches */
+    int k1_launches;         /* number of those launches */
 } dnspir_timing;
 
 /* Initialise the library on HIP device `device` (builds the NTT / Galois
@@ -68,6 +70,10 @@ int dnspir_shard_group_order(const dnspir_shard* shard);
 /* PIRDatabase::set_db (base_pir.rs:632-637): exactly 8 * group_order entries,
  * host memory [n_entries][N]. */
 int dnspir_shard_set_db(dnspir_shard* shard, const uint16_t* entries, size_t n_entries);
+/* Same with the entries already in device memory (no host copy). The call
+ * runs on the library's own stream: work that produces `entries_dev` on other
+ * streams must be complete before the call. */
+int dnspir_shard_set_db_device(dnspir_shard* shard, const uint16_t* entries_dev, size_t n_entries);
 /* PIRDatabase::get_a (base_pir.rs:897-903): out [8][N] host */
 int dnspir_shard_get_a(const dnspir_shard* shard, uint64_t* out);
 /* PIRDatabase::perform_pir (base_pir.rs:945-952): qb [N], qb_conj [N] (NULL
@@ -105,7 +111,8 @@ int dnspir_process_queries(dnspir_ctx* ctx, const dnspir_shard* const* fleet0, s
                            const uint8_t* query1, size_t qlen1, uint8_t* reply0, uint8_t* reply1,
                            dnspir_timing* timing);
 /* Same with query and reply bytes in device memory on the context's stream
- * (no host copies; the call returns after the stream has finished). */
+ * (no host copies; the call returns after the stream has finished). Work on
+ * other streams that writes query0_dev/query1_dev must be complete first. */
 int dnspir_process_queries_device(dnspir_ctx* ctx, const dnspir_shard* const* fleet0, size_t len0,
                                   const dnspir_shard* const* fleet1, size_t len1, const uint8_t* query0_dev,
                                   size_t qlen0, const uint8_t* query1_dev, size_t qlen1, uint8_t* reply0_dev,

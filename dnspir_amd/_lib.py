@@ -51,6 +51,7 @@ ip = ctypes.POINTER(ctypes.c_int)
 SIGNATURES = {
     "dnspir_init": (ci, [ci]),
     "dnspir_strerror": (ctypes.c_char_p, [ci]),
+    "dnspir_last_error": (ctypes.c_char_p, []),
     "dnspir_required_bytes": (sz, [ci]),
     "dnspir_shard_create": (ci, [ci, ci, vp, sz, ctypes.POINTER(vp)]),
     "dnspir_shard_destroy": (ci, [vp]),
@@ -66,6 +67,8 @@ SIGNATURES = {
     "dnspir_double_pir": (ci, [vp, vp, sz, vp, sz, u64p, u64p, u64p, u64p, ctypes.POINTER(Timing)]),
     "dnspir_process_queries": (ci, [vp, vp, sz, vp, sz, u8p, sz, u8p, sz, u8p, u8p, ctypes.POINTER(Timing)]),
     "dnspir_process_queries_device": (ci, [vp, vp, sz, vp, sz, vp, sz, vp, sz, vp, vp, ctypes.POINTER(Timing)]),
+    "dnspir_process_batch": (ci, [vp, vp, sz, u8p, sz, ci, u8p, ctypes.POINTER(Timing)]),
+    "dnspir_process_batch_device": (ci, [vp, vp, sz, vp, sz, ci, vp, ctypes.POINTER(Timing)]),
     "dnspir_database_shape": (None, [sz, ip, ip]),
     "dnspir_num_databases": (sz, [sz]),
     "dnspir_choose_index_group_secondary": (ci, [sz]),
@@ -97,5 +100,7 @@ def lib():
 
 def check(rc, where):
     if rc != 0:
+        if rc == -8:
+            where = f"{where} [{lib().dnspir_last_error().decode()}]"
         raise DnspirError(rc, where)
     return rc

@@ -3,6 +3,8 @@
 // plus the host-side client operations.
 #include <hip/hip_runtime.h>
 
+#include <stdio.h>
+
 #include <algorithm>
 #include <chrono>
 #include <map>
@@ -24,9 +26,15 @@ constexpr size_t REPLY_BYTES = 3 * (N * A_BITS / 8 + N * B_BITS / 8);
 constexpr size_t EXTRA_BYTES = (size_t)LANES * N * 8 + (size_t)LANES * N * 4;  // a_coef + a17
 constexpr size_t PTR_SLOTS = 2 * 2 * 256 + 1;  // two fleets x (slabs, a17s) + one scratch slot
 
-#define HIP_OK(x)                                     \
-    do {                                              \
-        if ((x) != hipSuccess) return DNSPIR_E_HIP;   \
+char g_last_error[256] = "";
+int hip_fail(hipError_t e, int line) {
+    snprintf(g_last_error, sizeof(g_last_error), "api.cpp:%d: %s", line, hipGetErrorString(e));
+    return DNSPIR_E_HIP;
+}
+#define HIP_OK(x)                                         \
+    do {                                                  \
+        hipError_t e_ = (x);                              \
+        if (e_ != hipSuccess) return hip_fail(e_, __LINE__); \
     } while (0)
 
 int loop_len(int l) { return l < N ? l : N / 2; }
@@ -184,12 +192,12 @@ struct dnspir_shard {
 struct dnspir_ctx {
     hipStream_t stream = nullptr;
     bool own = false;
-    DevBuf se, M, te, aout, cout, qcoef, qs, qp, acc, acoef, bcoef, a17o, b12o, ptrs, idx, qbytes, rbytes;
+    DevBuf se, M, te, aout, cout, qcoef, qs, qp, acc, outb, ql, acoef, bcoef, a17o, b12o, ptrs, idx, qbytes, rbytes;
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // start, primary end, end, k1 start, k1 end
     int k1_launches = 0;
     std::vector<const dnspir_shard*> cached[2];
     std::vector<uint64_t> stage;
-    int h_idx[18];  // host staging of the lane lists (outlives the async copy)
+    std::vector<long long> h_idx;  // host staging of the lane lists (outlives the async copy)
 };
 
 namespace {
@@ -250,51 +258,95 @@ int check_fleet(const dnspir_shard* const* fleet, size_t len, int l_p) {
     return 0;
 }
 
-// Runs perform_batched_double_pir on ring-element queries already in
-// c->qcoef ([2][6][N]); leaves the reply in c->acoef/bcoef/a17o/b12o with
-// lane index li = (#present slots before s)*3 + part.
-int run_double(dnspir_ctx* c, const dnspir_shard* const* fleets[2], const size_t lens[2], const bool present[2],
-               dnspir_timing* tm) {
-    const hipStream_t s = c->stream;
-    const int l_p = fleets[0][0]->l;
+// Lane lists of the secondary phase: lane index li -> entry offset, number
+// of nonzero columns, query slot. Uploaded as [ent_off (ll) | ncols (int) | slot (int)].
+int upload_lanes(dnspir_ctx* c, const std::vector<long long>& off, const std::vector<int>& ncols,
+                 const std::vector<int>& slot, const long long** d_off, const int** d_ncols, const int** d_slot) {
+    const size_t n = off.size();
+    int rc = c->idx.ensure(n * (sizeof(long long) + 2 * sizeof(int)) + 64);
+    if (rc) return rc;
+    c->h_idx.assign(2 * n, 0);
+    memcpy(c->h_idx.data(), off.data(), n * sizeof(long long));
+    int* ip = reinterpret_cast<int*>(c->h_idx.data() + n);
+    memcpy(ip, ncols.data(), n * sizeof(int));
+    memcpy(ip + n, slot.data(), n * sizeof(int));
+    HIP_OK(hipMemcpyAsync(c->idx.p, c->h_idx.data(), 2 * n * sizeof(long long), hipMemcpyHostToDevice, c->stream));
+    *d_off = c->idx.as<long long>();
+    *d_ncols = reinterpret_cast<const int*>(c->idx.as<long long>() + n);
+    *d_slot = *d_ncols + n;
+    return 0;
+}
+
+// Primary phase of one fleet for nq queries (qp [q][4][2][N]); leaves the
+// per-(query, shard) results for the epilogue in c->outb (limb-block shards)
+// or c->acc (rotated shards, nq = 1).
+int run_primary(dnspir_ctx* c, const uint32_t* const* slabs, int len, int l_p, const uint32_t* qp, int nq, bool* used_outb) {
     const int L_p = loop_len(l_p), nst_p = nstreams(l_p);
-    const int l_s = dnspir_choose_index_group_secondary(lens[0]);
-    const int L_s = loop_len(l_s), nst_s = nstreams(l_s);
+    const hipStream_t s = c->stream;
     int rc;
-    size_t maxlen = std::max(present[0] ? lens[0] : 0, present[1] ? lens[1] : 0);
-    if ((rc = c->qs.ensure((size_t)2 * 6 * 2 * N * 4))) return rc;
-    if ((rc = c->qp.ensure((size_t)2 * 4 * 2 * N * 4))) return rc;
-    if ((rc = c->acc.ensure(std::max<size_t>(1, maxlen) * 2 * N * LANES * 8))) return rc;
-    if ((rc = c->se.ensure((size_t)l_s * LANES * N * 2))) return rc;
-    int nlanes = 0;
-    int h_lanes[6], h_ncols[6], h_slot[6];
-    for (int sl = 0; sl < 2; ++sl)
-        if (present[sl])
-            for (int p = 0; p < 3; ++p) {
-                h_lanes[nlanes] = 3 * sl + p;
-                h_ncols[nlanes] = (int)(LANES * lens[sl]);
-                h_slot[nlanes] = sl;
-                ++nlanes;
-            }
+    if (L_p == N / 2) {
+        if ((rc = c->ql.ensure(qlimb_bytes(nq, nst_p)))) return rc;
+        if ((rc = c->outb.ensure((size_t)nq * len * 2 * N * LANES * 4))) return rc;
+        launch_k1b(slabs, len, qp, nq, c->ql.as<uint32_t>(), c->outb.as<uint32_t>(), nst_p, s);
+        *used_outb = true;
+    } else {
+        if (nq != 1) return DNSPIR_E_SHAPE;
+        if ((rc = c->acc.ensure((size_t)len * 2 * N * LANES * 8))) return rc;
+        unsigned long long* acc = c->acc.as<unsigned long long>();
+        launch_fill_u64(acc, (size_t)len * 2 * N * LANES, 0, s);
+        launch_k1(slabs, len, qp, acc, L_p, nst_p, k1_rows(L_p, len, nst_p), s);
+        *used_outb = false;
+    }
+    return 0;
+}
+
+// Secondary phase for `nlanes` lanes whose entries are described by the lane
+// lists; the reply (a, b per lane) lands in c->acoef/bcoef/a17o/b12o.
+int run_secondary(dnspir_ctx* c, const uint16_t* entries, long long col_stride, const long long* d_off, const int* d_ncols,
+                  const int* d_slot, int nlanes, int l_s) {
+    const int L_s = loop_len(l_s), nst_s = nstreams(l_s);
+    const hipStream_t s = c->stream;
+    int rc;
     if ((rc = c->M.ensure((size_t)nlanes * l_s * N * 4))) return rc;
     if ((rc = c->te.ensure((size_t)nst_s * nlanes * L_s * 2 * N * 4))) return rc;
     if ((rc = c->aout.ensure((size_t)nst_s * nlanes * 2 * N * 4))) return rc;
     if ((rc = c->cout.ensure((size_t)nst_s * nlanes * 2 * N * 4))) return rc;
-    if ((rc = c->acoef.ensure((size_t)6 * N * 8))) return rc;
-    if ((rc = c->bcoef.ensure((size_t)6 * N * 8))) return rc;
-    if ((rc = c->a17o.ensure((size_t)6 * N * 4))) return rc;
-    if ((rc = c->b12o.ensure((size_t)6 * N * 4))) return rc;
-    if ((rc = c->idx.ensure(64 * sizeof(int)))) return rc;
-    int* h_idx = c->h_idx;
-    for (int i = 0; i < 6; ++i) {
-        h_idx[i] = i < nlanes ? h_lanes[i] : 0;
-        h_idx[6 + i] = i < nlanes ? h_ncols[i] : 0;
-        h_idx[12 + i] = i < nlanes ? h_slot[i] : 0;
-    }
-    HIP_OK(hipMemcpyAsync(c->idx.p, h_idx, sizeof(c->h_idx), hipMemcpyHostToDevice, s));
-    const int* d_lanes = c->idx.as<int>();
-    const int* d_ncols = d_lanes + 6;
-    const int* d_slot = d_lanes + 12;
+    if ((rc = c->acoef.ensure((size_t)nlanes * N * 8))) return rc;
+    if ((rc = c->bcoef.ensure((size_t)nlanes * N * 8))) return rc;
+    if ((rc = c->a17o.ensure((size_t)nlanes * N * 4))) return rc;
+    if ((rc = c->b12o.ensure((size_t)nlanes * N * 4))) return rc;
+    launch_k5(entries, d_off, col_stride, c->M.as<uint32_t>(), c->te.as<uint32_t>(), l_s, d_ncols, nlanes, g.tables, s);
+    launch_chain_sec(c->te.as<uint32_t>(), c->aout.as<uint32_t>(), c->cout.as<uint32_t>(), c->qs.as<uint32_t>(), d_slot, L_s,
+                     nst_s, nlanes, g.tables, s);
+    launch_chain_out(c->aout.as<uint32_t>(), c->cout.as<uint32_t>(), nst_s, nlanes, c->acoef.as<uint64_t>(),
+                     c->bcoef.as<uint64_t>(), c->a17o.as<uint32_t>(), c->b12o.as<uint32_t>(), g.tables, s);
+    return 0;
+}
+
+// perform_batched_double_pir on ring-element queries already in c->qcoef
+// ([2][6][N]); leaves the reply in c->acoef/bcoef/a17o/b12o with lane index
+// li = (#present slots before s)*3 + part.
+int run_double(dnspir_ctx* c, const dnspir_shard* const* fleets[2], const size_t lens[2], const bool present[2],
+               dnspir_timing* tm) {
+    const hipStream_t s = c->stream;
+    const int l_p = fleets[0][0]->l;
+    const int l_s = dnspir_choose_index_group_secondary(lens[0]);
+    int rc;
+    if ((rc = c->qs.ensure((size_t)2 * 6 * 2 * N * 4))) return rc;
+    if ((rc = c->qp.ensure((size_t)2 * 4 * 2 * N * 4))) return rc;
+    if ((rc = c->se.ensure((size_t)l_s * LANES * N * 2))) return rc;
+    std::vector<long long> off;
+    std::vector<int> ncols, slot;
+    for (int sl = 0; sl < 2; ++sl)
+        if (present[sl])
+            for (int p = 0; p < 3; ++p) {
+                off.push_back((long long)(3 * sl + p) * N);
+                ncols.push_back((int)(LANES * lens[sl]));
+                slot.push_back(sl);
+            }
+    const long long* d_off;
+    const int *d_ncols, *d_slot;
+    if ((rc = upload_lanes(c, off, ncols, slot, &d_off, &d_ncols, &d_slot))) return rc;
     const uint32_t* const* slabs[2] = {nullptr, nullptr};
     const uint32_t* const* a17s[2] = {nullptr, nullptr};
     for (int sl = 0; sl < 2; ++sl)
@@ -303,25 +355,30 @@ int run_double(dnspir_ctx* c, const dnspir_shard* const* fleets[2], const size_t
     HIP_OK(hipEventRecord(c->ev[0], s));
     launch_qprep(c->qcoef.as<uint64_t>(), c->qs.as<uint32_t>(), c->qp.as<uint32_t>(), 2, g.tables, s);
     uint64_t streamed = 0;
+    c->k1_launches = 0;
     for (int sl = 0; sl < 2; ++sl) {
         if (!present[sl]) continue;
         const int len = (int)lens[sl];
-        unsigned long long* acc = c->acc.as<unsigned long long>();
-        launch_fill_u64(acc, (size_t)len * 2 * N * LANES, 0, s);
-        if (sl == 0 || !present[0]) HIP_OK(hipEventRecord(c->ev[3], s));
-        launch_k1(slabs[sl], len, c->qp.as<uint32_t>() + (size_t)sl * 4 * 2 * N, acc, L_p, nst_p, k1_rows(L_p, len, nst_p), s);
-        if (sl == 0 && present[1]) c->k1_launches = 2;
-        else if (sl == 0 || !present[0]) c->k1_launches = 1;
+        if (c->k1_launches == 0) HIP_OK(hipEventRecord(c->ev[3], s));
+        bool ob = false;
+        if ((rc = run_primary(c, slabs[sl], len, l_p, c->qp.as<uint32_t>() + (size_t)sl * 4 * 2 * N, 1, &ob))) return rc;
+        ++c->k1_launches;
         HIP_OK(hipEventRecord(c->ev[4], s));
-        launch_epi_primary(acc, len, a17s[sl], c->se.as<uint16_t>(), nullptr, sl, g.tables, s);
+        EpiLaunch e;
+        if (ob) e.outb = c->outb.as<uint32_t>();
+        else e.acc = c->acc.as<unsigned long long>();
+        e.nshards = len;
+        e.nq = 1;
+        e.a17s = a17s[sl];
+        e.se = c->se.as<uint16_t>() + (size_t)3 * sl * N;  // entry lanes 3*slot + part
+        e.lpr = LANES;
+        e.lane0_mult = 0;
+        launch_epi(e, g.tables, s);
         streamed += (uint64_t)len * slab_bytes(l_p);
     }
     HIP_OK(hipEventRecord(c->ev[1], s));
-    launch_k5(c->se.as<uint16_t>(), c->M.as<uint32_t>(), c->te.as<uint32_t>(), l_s, d_ncols, d_lanes, nlanes, g.tables, s);
-    launch_chain_sec(c->te.as<uint32_t>(), c->aout.as<uint32_t>(), c->cout.as<uint32_t>(), c->qs.as<uint32_t>(), d_slot, L_s,
-                     nst_s, nlanes, g.tables, s);
-    launch_chain_out(c->aout.as<uint32_t>(), c->cout.as<uint32_t>(), nst_s, nlanes, c->acoef.as<uint64_t>(),
-                     c->bcoef.as<uint64_t>(), c->a17o.as<uint32_t>(), c->b12o.as<uint32_t>(), g.tables, s);
+    if ((rc = run_secondary(c, c->se.as<uint16_t>(), (long long)LANES * N, d_off, d_ncols, d_slot, (int)off.size(), l_s)))
+        return rc;
     HIP_OK(hipEventRecord(c->ev[2], s));
     if (tm) tm->bytes_streamed = streamed;
     return 0;
@@ -348,6 +405,8 @@ double now_us() {
 }  // namespace
 
 extern "C" {
+
+const char* dnspir_last_error(void) { return g_last_error; }
 
 const char* dnspir_strerror(int code) {
     switch (code) {
@@ -438,13 +497,14 @@ static int set_db_impl(dnspir_shard* sh, const uint16_t* entries, size_t n_entri
     if (!on_device && (rc = g.pre_entries.ensure(n_entries * N * 2))) return rc;
     if ((rc = g.pre_M.ensure((size_t)LANES * l * N * 4))) return rc;
     if ((rc = g.pre_te.ensure((size_t)nst * LANES * L * 2 * N * 4))) return rc;
-    if ((rc = g.pre_ip.ensure((size_t)nst * LANES * L * 4 * N * 4))) return rc;
+    if ((rc = g.pre_ip.ensure((size_t)nst * LANES * L * 6 * N * 4))) return rc;
     if ((rc = g.pre_aout.ensure((size_t)nst * LANES * 2 * N * 4))) return rc;
-    if ((rc = g.pre_idx.ensure(16 * sizeof(int)))) return rc;
-    int h_idx[16];
+    if ((rc = g.pre_idx.ensure(8 * sizeof(long long) + 8 * sizeof(int)))) return rc;
+    long long h_idx[12];
+    int* h_nc = reinterpret_cast<int*>(h_idx + 8);
     for (int i = 0; i < 8; ++i) {
-        h_idx[i] = i;
-        h_idx[8 + i] = l;
+        h_idx[i] = (long long)i * N;
+        h_nc[i] = l;
     }
     HIP_OK(hipMemcpyAsync(g.pre_idx.p, h_idx, sizeof(h_idx), hipMemcpyHostToDevice, s));
     const uint16_t* ent = entries;
@@ -452,11 +512,11 @@ static int set_db_impl(dnspir_shard* sh, const uint16_t* entries, size_t n_entri
         HIP_OK(hipMemcpyAsync(g.pre_entries.p, entries, n_entries * N * 2, hipMemcpyHostToDevice, s));
         ent = g.pre_entries.as<uint16_t>();
     }
-    launch_k5(ent, g.pre_M.as<uint32_t>(), g.pre_te.as<uint32_t>(), l, g.pre_idx.as<int>() + 8,
-              g.pre_idx.as<int>(), LANES, g.tables, s);
+    launch_k5(ent, g.pre_idx.as<long long>(), (long long)LANES * N, g.pre_M.as<uint32_t>(), g.pre_te.as<uint32_t>(), l,
+              reinterpret_cast<const int*>(g.pre_idx.as<long long>() + 8), LANES, g.tables, s);
     launch_chain_pre(g.pre_te.as<uint32_t>(), g.pre_ip.as<uint32_t>(), g.pre_aout.as<uint32_t>(), L, nst,
                      sh->seeds == 0 ? 0 : 2, g.tables, s);
-    launch_relayout(g.pre_te.as<uint32_t>(), g.pre_ip.as<uint32_t>(), sh->slab, L, nst, g.tables, s);
+    launch_relayout(g.pre_ip.as<uint32_t>(), sh->slab, L, nst, s);
     launch_chain_out(g.pre_aout.as<uint32_t>(), nullptr, nst, LANES, sh->a_coef, nullptr, sh->a17, nullptr, g.tables, s);
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(s));
@@ -509,11 +569,24 @@ static int perform_common(const dnspir_shard* sh, const uint64_t* qb, const uint
     const uint32_t** pslab = c->ptrs.as<const uint32_t*>() + (PTR_SLOTS - 1);  // scratch slot outside the fleet cache
     HIP_OK(hipMemcpyAsync((void*)pslab, &slab, sizeof(slab), hipMemcpyHostToDevice, s));
     launch_qprep(c->qcoef.as<uint64_t>(), c->qs.as<uint32_t>(), c->qp.as<uint32_t>(), nq, g.tables, s);
-    unsigned long long* acc = c->acc.as<unsigned long long>();
-    launch_fill_u64(acc, (size_t)2 * N * LANES, 0, s);
-    if (split) launch_k1_split(pslab, 1, c->qp.as<uint32_t>(), acc, sh->L, sh->nst, k1_rows(sh->L, 1, sh->nst), s);
-    else launch_k1(pslab, 1, c->qp.as<uint32_t>(), acc, sh->L, sh->nst, k1_rows(sh->L, 1, sh->nst), s);
-    launch_epi_primary(acc, 1, nullptr, nullptr, c->bcoef.as<uint64_t>(), 0, g.tables, s);
+    EpiLaunch e;
+    e.nshards = 1;
+    e.bcoef = c->bcoef.as<uint64_t>();
+    if (sh->L == N / 2) {
+        if ((rc = c->ql.ensure(qlimb_bytes(nq, sh->nst)))) return rc;
+        if ((rc = c->outb.ensure((size_t)nq * 2 * N * LANES * 4))) return rc;
+        launch_k1b(pslab, 1, c->qp.as<uint32_t>(), nq, c->ql.as<uint32_t>(), c->outb.as<uint32_t>(), sh->nst, s);
+        e.outb = c->outb.as<uint32_t>();
+        e.nq = nq;
+        e.diag = split ? 1 : 0;
+    } else {
+        unsigned long long* acc = c->acc.as<unsigned long long>();
+        launch_fill_u64(acc, (size_t)2 * N * LANES, 0, s);
+        if (split) launch_k1_split(pslab, 1, c->qp.as<uint32_t>(), acc, sh->L, sh->nst, k1_rows(sh->L, 1, sh->nst), s);
+        else launch_k1(pslab, 1, c->qp.as<uint32_t>(), acc, sh->L, sh->nst, k1_rows(sh->L, 1, sh->nst), s);
+        e.acc = acc;
+    }
+    launch_epi(e, g.tables, s);
     HIP_OK(hipGetLastError());
     HIP_OK(hipMemcpyAsync(out, c->bcoef.p, (size_t)LANES * N * 8, hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));
@@ -609,7 +682,8 @@ int dnspir_double_pir(dnspir_ctx* c, const dnspir_shard* const* f0, size_t len0,
                                   hipMemcpyHostToDevice, c->stream));
     if ((rc = run_double(c, fleets, lens, present, tm))) return rc;
     HIP_OK(hipGetLastError());
-    std::vector<uint64_t> a(6 * N), b(6 * N);
+    const size_t nl = 3 * ((present[0] ? 1 : 0) + (present[1] ? 1 : 0));  // lanes carrying replies
+    std::vector<uint64_t> a(nl * N), b(nl * N);
     HIP_OK(hipMemcpyAsync(a.data(), c->acoef.p, a.size() * 8, hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipMemcpyAsync(b.data(), c->bcoef.p, b.size() * 8, hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
@@ -659,14 +733,14 @@ static int process_device_impl(dnspir_ctx* c, const dnspir_shard* const* f0, siz
             off += dnh::condensed_size(bits[v], N);
         }
         if (ql[sl] < off) return DNSPIR_E_SHORT_QUERY;
-        launch_decode_query(qb[sl], offs, bits, c->qcoef.as<uint64_t>() + (size_t)sl * 6 * N, c->stream);
+        launch_decode_query(qb[sl], 0, 1, offs, bits, c->qcoef.as<uint64_t>() + (size_t)sl * 6 * N, c->stream);
     }
     if ((rc = run_double(c, fleets, lens, present, tm))) return rc;
     uint8_t* rs[2] = {r0, r1};
     int li = 0;
     for (int sl = 0; sl < 2; ++sl) {
         if (!present[sl]) continue;
-        if (rs[sl]) launch_encode_reply(c->a17o.as<uint32_t>(), c->b12o.as<uint32_t>(), li, rs[sl], c->stream);
+        if (rs[sl]) launch_encode_reply(c->a17o.as<uint32_t>(), c->b12o.as<uint32_t>(), li, 1, rs[sl], c->stream);
         li += 3;
     }
     HIP_OK(hipGetLastError());
@@ -717,6 +791,109 @@ int dnspir_process_queries(dnspir_ctx* c, const dnspir_shard* const* f0, size_t 
         int lp = f0[0]->l;
         tm->bytes_streamed = (uint64_t)((q0 ? len0 : 0) + (q1 ? len1 : 0)) * slab_bytes(lp);
     }
+    return 0;
+}
+
+// ---- batched queries -----------------------------------------------------------
+// nq independent queries against one fleet, each answered as process_query
+// (pir_wrapper.rs:328-337) but sharing one pass over the shards: the primary
+// inner product runs as one MFMA contraction per 64 queries; every query then
+// gets its own secondary database (3 lanes each) and all of those run
+// together. Query q's reply bytes equal its single-query reply.
+static int batch_impl(dnspir_ctx* c, const dnspir_shard* const* fleet, size_t len, const uint8_t* qdev, size_t qlen, int nq,
+                      uint8_t* rdev, dnspir_timing* tm, double t0) {
+    int rc;
+    if (!fleet || len == 0 || !fleet[0] || nq <= 0 || !qdev || !rdev) return DNSPIR_E_ARG;
+    const int l_p = fleet[0]->l;
+    if (loop_len(l_p) != N / 2) return DNSPIR_E_SHAPE;
+    if ((rc = check_fleet(fleet, len, l_p))) return rc;
+    const int l_s = dnspir_choose_index_group_secondary(len);
+    const int conj_p = l_p == N, conj_s = l_s == N;
+    int offs[6], bits[6] = {GK_BITS, GK_BITS, QRY_BITS, QRY_BITS, QRY_BITS, QRY_BITS};
+    int pres[6] = {1, 1, 1, conj_p, 1, conj_s};
+    size_t off = 0;
+    for (int v = 0; v < 6; ++v) {
+        offs[v] = pres[v] ? (int)off : -1;
+        if (pres[v]) off += dnh::condensed_size(bits[v], N);
+    }
+    if (qlen < off) return DNSPIR_E_SHORT_QUERY;
+    const hipStream_t s = c->stream;
+    if ((rc = c->qcoef.ensure((size_t)nq * 6 * N * 8))) return rc;
+    if ((rc = c->qs.ensure((size_t)nq * 6 * 2 * N * 4))) return rc;
+    if ((rc = c->qp.ensure((size_t)nq * 4 * 2 * N * 4))) return rc;
+    if ((rc = c->se.ensure((size_t)nq * l_s * 3 * N * 2))) return rc;
+    const uint32_t* const* slabs;
+    const uint32_t* const* a17s;
+    if ((rc = upload_fleet(c, 0, fleet, len, &slabs, &a17s))) return rc;
+    std::vector<long long> loff;
+    std::vector<int> ncols, slot;
+    for (int q = 0; q < nq; ++q)
+        for (int p = 0; p < 3; ++p) {
+            loff.push_back(((long long)q * l_s * 3 + p) * N);
+            ncols.push_back((int)(LANES * len));
+            slot.push_back(q);
+        }
+    const long long* d_off;
+    const int *d_ncols, *d_slot;
+    if ((rc = upload_lanes(c, loff, ncols, slot, &d_off, &d_ncols, &d_slot))) return rc;
+    launch_decode_query(qdev, qlen, nq, offs, bits, c->qcoef.as<uint64_t>(), s);
+    HIP_OK(hipEventRecord(c->ev[0], s));
+    launch_qprep(c->qcoef.as<uint64_t>(), c->qs.as<uint32_t>(), c->qp.as<uint32_t>(), nq, g.tables, s);
+    HIP_OK(hipEventRecord(c->ev[3], s));
+    c->k1_launches = 0;
+    for (int q0 = 0; q0 < nq; q0 += 64) {
+        const int nqc = std::min(64, nq - q0);
+        bool ob = false;
+        if ((rc = run_primary(c, slabs, (int)len, l_p, c->qp.as<uint32_t>() + (size_t)q0 * 4 * 2 * N, nqc, &ob))) return rc;
+        ++c->k1_launches;
+        if (q0 == 0) HIP_OK(hipEventRecord(c->ev[4], s));
+        EpiLaunch e;
+        e.outb = c->outb.as<uint32_t>();
+        e.nshards = (int)len;
+        e.nq = nqc;
+        e.a17s = a17s;
+        e.se = c->se.as<uint16_t>() + (size_t)q0 * l_s * 3 * N;
+        e.se_qstride = (size_t)l_s * 3 * N;
+        e.lpr = 3;
+        e.lane0_mult = 0;
+        launch_epi(e, g.tables, s);
+    }
+    HIP_OK(hipEventRecord(c->ev[1], s));
+    if ((rc = run_secondary(c, c->se.as<uint16_t>(), 3LL * N, d_off, d_ncols, d_slot, 3 * nq, l_s))) return rc;
+    launch_encode_reply(c->a17o.as<uint32_t>(), c->b12o.as<uint32_t>(), 0, nq, rdev, s);
+    HIP_OK(hipEventRecord(c->ev[2], s));
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(s));
+    fill_timing(c, tm, now_us() - t0);
+    if (tm) tm->bytes_streamed = (uint64_t)((nq + 63) / 64) * len * slab_bytes(l_p);
+    return 0;
+}
+
+extern "C" int dnspir_process_batch_device(dnspir_ctx* c, const dnspir_shard* const* fleet, size_t len, const uint8_t* queries_dev,
+                                           size_t qlen, int nq, uint8_t* replies_dev, dnspir_timing* tm) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (!c) c = default_ctx();
+    std::lock_guard<std::mutex> lk(g.mu);
+    return batch_impl(c, fleet, len, queries_dev, qlen, nq, replies_dev, tm, now_us());
+}
+
+extern "C" int dnspir_process_batch(dnspir_ctx* c, const dnspir_shard* const* fleet, size_t len, const uint8_t* queries,
+                                    size_t qlen, int nq, uint8_t* replies, dnspir_timing* tm) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (!c) c = default_ctx();
+    if (!queries || !replies || nq <= 0) return DNSPIR_E_ARG;
+    std::lock_guard<std::mutex> lk(g.mu);
+    const double t0 = now_us();
+    if ((rc = c->qbytes.ensure((size_t)nq * qlen + 16))) return rc;
+    if ((rc = c->rbytes.ensure((size_t)nq * REPLY_BYTES))) return rc;
+    HIP_OK(hipMemcpyAsync(c->qbytes.p, queries, (size_t)nq * qlen, hipMemcpyHostToDevice, c->stream));
+    rc = batch_impl(c, fleet, len, c->qbytes.as<uint8_t>(), qlen, nq, c->rbytes.as<uint8_t>(), nullptr, t0);
+    if (rc) return rc;
+    HIP_OK(hipMemcpy(replies, c->rbytes.p, (size_t)nq * REPLY_BYTES, hipMemcpyDeviceToHost));
+    fill_timing(c, tm, now_us() - t0);
+    if (tm) tm->bytes_streamed = (uint64_t)((nq + 63) / 64) * len * slab_bytes(fleet[0]->l);
     return 0;
 }
 

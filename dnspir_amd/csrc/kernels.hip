@@ -104,19 +104,20 @@ __device__ __forceinline__ int brv_bits(int v, int bits) {
 // K5: database encoding (base_pir.rs:662-772), modulus p1 for steps 1-5
 // ---------------------------------------------------------------------------
 
-// steps 1-2: column j, lane -> NTT of the entry's coefficient vector.
-// entries [j][8][N] u16; M [lane_idx][j][slot]
-__global__ __launch_bounds__(256) void k5_col_ntt_kernel(const uint16_t* __restrict__ entries, uint32_t* __restrict__ M,
-                                                         int l, const int* __restrict__ ncols_nonzero,
-                                                         const int* __restrict__ lanes, const DevTables* __restrict__ T) {
+// steps 1-2: column j of lane index li -> NTT of the entry's coefficient
+// vector; the entry sits at entries + ent_off[li] + j * col_stride (u16 [N]).
+// M [lane_idx][j][slot]
+__global__ __launch_bounds__(256) void k5_col_ntt_kernel(const uint16_t* __restrict__ entries, const long long* __restrict__ ent_off,
+                                                         long long col_stride, uint32_t* __restrict__ M, int l,
+                                                         const int* __restrict__ ncols_nonzero, const DevTables* __restrict__ T) {
     __shared__ uint32_t a[N];
-    const int j = blockIdx.x, li = blockIdx.y, lane = lanes[li];
+    const int j = blockIdx.x, li = blockIdx.y;
     uint32_t* out = M + ((size_t)li * l + j) * N;
     if (j >= ncols_nonzero[li]) {
         for (int s = threadIdx.x; s < N; s += blockDim.x) out[s] = 0;
         return;
     }
-    const uint16_t* src = entries + ((size_t)j * LANES + lane) * N;
+    const uint16_t* src = entries + ent_off[li] + (long long)j * col_stride;
     for (int s = threadIdx.x; s < N; s += blockDim.x) a[s] = src[s];
     __syncthreads();
     ntt_fwd(a, T, 0, threadIdx.x, blockDim.x);
@@ -348,6 +349,7 @@ __global__ __launch_bounds__(1024) void chain_kernel(const uint32_t* __restrict_
     }
     sig_upd[0] = T->sigma5[tid];
     sig_upd[1] = T->sigma5[tid + 1024];
+    int gpos_upd[2] = {T->slot_gpos[tid], T->slot_gpos[tid + 1024]};
     // phase A input gather indices: radix-4 groups g, g + 256 of modulus rA
     const int rA = (tid >> 8) & 1, gA = tid & 255;
     int sigA[8];
@@ -440,9 +442,12 @@ __global__ __launch_bounds__(1024) void chain_kernel(const uint32_t* __restrict_
                                     (uint64_t)te[k] * cqb[k];
                 sc[cur ^ 1][r][s] = red64(cv, p, K);
             } else {
-                uint32_t* dst = ip_tmp + (((size_t)(st * nlanes + li) * L + i) * 4) * N;
-                dst[(0 * 2 + r) * N + s] = d0 >= p ? d0 - p : d0;
-                dst[(1 * 2 + r) * N + s] = d1 >= p ? d1 - p : d1;
+                // IPGK rows and the TE row, Galois order: [arr][r][gpos]
+                uint32_t* dst = ip_tmp + (((size_t)(st * nlanes + li) * L + i) * 6) * N;
+                const int gp = gpos_upd[k >> 1];
+                dst[(0 * 2 + r) * N + gp] = te[k];
+                dst[(1 * 2 + r) * N + gp] = d0 >= p ? d0 - p : d0;
+                dst[(2 * 2 + r) * N + gp] = d1 >= p ? d1 - p : d1;
             }
         }
         cur ^= 1;
@@ -462,29 +467,209 @@ __global__ __launch_bounds__(1024) void chain_kernel(const uint32_t* __restrict_
     }
 }
 
-// lane-major temporaries -> the streaming slab layout (rotated Galois order,
-// 8 lanes interleaved). One block per (row, array, r*nst + st).
-__global__ __launch_bounds__(256) void relayout_kernel(const uint32_t* __restrict__ te_tmp, const uint32_t* __restrict__ ip_tmp,
-                                                       uint32_t* __restrict__ slab, int L, int nst,
-                                                       const DevTables* __restrict__ T) {
+// Shard rows (Galois order, lane-major temporaries rows[st][lane][i][arr][r][gpos]
+// written by the preprocessing chain) -> the slab.
+//
+// Rotated layout (any L): [r][st][i][arr][gpos][lane], row i rotated by
+// L-1-i inside each half; used by k1_kernel.
+__global__ __launch_bounds__(256) void relayout_rot_kernel(const uint32_t* __restrict__ rows_tmp, uint32_t* __restrict__ slab,
+                                                           int L, int nst) {
     __shared__ uint32_t rows[LANES][N];
     const int i = blockIdx.x, arr = blockIdx.y;
     const int r = blockIdx.z / nst, st = blockIdx.z % nst;
     const int tid = threadIdx.x, nt = blockDim.x;
     for (int idx = tid; idx < LANES * N; idx += nt) {
-        const int lane = idx / N, s = idx % N;
-        uint32_t v;
-        if (arr == 0) v = te_tmp[(((size_t)(st * LANES + lane) * L + i) * 2 + r) * N + s];
-        else v = ip_tmp[(((size_t)(st * LANES + lane) * L + i) * 4 + (arr - 1) * 2 + r) * N + s];
-        rows[lane][s] = v;
+        const int lane = idx / N, g = idx % N;
+        rows[lane][g] = rows_tmp[(((size_t)(st * LANES + lane) * L + i) * 6 + arr * 2 + r) * N + g];
     }
     __syncthreads();
     uint32_t* out = slab + ((((size_t)(r * nst + st) * L + i) * 3 + arr) * N) * LANES;
     for (int o = tid; o < N * LANES; o += nt) {
         const int g = o >> 3, lane = o & 7;
         const int hb = g & ~(HALFN - 1);
-        const int rot = hb + (((g & (HALFN - 1)) + L - 1 - i) & (HALFN - 1));
-        out[o] = rows[lane][T->gpos_slot[rot]];
+        out[o] = rows[lane][hb + (((g & (HALFN - 1)) + L - 1 - i) & (HALFN - 1))];
+    }
+}
+
+// Output-major limb-block layout (L = N/2), read by k1b_kernel. Logically
+// M2[r][half][a][s][j][lane] = row i = (a - j + L - 1) mod N/2 at Galois
+// position (half, j): the closed form of main_pir_loop as a plain matrix,
+// out[a] = sum_(s,j) M2[a][s][j] Q_s[j] (s = stream*3 + array).
+// Stored as 4 KiB blocks, one per (r, half, a-group g of 4, s, j-chunk c of 32):
+// byte v*1024 + l*16 + kk = signed base-256 digit v of x (balanced digits,
+// (x + 0x808080) ^ 0x808080), lane l = h*32 + (a%4)*8 + lane8, j = 32c + 16h + kk:
+// exactly the operand a 64-lane wave feeds to v_mfma_i32_32x32x32_i8.
+__device__ __forceinline__ uint32_t limbs8(uint32_t x) { return (x + 0x00808080u) ^ 0x00808080u; }
+
+__global__ __launch_bounds__(256) void relayout_mb_kernel(const uint32_t* __restrict__ rows_tmp, uint32_t* __restrict__ slab,
+                                                          int nst) {
+    // tile of 63 rows x 8 lanes x 32 positions covering a in [a0, a0+32), j in [j0, j0+32)
+    __shared__ uint32_t tile[63][LANES][32];
+    const int S = 3 * nst;
+    const int c = blockIdx.x & 31, at = blockIdx.x >> 5;  // j-chunk, a-tile (32 a's)
+    const int half = blockIdx.y & 1, r = blockIdx.y >> 1;
+    const int s = blockIdx.z;
+    const int st = s / 3, arr = s % 3;
+    const int a0 = at * 32, j0 = c * 32;
+    const int ibase = (a0 - j0 - 31 + (HALFN - 1)) & (HALFN - 1);  // row of (a0, j0 + 31)
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < 63 * LANES * 32; idx += 256) {
+        const int jl = idx & 31, lane = (idx >> 5) & 7, ri = idx >> 8;
+        const int i = (ibase + ri) & (HALFN - 1);
+        tile[ri][lane][jl] = rows_tmp[(((size_t)(st * LANES + lane) * HALFN + i) * 6 + arr * 2 + r) * N + half * HALFN + j0 + jl];
+    }
+    __syncthreads();
+    // 8 blocks (a-groups) x 64 lanes; each item writes the 4 limb chunks of one lane
+    for (int it = tid; it < 8 * 64; it += 256) {
+        const int gi = it >> 6, l = it & 63;
+        const int h = l >> 5, ai = (l >> 3) & 3, lane8 = l & 7;
+        const int a = a0 + gi * 4 + ai;
+        uint32_t w[16];
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk) {
+            const int jl = 16 * h + kk;
+            // row i = a - j + L-1  ->  tile row (a - a0) - jl + 31
+            w[kk] = limbs8(tile[(a - a0) - jl + 31][lane8][jl]);
+        }
+        const int g = a0 / 4 + gi;
+        uint32_t* blk = slab + ((((size_t)(r * 2 + half) * (HALFN / 4) + g) * S + s) * 32 + c) * 1024;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            u32x4 o;
+            o.x = ((w[0] >> (8 * v)) & 255) | (((w[1] >> (8 * v)) & 255) << 8) | (((w[2] >> (8 * v)) & 255) << 16) | (((w[3] >> (8 * v)) & 255) << 24);
+            o.y = ((w[4] >> (8 * v)) & 255) | (((w[5] >> (8 * v)) & 255) << 8) | (((w[6] >> (8 * v)) & 255) << 16) | (((w[7] >> (8 * v)) & 255) << 24);
+            o.z = ((w[8] >> (8 * v)) & 255) | (((w[9] >> (8 * v)) & 255) << 8) | (((w[10] >> (8 * v)) & 255) << 16) | (((w[11] >> (8 * v)) & 255) << 24);
+            o.w = ((w[12] >> (8 * v)) & 255) | (((w[13] >> (8 * v)) & 255) << 8) | (((w[14] >> (8 * v)) & 255) << 16) | (((w[15] >> (8 * v)) & 255) << 24);
+            *reinterpret_cast<u32x4*>(blk + v * 256 + l * 4) = o;
+        }
+    }
+}
+
+// Query vectors in Galois order (qp [q][4][r][gpos]) -> B operands of
+// k1b_kernel: ql[r][half][s][c][ct][v][l] (16 bytes each): limb v of
+// Q_b[s][j], b = 32 ct + l % 32, j = 32c + 16 (l / 32) + kk; zero for b >= nq.
+__global__ __launch_bounds__(256) void qlimb_kernel(const uint32_t* __restrict__ qp, uint32_t* __restrict__ ql, int nq, int nst,
+                                                    int CT) {
+    const int S = 3 * nst;
+    const int c = blockIdx.x, sidx = blockIdx.y;  // sidx = (r*2 + half)*S + s
+    const int s = sidx % S, rh = sidx / S;
+    const int half = rh & 1, r = rh >> 1;
+    const int st = s / 3, arr = s % 3;
+    const int vq = arr == 0 ? 2 + st : arr - 1;
+    for (int it = threadIdx.x; it < CT * 64; it += blockDim.x) {
+        const int ct = it >> 6, l = it & 63;
+        const int b = 32 * ct + (l & 31), h = l >> 5;
+        uint32_t w[16];
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk) {
+            const int j = 32 * c + 16 * h + kk;
+            w[kk] = b < nq ? limbs8(qp[(((size_t)b * 4 + vq) * 2 + r) * N + half * HALFN + j]) : 0;
+        }
+        uint32_t* dst = ql + ((((size_t)sidx * 32 + c) * CT + ct) * 4) * 256;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            u32x4 o;
+            o.x = ((w[0] >> (8 * v)) & 255) | (((w[1] >> (8 * v)) & 255) << 8) | (((w[2] >> (8 * v)) & 255) << 16) | (((w[3] >> (8 * v)) & 255) << 24);
+            o.y = ((w[4] >> (8 * v)) & 255) | (((w[5] >> (8 * v)) & 255) << 8) | (((w[6] >> (8 * v)) & 255) << 16) | (((w[7] >> (8 * v)) & 255) << 24);
+            o.z = ((w[8] >> (8 * v)) & 255) | (((w[9] >> (8 * v)) & 255) << 8) | (((w[10] >> (8 * v)) & 255) << 16) | (((w[11] >> (8 * v)) & 255) << 24);
+            o.w = ((w[12] >> (8 * v)) & 255) | (((w[13] >> (8 * v)) & 255) << 8) | (((w[14] >> (8 * v)) & 255) << 16) | (((w[15] >> (8 * v)) & 255) << 24);
+            *reinterpret_cast<u32x4*>(dst + v * 256 + l * 4) = o;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K1 (single, split and batched queries) on the limb-block layout: for each
+// (shard, r, half) the product OUT[(a, lane)][b] = sum_(s,j) M2[(a,lane)][(s,j)] Q_b[(s,j)]
+// on int8 MFMA with exact limb decomposition: both operands are split into 4
+// signed base-256 digits (|digit| <= 128), the 16 digit products are summed in
+// int32 per weight w = u + v (|sum| <= 4 * 6144 * 2^14 < 2^29), and
+// OUT = sum_w 2^(8w) ACC_w mod p. One wave per a-group (32 output rows),
+// CT column tiles of 32 queries, A blocks prefetched PD K-steps ahead, B tiles
+// shared through LDS (double buffered). Writes outb[q][shard][r][gpos][lane].
+// ---------------------------------------------------------------------------
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+template <int CT, int PD>
+__global__ __launch_bounds__(256) void k1b_kernel(const uint32_t* const* __restrict__ slabs, const v4i* __restrict__ ql,
+                                                  uint32_t* __restrict__ outb, int nst, int nq, size_t q_stride) {
+    __shared__ v4i Bt[2][CT][4][64];
+    const int S = 3 * nst;
+    const int half = blockIdx.y;
+    const int r = blockIdx.z & 1, sh = blockIdx.z >> 1;
+    const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
+    const int g = blockIdx.x * 4 + wave;
+    const v4i* A = reinterpret_cast<const v4i*>(slabs[sh]) + (((size_t)(r * 2 + half) * (HALFN / 4) + g) * S) * 32 * 256;
+    const v4i* Q = ql + ((size_t)(r * 2 + half) * S) * 32 * CT * 256;
+    const int nk = S * 32;
+    v16i acc[7][CT];
+#pragma unroll
+    for (int w = 0; w < 7; ++w)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+            for (int e = 0; e < 16; ++e) acc[w][ct][e] = 0;
+    v4i abuf[PD][4];
+#pragma unroll
+    for (int d = 0; d < PD; ++d)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) abuf[d][v] = __builtin_nontemporal_load(A + (size_t)d * 256 + v * 64 + l);
+    for (int it = tid; it < CT * 256; it += 256) (&Bt[0][0][0][0])[it] = Q[it];
+    for (int kk0 = 0; kk0 < nk; kk0 += PD) {
+#pragma unroll
+        for (int d = 0; d < PD; ++d) {
+            const int kk = kk0 + d;
+            const int cur = d & 1;
+            __syncthreads();
+            if (kk + 1 < nk)
+                for (int it = tid; it < CT * 256; it += 256)
+                    (&Bt[cur ^ 1][0][0][0])[it] = Q[(size_t)(kk + 1) * CT * 256 + it];
+            v4i a[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) a[v] = abuf[d][v];
+            if (kk + PD < nk) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) abuf[d][v] = __builtin_nontemporal_load(A + (size_t)(kk + PD) * 256 + v * 64 + l);
+            }
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct) {
+                v4i b[4];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) b[v] = Bt[cur][ct][v][l];
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v)
+                        acc[u + v][ct] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[u], b[v], acc[u + v][ct], 0, 0, 0);
+            }
+        }
+    }
+    const uint32_t p = mod_p(r);
+    const int K = mod_k(r);
+    uint32_t cw[7];
+    {
+        uint64_t x = 1;
+#pragma unroll
+        for (int w = 0; w < 7; ++w) {
+            cw[w] = (uint32_t)(x % p);
+            x = (x % p) * 256;
+        }
+    }
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct) {
+        const int q = ct * 32 + (l & 31);
+        if (q >= nq) continue;
+        uint32_t* o = outb + (size_t)q * q_stride + ((size_t)sh * 2 + r) * N * LANES;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            int64_t t = 0;
+#pragma unroll
+            for (int w = 0; w < 7; ++w) t += (int64_t)acc[w][ct][e] * cw[w];
+            const int row = 8 * (e >> 2) + 4 * (l >> 5) + (e & 3);
+            const int a = g * 4 + (row >> 3), lane8 = row & 7;
+            o[(half * HALFN + a) * LANES + lane8] = from_i64(t, p, K);
+        }
     }
 }
 
@@ -634,22 +819,38 @@ __global__ __launch_bounds__(256) void k1_split_kernel(const uint32_t* const* __
     for (int k = 0; k < 4; ++k) atomicAdd(o + k, (unsigned long long)red64(a[k], p, K));
 }
 
-// primary epilogue per (shard, lane): reduce, Galois order -> slot order,
-// inverse NTT, CRT (from_simd_element, base_pir.rs:905-935); then either the
-// b coefficients (perform_pir) or, with the shard's 17-bit a-part, the three
-// base-1025 digits of a*2^12 + b written as secondary entries
-// (double_pir.rs:157-177, 70-81): column shard*8 + lane, lane 3*slot + part.
-__global__ __launch_bounds__(256) void epi_primary_kernel(const unsigned long long* __restrict__ acc,
-                                                          const uint32_t* const* __restrict__ a17s, uint16_t* __restrict__ se,
-                                                          uint64_t* __restrict__ bcoef, int slot,
-                                                          const DevTables* __restrict__ T) {
+// primary epilogue per (query, shard, lane): reduce, Galois order -> slot
+// order, inverse NTT, CRT (from_simd_element, base_pir.rs:905-935); then
+// either the b coefficients (perform_pir) or, with the shard's 17-bit a-part,
+// the three base-1025 digits of a*2^12 + b written as secondary entries
+// (double_pir.rs:157-177, 70-81): column shard*8 + lane, entry lane
+// lane0_mult*q + part, at se + q*se_qstride + (col*lpr + lane)*N.
+struct EpiArgs {
+    const uint32_t* outb;             // [q][shard][r][gpos][lane] reduced (k1b), or null
+    const unsigned long long* acc;    // [shard][r][gpos][lane] partial sums (k1), or null
+    size_t q_stride;                  // elements per query in outb
+    const uint32_t* const* a17s;      // per shard: [lane][N]
+    uint16_t* se;                     // secondary entries, or null
+    size_t se_qstride;
+    int lpr, lane0_mult;
+    uint64_t* bcoef;                  // [shard][lane][N] (query index = lane when diag), or null
+    int diag;                         // query q = lane (perform_pir_split)
+};
+__global__ __launch_bounds__(256) void epi_primary_kernel(EpiArgs A, const DevTables* __restrict__ T) {
     __shared__ uint32_t w[2][N];
     const int sh = blockIdx.x, lane = blockIdx.y;
+    const int q = A.diag ? lane : blockIdx.z;
     const int tid = threadIdx.x, nt = blockDim.x;
     for (int s = tid; s < N; s += nt) {
         const int g = T->slot_gpos[s];
-        w[0][s] = red64(acc[(((size_t)sh * 2 + 0) * N + g) * LANES + lane], P1, 26);
-        w[1][s] = red64(acc[(((size_t)sh * 2 + 1) * N + g) * LANES + lane], P2, 25);
+        if (A.outb) {
+            const uint32_t* o = A.outb + (size_t)q * A.q_stride + (size_t)sh * 2 * N * LANES;
+            w[0][s] = o[(size_t)g * LANES + lane];
+            w[1][s] = o[((size_t)N + g) * LANES + lane];
+        } else {
+            w[0][s] = red64(A.acc[(((size_t)sh * 2 + 0) * N + g) * LANES + lane], P1, 26);
+            w[1][s] = red64(A.acc[(((size_t)sh * 2 + 1) * N + g) * LANES + lane], P2, 25);
+        }
     }
     __syncthreads();
     {
@@ -658,13 +859,13 @@ __global__ __launch_bounds__(256) void epi_primary_kernel(const unsigned long lo
     }
     for (int s = tid; s < N; s += nt) {
         const uint64_t b = crt(w[0][s], w[1][s]);
-        if (bcoef) bcoef[((size_t)sh * LANES + lane) * N + s] = b;
-        if (se) {
+        if (A.bcoef) A.bcoef[((size_t)sh * LANES + lane) * N + s] = b;
+        if (A.se) {
             const uint32_t b12 = mod_switch_enc(b, 12);
-            const uint32_t a17 = a17s[sh][(size_t)lane * N + s];
+            const uint32_t a17 = A.a17s[sh][(size_t)lane * N + s];
             uint32_t v = (a17 << 12) + b12;
             const int col = sh * LANES + lane;
-            uint16_t* dst = se + ((size_t)col * LANES + 3 * slot) * N + s;
+            uint16_t* dst = A.se + (size_t)q * A.se_qstride + ((size_t)col * A.lpr + A.lane0_mult * q) * N + s;
             dst[0] = (uint16_t)(v % T_PLAIN);
             v /= T_PLAIN;
             dst[N] = (uint16_t)(v % T_PLAIN);
@@ -694,8 +895,8 @@ void launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s) {
     hipLaunchKernelGGL(fill_u32_kernel, dim3(blocks), dim3(256), 0, s, p, n, v);
 }
 
-void launch_k5(const uint16_t* entries, uint32_t* M, uint32_t* te_tmp, int l, const int* ncols_nonzero, const int* lanes,
-               int nlanes, const DevTables* T, hipStream_t s) {
+void launch_k5(const uint16_t* entries, const long long* ent_off, long long col_stride, uint32_t* M, uint32_t* te_tmp, int l,
+               const int* ncols_nonzero, int nlanes, const DevTables* T, hipStream_t s) {
     const int L = l < N ? l : N / 2;
     const int conj = l == N;
     const int kq = N / 2 / L;
@@ -704,7 +905,7 @@ void launch_k5(const uint16_t* entries, uint32_t* M, uint32_t* te_tmp, int l, co
     int scale = 1;
     for (int x = 1; x < (int)T_PLAIN; ++x)
         if ((x * l) % (int)T_PLAIN == 1) { scale = x; break; }
-    hipLaunchKernelGGL(k5_col_ntt_kernel, dim3(l, nlanes), dim3(256), 0, s, entries, M, l, ncols_nonzero, lanes, T);
+    hipLaunchKernelGGL(k5_col_ntt_kernel, dim3(l, nlanes), dim3(256), 0, s, entries, ent_off, col_stride, M, l, ncols_nonzero, T);
     hipLaunchKernelGGL(k5_rows_kernel, dim3(N / K5_TS, nlanes), dim3(256), (size_t)l * K5_TS * 4, s, M, l, logl, kq, conj, T);
     hipLaunchKernelGGL(k5_cols_final_kernel, dim3(l, nlanes), dim3(256), 0, s, M, te_tmp, l, L, conj, nlanes, scale, T);
 }
@@ -721,9 +922,123 @@ void launch_chain_sec(const uint32_t* te_tmp, uint32_t* a_out, uint32_t* c_out, 
                        qs, lane_slot, L, nlanes, 2, T);
 }
 
-void launch_relayout(const uint32_t* te_tmp, const uint32_t* ip_tmp, uint32_t* slab, int L, int nst, const DevTables* T,
-                     hipStream_t s) {
-    hipLaunchKernelGGL(relayout_kernel, dim3(L, 3, 2 * nst), dim3(256), 0, s, te_tmp, ip_tmp, slab, L, nst, T);
+// K1 for one or two queries on the limb-block layout: the same exact limb
+// arithmetic as k1b_kernel, on the vector ALU (v_dot4_i32_i8 over 4 digit
+// pairs at a time) so that many waves per CU keep enough loads in flight to
+// stream at HBM rate. Lane l of a wave owns output row l % 32 and k-half l / 32
+// of its a-group; the two halves are summed at the end.
+template <int NQ, int PD>
+__global__ __launch_bounds__(256) void k1d_kernel(const uint32_t* const* __restrict__ slabs, const uint32_t* __restrict__ ql,
+                                                  uint32_t* __restrict__ outb, int nst, size_t q_stride) {
+    // query digits for this (r, half): [s*32 + c][q][v][h] x 16 bytes
+    __shared__ u32x4 qs[6 * 32][NQ][4][2];
+    const int S = 3 * nst;
+    const int half = blockIdx.y;
+    const int r = blockIdx.z & 1, sh = blockIdx.z >> 1;
+    const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63, h = l >> 5;
+    const int CT = 1;  // ql built for <= 32 queries
+    const u32x4* Q = reinterpret_cast<const u32x4*>(ql) + ((size_t)(r * 2 + half) * S) * 32 * CT * 256;
+    for (int it = tid; it < S * 32 * NQ * 4 * 2; it += 256) {
+        const int hh = it & 1, v = (it >> 1) & 3, q = (it >> 3) % NQ, kc = it / (8 * NQ);
+        qs[kc][q][v][hh] = Q[(size_t)kc * CT * 256 + v * 64 + hh * 32 + q];
+    }
+    __syncthreads();
+    const int g = blockIdx.x * 4 + wave;
+    const u32x4* A = reinterpret_cast<const u32x4*>(slabs[sh]) + (((size_t)(r * 2 + half) * (HALFN / 4) + g) * S) * 32 * 256;
+    const int nk = S * 32;
+    int acc[NQ][7];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+        for (int w = 0; w < 7; ++w) acc[q][w] = 0;
+    u32x4 abuf[PD][4];
+#pragma unroll
+    for (int d = 0; d < PD; ++d)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) abuf[d][v] = __builtin_nontemporal_load(A + (size_t)d * 256 + v * 64 + l);
+    for (int kk0 = 0; kk0 < nk; kk0 += PD) {
+#pragma unroll
+        for (int d = 0; d < PD; ++d) {
+            const int kk = kk0 + d;
+            u32x4 a[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) a[v] = abuf[d][v];
+            if (kk + PD < nk) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) abuf[d][v] = __builtin_nontemporal_load(A + (size_t)(kk + PD) * 256 + v * 64 + l);
+            }
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                u32x4 b[4];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) b[v] = qs[kk][q][v][h];
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        int t = acc[q][u + v];
+                        t = __builtin_amdgcn_sdot4((int)a[u].x, (int)b[v].x, t, false);
+                        t = __builtin_amdgcn_sdot4((int)a[u].y, (int)b[v].y, t, false);
+                        t = __builtin_amdgcn_sdot4((int)a[u].z, (int)b[v].z, t, false);
+                        t = __builtin_amdgcn_sdot4((int)a[u].w, (int)b[v].w, t, false);
+                        acc[q][u + v] = t;
+                    }
+            }
+        }
+    }
+    const uint32_t p = mod_p(r);
+    const int K = mod_k(r);
+    uint32_t cw[7];
+    {
+        uint64_t x = 1;
+#pragma unroll
+        for (int w = 0; w < 7; ++w) {
+            cw[w] = (uint32_t)(x % p);
+            x = (x % p) * 256;
+        }
+    }
+    const int row = l & 31;
+    const int a = g * 4 + (row >> 3), lane8 = row & 7;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        int64_t t = 0;
+#pragma unroll
+        for (int w = 0; w < 7; ++w) {
+            const int tot = acc[q][w] + __shfl_xor(acc[q][w], 32);
+            t += (int64_t)tot * cw[w];
+        }
+        if (h == 0) outb[(size_t)q * q_stride + (((size_t)sh * 2 + r) * N + half * HALFN + a) * LANES + lane8] = from_i64(t, p, K);
+    }
+}
+
+void launch_relayout(const uint32_t* rows_tmp, uint32_t* slab, int L, int nst, hipStream_t s) {
+    if (L == HALFN)
+        hipLaunchKernelGGL(relayout_mb_kernel, dim3(32 * 32, 4, 3 * nst), dim3(256), 0, s, rows_tmp, slab, nst);
+    else
+        hipLaunchKernelGGL(relayout_rot_kernel, dim3(L, 3, 2 * nst), dim3(256), 0, s, rows_tmp, slab, L, nst);
+}
+
+size_t qlimb_bytes(int nq, int nst) {
+    const int CT = nq > 32 ? 2 : 1;
+    return (size_t)2 * 2 * 3 * nst * 32 * CT * 4 * 1024;
+}
+
+void launch_k1b(const uint32_t* const* slabs, int nshards, const uint32_t* qp, int nq, uint32_t* ql, uint32_t* outb, int nst,
+                hipStream_t s) {
+    const int CT = nq > 32 ? 2 : 1;
+    const int S = 3 * nst;
+    hipLaunchKernelGGL(qlimb_kernel, dim3(32, 4 * S), dim3(128), 0, s, qp, ql, nq, nst, CT);
+    const size_t qstride = (size_t)nshards * 2 * N * LANES;
+    if (nq == 1)
+        hipLaunchKernelGGL((k1d_kernel<1, 2>), dim3(HALFN / 16, 2, nshards * 2), dim3(256), 0, s, slabs, ql, outb, nst, qstride);
+    else if (nq == 2)
+        hipLaunchKernelGGL((k1d_kernel<2, 2>), dim3(HALFN / 16, 2, nshards * 2), dim3(256), 0, s, slabs, ql, outb, nst, qstride);
+    else if (CT == 1)
+        hipLaunchKernelGGL((k1b_kernel<1, 4>), dim3(HALFN / 16, 2, nshards * 2), dim3(256), 0, s, slabs, (const v4i*)ql, outb,
+                           nst, nq, qstride);
+    else
+        hipLaunchKernelGGL((k1b_kernel<2, 4>), dim3(HALFN / 16, 2, nshards * 2), dim3(256), 0, s, slabs, (const v4i*)ql, outb,
+                           nst, nq, qstride);
 }
 
 void launch_chain_out(const uint32_t* a_out, const uint32_t* c_out, int nst, int nlanes, uint64_t* a_coef, uint64_t* b_coef,
@@ -749,9 +1064,19 @@ void launch_k1_split(const uint32_t* const* slabs, int nshards, const uint32_t* 
                        rows_per_block);
 }
 
-void launch_epi_primary(const unsigned long long* acc, int nshards, const uint32_t* const* a17s, uint16_t* se,
-                        uint64_t* bcoef, int slot, const DevTables* T, hipStream_t s) {
-    hipLaunchKernelGGL(epi_primary_kernel, dim3(nshards, LANES), dim3(256), 0, s, acc, a17s, se, bcoef, slot, T);
+void launch_epi(const EpiLaunch& e, const DevTables* T, hipStream_t s) {
+    EpiArgs A;
+    A.outb = e.outb;
+    A.acc = e.acc;
+    A.q_stride = (size_t)e.nshards * 2 * N * LANES;
+    A.a17s = e.a17s;
+    A.se = e.se;
+    A.se_qstride = e.se_qstride;
+    A.lpr = e.lpr;
+    A.lane0_mult = e.lane0_mult;
+    A.bcoef = e.bcoef;
+    A.diag = e.diag;
+    hipLaunchKernelGGL(epi_primary_kernel, dim3(e.nshards, LANES, e.diag ? 1 : e.nq), dim3(256), 0, s, A, T);
 }
 
 // ---------------------------------------------------------------------------
@@ -815,9 +1140,11 @@ struct QueryLayout {
     int offs[6];
     int bits[6];
 };
-__global__ __launch_bounds__(256) void decode_query_kernel(const uint8_t* __restrict__ qbytes, QueryLayout lay,
+__global__ __launch_bounds__(256) void decode_query_kernel(const uint8_t* __restrict__ qbytes, size_t qstride, QueryLayout lay,
                                                            uint64_t* __restrict__ qcoef) {
     const int v = blockIdx.y;
+    qbytes += (size_t)blockIdx.z * qstride;
+    qcoef += (size_t)blockIdx.z * 6 * N;
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= N) return;
     uint64_t out = 0;
@@ -832,6 +1159,8 @@ __global__ __launch_bounds__(256) void decode_query_kernel(const uint8_t* __rest
 __global__ __launch_bounds__(256) void encode_reply_kernel(const uint32_t* __restrict__ a17, const uint32_t* __restrict__ b12,
                                                            int lane0, uint8_t* __restrict__ reply) {
     constexpr int AB = N * 17 / 8, BB = N * 12 / 8;
+    lane0 += 3 * blockIdx.y;
+    reply += (size_t)blockIdx.y * 3 * (AB + BB);
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= 3 * (AB + BB)) return;
     const int p = k / (AB + BB);
@@ -840,18 +1169,19 @@ __global__ __launch_bounds__(256) void encode_reply_kernel(const uint32_t* __res
     reply[k] = o < AB ? condense_byte(a17 + (size_t)li * N, 17, o) : condense_byte(b12 + (size_t)li * N, 12, o - AB);
 }
 
-void launch_decode_query(const uint8_t* qbytes, const int offs[6], const int bits[6], uint64_t* qcoef, hipStream_t s) {
+void launch_decode_query(const uint8_t* qbytes, size_t qstride, int nq, const int offs[6], const int bits[6], uint64_t* qcoef,
+                         hipStream_t s) {
     QueryLayout lay;
     for (int i = 0; i < 6; ++i) {
         lay.offs[i] = offs[i];
         lay.bits[i] = bits[i];
     }
-    hipLaunchKernelGGL(decode_query_kernel, dim3(N / 256, 6), dim3(256), 0, s, qbytes, lay, qcoef);
+    hipLaunchKernelGGL(decode_query_kernel, dim3(N / 256, 6, nq), dim3(256), 0, s, qbytes, qstride, lay, qcoef);
 }
 
-void launch_encode_reply(const uint32_t* a17, const uint32_t* b12, int lane0, uint8_t* reply, hipStream_t s) {
+void launch_encode_reply(const uint32_t* a17, const uint32_t* b12, int lane0, int nq, uint8_t* reply, hipStream_t s) {
     const int total = 3 * (N * 17 / 8 + N * 12 / 8);
-    hipLaunchKernelGGL(encode_reply_kernel, dim3((total + 255) / 256), dim3(256), 0, s, a17, b12, lane0, reply);
+    hipLaunchKernelGGL(encode_reply_kernel, dim3((total + 255) / 256, nq), dim3(256), 0, s, a17, b12, lane0, reply);
 }
 
 }  // namespace dn

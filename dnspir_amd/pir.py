@@ -187,6 +187,36 @@ def process_queries_device(fleet0, fleet1, query0_dev, query1_dev, reply0_dev, r
     return tm
 
 
+def process_batch(fleet, queries, ctx=None):
+    """B independent queries against one fleet in one pass (host bytes);
+    returns (list of reply bytes, Timing)."""
+    init()
+    f, n = _fleet(fleet)
+    qlen = len(queries[0])
+    if any(len(q) != qlen for q in queries):
+        raise DnspirError(-5, "process_batch: all queries of a fleet have the same size")
+    buf = np.frombuffer(b"".join(bytes(q) for q in queries), np.uint8).copy()
+    rb = reply_bytes()
+    out = np.zeros(len(queries) * rb, np.uint8)
+    tm = Timing()
+    check(lib().dnspir_process_batch(ctx.h if ctx else None, f, n, _ptr(buf, u8p), qlen, len(queries),
+                                     _ptr(out, u8p), ctypes.byref(tm)), "process_batch")
+    return [bytes(out[i * rb:(i + 1) * rb]) for i in range(len(queries))], tm
+
+
+def process_batch_device(fleet, queries_dev, qlen, nq, replies_dev, ctx=None):
+    """Batched queries with packed query / reply bytes in device memory."""
+    import torch
+    init()
+    f, n = _fleet(fleet)
+    tm = Timing()
+    torch.cuda.current_stream().synchronize()
+    check(lib().dnspir_process_batch_device(ctx.h if ctx else None, f, n, ctypes.c_void_p(queries_dev.data_ptr()), qlen, nq,
+                                            ctypes.c_void_p(replies_dev.data_ptr()), ctypes.byref(tm)),
+          "process_batch_device")
+    return tm
+
+
 def get_database_shape(num_entries):
     lp, ls = ctypes.c_int(), ctypes.c_int()
     lib().dnspir_database_shape(num_entries, ctypes.byref(lp), ctypes.byref(ls))

@@ -53,6 +53,8 @@ typedef struct {
  * tables). Must precede every other device call. */
 int dnspir_init(int device);
 const char* dnspir_strerror(int code);
+/* text of the last HIP failure behind a DNSPIR_E_HIP (call site and HIP message) */
+const char* dnspir_last_error(void);
 
 /* ---- one shard: PIRDatabase (base_pir.rs:225-1247) ------------------------ */
 
@@ -117,6 +119,18 @@ int dnspir_process_queries_device(dnspir_ctx* ctx, const dnspir_shard* const* fl
                                   const dnspir_shard* const* fleet1, size_t len1, const uint8_t* query0_dev,
                                   size_t qlen0, const uint8_t* query1_dev, size_t qlen1, uint8_t* reply0_dev,
                                   uint8_t* reply1_dev, dnspir_timing* timing);
+/* Batched serving: nq independent queries (each answered exactly as
+ * process_query, pir_wrapper.rs:328-337) against one fleet of half- or
+ * full-size shards, sharing one pass over the shards (an int8-MFMA contraction
+ * per 64 queries). Queries are packed back to back, qlen bytes each (all the
+ * same size for one fleet); replies are packed dnspir_reply_bytes() each.
+ * No counterpart in the reference, which batches at most 2 queries
+ * (BATCH_COUNT, double_pir.rs:37). Host buffers: */
+int dnspir_process_batch(dnspir_ctx* ctx, const dnspir_shard* const* fleet, size_t len, const uint8_t* queries,
+                         size_t qlen, int nq, uint8_t* replies, dnspir_timing* timing);
+/* Same with queries and replies in device memory. */
+int dnspir_process_batch_device(dnspir_ctx* ctx, const dnspir_shard* const* fleet, size_t len, const uint8_t* queries_dev,
+                                size_t qlen, int nq, uint8_t* replies_dev, dnspir_timing* timing);
 /* get_database_shape (pir_wrapper.rs:219-231): index-group orders */
 void dnspir_database_shape(size_t num_entries, int* l_primary, int* l_secondary);
 /* num_databases (pir_wrapper.rs:238-241) */

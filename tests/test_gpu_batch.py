@@ -1,0 +1,111 @@
+"""GPU batched queries (int8-MFMA contraction over the fleet): every reply of a
+batch is byte-identical to the same query answered alone and to the oracle,
+and decrypts to the requested entry."""
+import sys
+
+import numpy as np
+import pytest
+
+import pyoracle as orc
+
+pytestmark = pytest.mark.gpu
+N = 2048
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def pir():
+    from dnspir_amd import pir as p
+    p.init(0)
+    return p
+
+
+_fleets = {}
+
+
+def _fleet(pir, l, nsh, with_oracle=False):
+    key = (l, nsh, with_oracle)
+    if key not in _fleets:
+        _say(f"[batch] preprocessing {nsh} shards l={l}")
+        g, o, e = [], [], []
+        for k in range(nsh):
+            ent = np.random.default_rng(100 + k).integers(0, 1024, (8 * l, N)).astype(np.uint16)
+            s = pir.PIRDatabase(l)
+            s.set_db(ent)
+            g.append(s)
+            e.append(ent)
+            if with_oracle:
+                os_ = orc.Shard(11, l)
+                os_.set_db(ent)
+                o.append(os_)
+        _fleets[key] = (g, o, e)
+    return _fleets[key]
+
+
+def _queries(pir, n, l, nsh, seed):
+    ls = pir.choose_index_group_secondary(nsh)
+    rng = pir.Rng(bytes([seed] * 32))
+    idx = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        pi = int(idx.integers(0, 8 * l))
+        si = int(idx.integers(0, nsh))
+        q, sk = pir.prepare_query(rng, pi, si, l, ls)
+        out.append((pi, si, q, sk))
+    return out
+
+
+@pytest.mark.parametrize("l,nsh,nq", [(1024, 2, 5), (2048, 2, 3)])
+def test_batch_matches_single_and_oracle(pir, l, nsh, nq):
+    g, o, e = _fleet(pir, l, nsh, with_oracle=True)
+    qs = _queries(pir, nq, l, nsh, 11)
+    _say(f"[batch] l={l} nq={nq}")
+    reps, tm = pir.process_batch(g, [q for _, _, q, _ in qs])
+    for k, (pi, si, q, sk) in enumerate(qs):
+        single, _ = pir.process_queries(g, None, q, None)
+        assert reps[k] == single[0], k
+        if k < 2:
+            oreps, _, _ = orc.process_queries(o, None, q, None)
+            assert reps[k] == oreps[0], k
+        assert np.array_equal(pir.process_reply(reps[k], sk), e[si][pi]), k
+
+
+@pytest.mark.parametrize("nq", [33, 64, 70])
+def test_batch_two_column_tiles_and_chunks(pir, nq):
+    """33..64 queries use two MFMA column tiles; 70 needs two passes."""
+    g, _, e = _fleet(pir, 1024, 3)
+    qs = _queries(pir, nq, 1024, 3, nq)
+    _say(f"[batch] nq={nq}")
+    reps, tm = pir.process_batch(g, [q for _, _, q, _ in qs])
+    for k in (0, 31, 32, nq - 1):
+        pi, si, q, sk = qs[k]
+        single, _ = pir.process_queries(g, None, q, None)
+        assert reps[k] == single[0], k
+    for k, (pi, si, q, sk) in enumerate(qs):
+        assert np.array_equal(pir.process_reply(reps[k], sk), e[si][pi]), k
+
+
+def test_batch_same_alone_and_in_mix(pir):
+    """A query's reply does not depend on the other queries in its batch."""
+    g, _, _ = _fleet(pir, 1024, 3)
+    qs = _queries(pir, 12, 1024, 3, 77)
+    mix, _ = pir.process_batch(g, [q for _, _, q, _ in qs])
+    alone, _ = pir.process_batch(g, [qs[5][2]])
+    pair, _ = pir.process_batch(g, [qs[9][2], qs[5][2]])
+    assert alone[0] == mix[5] == pair[1]
+    assert pair[0] == mix[9]
+
+
+def test_batch_rejects_mixed_sizes_and_short(pir):
+    from dnspir_amd import DnspirError
+    g, _, _ = _fleet(pir, 1024, 3)
+    qs = _queries(pir, 2, 1024, 3, 5)
+    with pytest.raises(DnspirError):
+        pir.process_batch(g, [qs[0][2], qs[1][2][:-1]])
+    with pytest.raises(DnspirError) as ei:
+        pir.process_batch(g, [qs[0][2][:-8], qs[1][2][:-8]])
+    assert ei.value.code == -1

@@ -82,6 +82,99 @@ def cpu_baseline(lp, nshards, ls, query_block, threads):
     }
 
 
+def make_shards(pir, lp, count, seed0):
+    """`count` shards of seeded uniform 2560-byte entries (device generated)."""
+    import torch
+    gen = torch.Generator(device="cuda")
+    shards, entries = [], []
+    for k in range(count):
+        gen.manual_seed(seed0 + k)
+        raw = torch.randint(0, 256, (8 * lp, 2560), dtype=torch.uint8, device="cuda", generator=gen)
+        coeffs = pack_entries(raw)
+        del raw
+        s = pir.PIRDatabase(lp)
+        s.set_db_device(coeffs)
+        shards.append(s)
+        entries.append(coeffs)
+    torch.cuda.synchronize()
+    return shards, entries
+
+
+def batched(pir, entries_total, sizes, reps):
+    """Batched serving at `entries_total` entries: every shard distinct and
+    resident; batches of B concurrent clients (own keys each) through
+    dnspir_process_batch_device; unbatched = the same clients one query at a
+    time through dnspir_process_queries_device."""
+    import torch
+    lp, ls = pir.get_database_shape(entries_total)
+    nsh = pir.num_databases(entries_total)
+    t0 = time.perf_counter()
+    shards, entries = make_shards(pir, lp, nsh, 5000)
+    log(f"[bench] batched: {nsh} shards of l={lp} preprocessed in {time.perf_counter() - t0:.1f} s")
+    bmax = max(sizes)
+    idx_rng = np.random.default_rng(42)
+    rng = pir.Rng(bytes([42] + [0] * 31))
+    qs = []
+    for _ in range(bmax):
+        pi = int(idx_rng.integers(0, 8 * lp))
+        si = int(idx_rng.integers(0, nsh))
+        q, sk = pir.prepare_query(rng, pi, si, lp, ls)
+        qs.append((pi, si, q, sk))
+    qlen = len(qs[0][2])
+    qdev = torch.from_numpy(np.frombuffer(b"".join(q for _, _, q, _ in qs), np.uint8).copy()).cuda()
+    rb = pir.reply_bytes()
+    rdev = torch.zeros(bmax * rb, dtype=torch.uint8, device="cuda")
+    ctx = pir.Context()
+    # unbatched: single queries back to back
+    single_ms = []
+    for k in range(min(8, bmax) + 1):
+        t = time.perf_counter()
+        pir.process_queries_device(shards, None, qdev[k * qlen:(k + 1) * qlen], None, rdev[:rb], None, ctx=ctx)
+        if k:
+            single_ms.append((time.perf_counter() - t) * 1e3)
+    qps_single = 1e3 / float(np.mean(single_ms))
+    sweep, detail = {}, {}
+    for B in sizes:
+        pir.process_batch_device(shards, qdev, qlen, B, rdev, ctx=ctx)  # warmup
+        times, k1 = [], []
+        for _ in range(reps):
+            t = time.perf_counter()
+            tm = pir.process_batch_device(shards, qdev, qlen, B, rdev, ctx=ctx)
+            times.append((time.perf_counter() - t) * 1e3)
+            k1.append(tm.k1_us / 1e3)
+        ms = float(np.mean(times))
+        sweep[str(B)] = round(B * 1e3 / ms, 1)
+        detail[str(B)] = {"ms_per_batch": round(ms, 3), "primary_ms": round(tm.primary_us / 1e3, 3),
+                          "secondary_ms": round(tm.secondary_us / 1e3, 3), "k1_ms": round(float(np.mean(k1)), 3),
+                          "p99_batch_ms": round(float(np.percentile(times, 99)), 3)}
+        log(f"[bench] batch {B}: {ms:.2f} ms, {sweep[str(B)]} queries/s")
+    # decryption check of every reply of the largest batch
+    out = rdev.cpu().numpy()
+    ok = True
+    for j, (pi, si, q, sk) in enumerate(qs[:bmax]):
+        got = pir.process_reply(out[j * rb:(j + 1) * rb].tobytes(), sk)
+        ok &= bool(np.array_equal(got, entries[si][pi].cpu().numpy().astype(np.uint16)))
+    b64 = detail.get("64")
+    mfma = None
+    if b64:
+        macs = 64 * nsh * 48 * 2048 * lp  # modular MACs of one 64-query primary pass
+        ops = 2 * 4 * 4 * macs
+        mfma = {"int8_ops": ops, "k1_ms": b64["k1_ms"], "achieved_tops": round(ops / (b64["k1_ms"] * 1e-3) / 1e12, 1),
+                "peak_tops": 5000.0, "frac": round(ops / (b64["k1_ms"] * 1e-3) / 5.0e15, 4)}
+    del shards, entries
+    torch.cuda.empty_cache()
+    return {
+        "workload": (f"{entries_total} entries = {nsh} shards x {8 * lp} entries (l={lp}), all resident; "
+                     f"secondary l={ls}; B concurrent clients with their own keys"),
+        "queries_per_s": sweep,
+        "unbatched_queries_per_s": round(qps_single, 1),
+        "speedup_b64_vs_unbatched": round(sweep["64"] / qps_single, 2) if "64" in sweep else None,
+        "detail": detail,
+        "mfma_int8_b64": mfma,
+        "decrypt_ok": ok,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +184,10 @@ def main():
     ap.add_argument("--resident", type=int, default=64)
     ap.add_argument("--queries", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batch-entries", type=int, default=262144)
+    ap.add_argument("--batch-sizes", default="1,8,16,32,64,128")
+    ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (PMC calibration)")
@@ -123,21 +220,9 @@ def main():
 
     # resident shards: seeded uniform 2560-byte entries packed into coefficients
     t0 = time.perf_counter()
-    gen = torch.Generator(device="cuda")
-    shards, entries = [], []
-    for k in range(resident):
-        gen.manual_seed(1000 + k)
-        raw = torch.randint(0, 256, (8 * lp, 2560), dtype=torch.uint8, device="cuda", generator=gen)
-        coeffs = pack_entries(raw)
-        del raw
-        s = pir.PIRDatabase(lp)
-        s.set_db_device(coeffs)
-        shards.append(s)
-        entries.append(coeffs)
-        if (k + 1) % 16 == 0:
-            log(f"[bench] preprocessed {k + 1}/{resident} shards ({time.perf_counter() - t0:.1f} s)")
-    torch.cuda.synchronize()
+    shards, entries = make_shards(pir, lp, resident, 1000)
     t_pre = time.perf_counter() - t0
+    log(f"[bench] preprocessed {resident} shards in {t_pre:.1f} s")
     fleet = [shards[k % resident] for k in range(nshards)]
 
     # seeded queries, staged in HBM
@@ -189,6 +274,12 @@ def main():
     tag = "2p22" if args.entries == 1 << 22 else f"e{args.entries}"
     traffic = read_traffic(tag)
 
+    batch = None
+    if not args.no_batched:
+        del shards, fleet, entries
+        torch.cuda.empty_cache()
+        batch = batched(pir, args.batch_entries, [int(x) for x in args.batch_sizes.split(",")], args.batch_reps)
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -233,7 +324,7 @@ def main():
         "preprocess_s": round(t_pre, 2),
         "roofline": {
             "bound": "hbm",
-            "kernel": "k1_kernel",
+            "kernel": "k1d_kernel (one query: 256 shards in one launch)",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -243,6 +334,7 @@ def main():
             "bytes_per_launch": bytes_streamed,
         },
         "cpu_baseline": cpu,
+        "batched": batch,
     }
     print(json.dumps(out), flush=True)
 

@@ -1024,10 +1024,11 @@ size_t qlimb_bytes(int nq, int nst) {
 }
 
 void launch_k1b(const uint32_t* const* slabs, int nshards, const uint32_t* qp, int nq, uint32_t* ql, uint32_t* outb, int nst,
-                hipStream_t s) {
+                hipStream_t s, hipEvent_t main_start) {
     const int CT = nq > 32 ? 2 : 1;
     const int S = 3 * nst;
     hipLaunchKernelGGL(qlimb_kernel, dim3(32, 4 * S), dim3(128), 0, s, qp, ql, nq, nst, CT);
+    if (main_start) (void)hipEventRecord(main_start, s);
     const size_t qstride = (size_t)nshards * 2 * N * LANES;
     if (nq == 1)
         hipLaunchKernelGGL((k1d_kernel<1, 2>), dim3(HALFN / 16, 2, nshards * 2), dim3(256), 0, s, slabs, ql, outb, nst, qstride);

@@ -27,8 +27,9 @@ void launch_relayout(const uint32_t* rows_tmp, uint32_t* slab, int L, int nst, h
 size_t qlimb_bytes(int nq, int nst);
 // K1 on limb-block shards for nq <= 64 queries (qp [q][4][2][N] Galois order):
 // outb [q][shard][r][gpos][lane] reduced residues
+// main_start (optional) is recorded right before the inner-product launch
 void launch_k1b(const uint32_t* const* slabs, int nshards, const uint32_t* qp, int nq, uint32_t* ql, uint32_t* outb, int nst,
-                hipStream_t s);
+                hipStream_t s, hipEvent_t main_start = nullptr);
 void launch_chain_out(const uint32_t* a_out, const uint32_t* c_out, int nst, int nlanes, uint64_t* a_coef, uint64_t* b_coef,
                       uint32_t* a17, uint32_t* b12, const DevTables* T, hipStream_t s);
 void launch_qprep(const uint64_t* qcoef, uint32_t* qs, uint32_t* qp, int nq, const DevTables* T, hipStream_t s);

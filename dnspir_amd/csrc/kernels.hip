@@ -101,118 +101,8 @@ __device__ __forceinline__ int brv_bits(int v, int bits) {
 }
 
 // ---------------------------------------------------------------------------
-// K5: database encoding (base_pir.rs:662-772), modulus p1 for steps 1-5
-// ---------------------------------------------------------------------------
-
-// steps 1-2: column j of lane index li -> NTT of the entry's coefficient
-// vector; the entry sits at entries + ent_off[li] + j * col_stride (u16 [N]).
-// M [lane_idx][j][slot]
-__global__ __launch_bounds__(256) void k5_col_ntt_kernel(const uint16_t* __restrict__ entries, const long long* __restrict__ ent_off,
-                                                         long long col_stride, uint32_t* __restrict__ M, int l,
-                                                         const int* __restrict__ ncols_nonzero, const DevTables* __restrict__ T) {
-    __shared__ uint32_t a[N];
-    const int j = blockIdx.x, li = blockIdx.y;
-    uint32_t* out = M + ((size_t)li * l + j) * N;
-    if (j >= ncols_nonzero[li]) {
-        for (int s = threadIdx.x; s < N; s += blockDim.x) out[s] = 0;
-        return;
-    }
-    const uint16_t* src = entries + ent_off[li] + (long long)j * col_stride;
-    for (int s = threadIdx.x; s < N; s += blockDim.x) a[s] = src[s];
-    __syncthreads();
-    ntt_fwd(a, T, 0, threadIdx.x, blockDim.x);
-    for (int s = threadIdx.x; s < N; s += blockDim.x) out[s] = a[s];
-}
-
-// step 3 (coset_ntt, base_pir.rs:397-444) for a tile of TS slots of one lane,
-// in place in M. Row s: twist by psi^(-coset j), DFT over j, then
-// new[i] = evaluation at psi^(-z 5^i) (main half) / psi^(z 5^(i-N/2)) (conj
-// half), z = e_s * k mod 2N.
-constexpr int K5_TS = 8;
-__global__ __launch_bounds__(256) void k5_rows_kernel(uint32_t* __restrict__ M, int l, int logl, int kq, int conj,
-                                                      const DevTables* __restrict__ T) {
-    extern __shared__ uint32_t tile[];  // [l][TS]
-    const int s0 = blockIdx.x * K5_TS, li = blockIdx.y;
-    uint32_t* base = M + (size_t)li * l * N;
-    const int tid = threadIdx.x, nt = blockDim.x;
-    const int step = M2N / l;
-    for (int idx = tid; idx < l * K5_TS; idx += nt) {
-        const int j = idx / K5_TS, d = idx % K5_TS;
-        const int s = s0 + d;
-        const int z = (T->slot_exp[s] * kq) & (M2N - 1);
-        const int coset = z % step;
-        const int e = (M2N - ((coset * j) & (M2N - 1))) & (M2N - 1);
-        tile[idx] = mul_shoup(base[(size_t)j * N + s], T->psipow[0][e], T->psipowq[0][e], P1);
-    }
-    __syncthreads();
-    dft_dif_p1(tile, logl, K5_TS, T, tid, nt);
-    for (int idx = tid; idx < l * K5_TS; idx += nt) {
-        const int i = idx / K5_TS, d = idx % K5_TS;
-        const int s = s0 + d;
-        const int z = (T->slot_exp[s] * kq) & (M2N - 1);
-        const int coset = z % step;
-        int y;
-        if (conj && i >= N / 2) y = (z * T->pow5[i - N / 2]) & (M2N - 1);
-        else y = ((M2N - z) * T->pow5[i]) & (M2N - 1);
-        const int u = (((M2N - y) & (M2N - 1)) - coset) / step;  // natural DFT index
-        base[(size_t)i * N + s] = tile[brv_bits(u, logl) * K5_TS + d];
-    }
-}
-
-// steps 4-6: final column c takes column j = rev(c) (step 5), gathers its
-// slots by sigma_{5^-j} (step 4), then inverse NTT, centered lift, times
-// l^-1 mod t, centered lift, embedding into p1 and p2 and forward NTTs.
-// te_tmp [st][lane_idx][row][r][slot]
-__global__ __launch_bounds__(256) void k5_cols_final_kernel(const uint32_t* __restrict__ M, uint32_t* __restrict__ te_tmp,
-                                                            int l, int L, int conj, int nlanes, int scale,
-                                                            const DevTables* __restrict__ T) {
-    __shared__ uint32_t a[2][N];
-    const int c = blockIdx.x, li = blockIdx.y;
-    const int tid = threadIdx.x, nt = blockDim.x;
-    int j;
-    if (conj) j = c < N / 2 ? N / 2 - 1 - c : N / 2 + (N - 1 - c);
-    else j = l - 1 - c;
-    const uint32_t* col = M + ((size_t)li * l + j) * N;
-    const int au = T->inv5pow[j & (HALFN - 1)];
-    for (int s = tid; s < N; s += nt) a[0][s] = col[T->exp_slot[(au * T->slot_exp[s]) & (M2N - 1)]];
-    __syncthreads();
-    ntt_inv(a[0], T, 0, tid, nt);
-    for (int s = tid; s < N; s += nt) {
-        int32_t x = (int32_t)a[0][s];
-        if (x > (int32_t)(P1 / 2)) x -= (int32_t)P1;
-        int32_t xt = x % (int32_t)T_PLAIN;
-        if (xt < 0) xt += T_PLAIN;
-        int32_t y = (int32_t)(((int64_t)xt * scale) % T_PLAIN);
-        if (y > (int32_t)(T_PLAIN / 2)) y -= (int32_t)T_PLAIN;
-        a[0][s] = y < 0 ? (uint32_t)(y + (int32_t)P1) : (uint32_t)y;
-        a[1][s] = y < 0 ? (uint32_t)(y + (int32_t)P2) : (uint32_t)y;
-    }
-    __syncthreads();
-    const int half = tid >= nt / 2;
-    ntt_fwd(a[half], T, half, tid - half * (nt / 2), nt / 2);
-    const int st = c < L ? 0 : 1;
-    const int row = c < L ? c : c - L;
-    uint32_t* dst = te_tmp + (((size_t)(st * nlanes + li) * L + row) * 2) * N;
-    for (int s = tid; s < N; s += nt) {
-        dst[s] = a[0][s];
-        dst[N + s] = a[1][s];
-    }
-}
-
-// ---------------------------------------------------------------------------
-// K4/K6: the sequential a-chain (preprocess_loop, base_pir.rs:819-862, with
-// partial_gal :496-594). One block of 1024 threads per (lane, stream). Step i:
-//   A  gather sigma_5(a) and inverse NTT in p1 and p2     (threads 0..511)
-//   B  CRT, centered lift, balanced base-2^26 digits D0, D1 (all threads)
-//   C  forward NTTs of D0, D1 in p1 and p2                  (256 threads each)
-//   D  a <- D0 gka0 + D1 gka1 + qa TE_i (per slot)
-// SEC additionally runs the secondary b-recurrence in gathered form,
-// c <- sigma_5(c) + D0 gkb0 + D1 gkb1 + TE_i qb (main_pir_loop,
-// base_pir.rs:1085-1247), so the digits never leave the chip; PRE stores the
-// digits as the shard's IPGK rows.
-// NTT passes are radix-8 (radix-4 for the last two stages) over registers with
-// LDS exchanges; values stay in [0, 2p) between butterflies (Harvey's lazy
-// reduction with Shoup multiplications); twiddles live in LDS.
+// register-radix NTT passes (radix 8, radix 4 for stages 9-10) over an LDS
+// array; Harvey lazy values in [0, 2p), Shoup multiplications by twiddles
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t red2p(uint32_t a, uint32_t p2) {  // [0, 4p) -> [0, 2p)
     const uint32_t b = a - p2;
@@ -307,6 +197,157 @@ __device__ __forceinline__ void inv_pass8(uint32_t* buf, const uint32_t* tw, con
     for (int c = 0; c < 8; ++c) buf[j0 + c * q] = v[c];
 }
 
+// first inverse pass: stages 10 and 9 for the radix-4 groups g, g + 256
+__device__ __forceinline__ void inv_pass4(uint32_t* buf, const uint32_t* tw, const uint32_t* twq, uint32_t p, int g) {
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int i = g + 256 * k;
+        uint32_t v0 = buf[4 * i], v1 = buf[4 * i + 1], v2 = buf[4 * i + 2], v3 = buf[4 * i + 3];
+        bf_gs(v0, v1, tw[1024 + 2 * i], twq[1024 + 2 * i], p);
+        bf_gs(v2, v3, tw[1025 + 2 * i], twq[1025 + 2 * i], p);
+        const uint32_t w = tw[512 + i], wq = twq[512 + i];
+        bf_gs(v0, v2, w, wq, p);
+        bf_gs(v1, v3, w, wq, p);
+        buf[4 * i] = v0; buf[4 * i + 1] = v1; buf[4 * i + 2] = v2; buf[4 * i + 3] = v3;
+    }
+}
+// whole-block transforms by 256 threads (g = thread index) on an LDS array;
+// inputs in [0, 2p), outputs in [0, p); the inverse leaves the N factor in
+__device__ void ntt256_fwd(uint32_t* buf, const uint32_t* tw, const uint32_t* twq, uint32_t p, int g) {
+    for (int P = 0; P < 3; ++P) {
+        fwd_pass8(buf, tw, twq, p, P, g);
+        __syncthreads();
+    }
+    fwd_pass4(buf, tw, twq, p, g);
+    __syncthreads();
+    for (int j = g; j < N; j += 256) buf[j] = buf[j] >= p ? buf[j] - p : buf[j];
+    __syncthreads();
+}
+__device__ void ntt256_inv_unscaled(uint32_t* buf, const uint32_t* tw, const uint32_t* twq, uint32_t p, int g) {
+    inv_pass4(buf, tw, twq, p, g);
+    __syncthreads();
+    for (int P = 2; P >= 0; --P) {
+        inv_pass8(buf, tw, twq, p, P, g);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K5: database encoding (base_pir.rs:662-772), modulus p1 for steps 1-5
+// ---------------------------------------------------------------------------
+
+// steps 1-2: column j of lane index li -> NTT of the entry's coefficient
+// vector; the entry sits at entries + ent_off[li] + j * col_stride (u16 [N]).
+// M [lane_idx][j][slot]
+__global__ __launch_bounds__(256) void k5_col_ntt_kernel(const uint16_t* __restrict__ entries, const long long* __restrict__ ent_off,
+                                                         long long col_stride, uint32_t* __restrict__ M, int l,
+                                                         const int* __restrict__ ncols_nonzero, const DevTables* __restrict__ T) {
+    __shared__ uint32_t a[N];
+    const int j = blockIdx.x, li = blockIdx.y;
+    uint32_t* out = M + ((size_t)li * l + j) * N;
+    if (j >= ncols_nonzero[li]) {
+        for (int s = threadIdx.x; s < N; s += blockDim.x) out[s] = 0;
+        return;
+    }
+    const uint16_t* src = entries + ent_off[li] + (long long)j * col_stride;
+    for (int s = threadIdx.x; s < N; s += blockDim.x) a[s] = src[s];
+    __syncthreads();
+    ntt256_fwd(a, T->psi[0], T->psiq[0], P1, threadIdx.x);
+    for (int s = threadIdx.x; s < N; s += blockDim.x) out[s] = a[s];
+}
+
+// step 3 (coset_ntt, base_pir.rs:397-444) for a tile of TS slots of one lane,
+// in place in M. Row s: twist by psi^(-coset j), DFT over j, then
+// new[i] = evaluation at psi^(-z 5^i) (main half) / psi^(z 5^(i-N/2)) (conj
+// half), z = e_s * k mod 2N.
+constexpr int K5_TS = 8;
+__global__ __launch_bounds__(256) void k5_rows_kernel(uint32_t* __restrict__ M, int l, int logl, int kq, int conj,
+                                                      const DevTables* __restrict__ T) {
+    extern __shared__ uint32_t tile[];  // [l][TS]
+    const int s0 = blockIdx.x * K5_TS, li = blockIdx.y;
+    uint32_t* base = M + (size_t)li * l * N;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int step = M2N / l;
+    for (int idx = tid; idx < l * K5_TS; idx += nt) {
+        const int j = idx / K5_TS, d = idx % K5_TS;
+        const int s = s0 + d;
+        const int z = (T->slot_exp[s] * kq) & (M2N - 1);
+        const int coset = z % step;
+        const int e = (M2N - ((coset * j) & (M2N - 1))) & (M2N - 1);
+        tile[idx] = mul_shoup(base[(size_t)j * N + s], T->psipow[0][e], T->psipowq[0][e], P1);
+    }
+    __syncthreads();
+    dft_dif_p1(tile, logl, K5_TS, T, tid, nt);
+    for (int idx = tid; idx < l * K5_TS; idx += nt) {
+        const int i = idx / K5_TS, d = idx % K5_TS;
+        const int s = s0 + d;
+        const int z = (T->slot_exp[s] * kq) & (M2N - 1);
+        const int coset = z % step;
+        int y;
+        if (conj && i >= N / 2) y = (z * T->pow5[i - N / 2]) & (M2N - 1);
+        else y = ((M2N - z) * T->pow5[i]) & (M2N - 1);
+        const int u = (((M2N - y) & (M2N - 1)) - coset) / step;  // natural DFT index
+        base[(size_t)i * N + s] = tile[brv_bits(u, logl) * K5_TS + d];
+    }
+}
+
+// steps 4-6: final column c takes column j = rev(c) (step 5), gathers its
+// slots by sigma_{5^-j} (step 4), then inverse NTT, centered lift, times
+// l^-1 mod t, centered lift, embedding into p1 and p2 and forward NTTs.
+// te_tmp [st][lane_idx][row][r][slot]
+__global__ __launch_bounds__(256) void k5_cols_final_kernel(const uint32_t* __restrict__ M, uint32_t* __restrict__ te_tmp,
+                                                            int l, int L, int conj, int nlanes, int scale,
+                                                            const DevTables* __restrict__ T) {
+    __shared__ uint32_t a[2][N];
+    const int c = blockIdx.x, li = blockIdx.y;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    int j;
+    if (conj) j = c < N / 2 ? N / 2 - 1 - c : N / 2 + (N - 1 - c);
+    else j = l - 1 - c;
+    const uint32_t* col = M + ((size_t)li * l + j) * N;
+    const int au = T->inv5pow[j & (HALFN - 1)];
+    for (int s = tid; s < N; s += nt) a[0][s] = col[T->exp_slot[(au * T->slot_exp[s]) & (M2N - 1)]];
+    __syncthreads();
+    ntt256_inv_unscaled(a[0], T->ipsi[0], T->ipsiq[0], P1, tid);
+    for (int s = tid; s < N; s += nt) {
+        uint32_t u = a[0][s];
+        u = u >= P1 ? u - P1 : u;
+        int32_t x = (int32_t)mul_shoup(u, T->ninv[0], T->ninvq[0], P1);
+        if (x > (int32_t)(P1 / 2)) x -= (int32_t)P1;
+        int32_t xt = x % (int32_t)T_PLAIN;
+        if (xt < 0) xt += T_PLAIN;
+        int32_t y = (int32_t)(((int64_t)xt * scale) % T_PLAIN);
+        if (y > (int32_t)(T_PLAIN / 2)) y -= (int32_t)T_PLAIN;
+        a[0][s] = y < 0 ? (uint32_t)(y + (int32_t)P1) : (uint32_t)y;
+        a[1][s] = y < 0 ? (uint32_t)(y + (int32_t)P2) : (uint32_t)y;
+    }
+    __syncthreads();
+    ntt256_fwd(a[0], T->psi[0], T->psiq[0], P1, tid);
+    ntt256_fwd(a[1], T->psi[1], T->psiq[1], P2, tid);
+    const int st = c < L ? 0 : 1;
+    const int row = c < L ? c : c - L;
+    uint32_t* dst = te_tmp + (((size_t)(st * nlanes + li) * L + row) * 2) * N;
+    for (int s = tid; s < N; s += nt) {
+        dst[s] = a[0][s];
+        dst[N + s] = a[1][s];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K4/K6: the sequential a-chain (preprocess_loop, base_pir.rs:819-862, with
+// partial_gal :496-594). One block of 1024 threads per (lane, stream). Step i:
+//   A  gather sigma_5(a) and inverse NTT in p1 and p2     (threads 0..511)
+//   B  CRT, centered lift, balanced base-2^26 digits D0, D1 (all threads)
+//   C  forward NTTs of D0, D1 in p1 and p2                  (256 threads each)
+//   D  a <- D0 gka0 + D1 gka1 + qa TE_i (per slot)
+// SEC additionally runs the secondary b-recurrence in gathered form,
+// c <- sigma_5(c) + D0 gkb0 + D1 gkb1 + TE_i qb (main_pir_loop,
+// base_pir.rs:1085-1247), so the digits never leave the chip; PRE stores the
+// digits as the shard's IPGK rows.
+// NTT passes are radix-8 (radix-4 for the last two stages) over registers with
+// LDS exchanges; values stay in [0, 2p) between butterflies (Harvey's lazy
+// reduction with Shoup multiplications); twiddles live in LDS.
+// ---------------------------------------------------------------------------
 constexpr uint32_t NINV_Q2INV_P1 = 67055624u;  // N^-1 p2^-1 mod p1
 constexpr uint32_t NINV_Q1INV_P2 = 8189u;      // N^-1 p1^-1 mod p2
 
@@ -594,7 +635,11 @@ typedef int v16i __attribute__((ext_vector_type(16)));
 template <int CT, int PD>
 __global__ __launch_bounds__(256) void k1b_kernel(const uint32_t* const* __restrict__ slabs, const v4i* __restrict__ ql,
                                                   uint32_t* __restrict__ outb, int nst, int nq, size_t q_stride) {
-    __shared__ v4i Bt[2][CT][4][64];
+    // B tiles of the next PD-1 K-steps sit in an LDS ring of PD stages; the
+    // tile PD steps ahead travels through registers (loaded one step earlier)
+    constexpr int NS = PD;
+    constexpr int D = NS - 1;
+    __shared__ v4i Bt[NS][CT][4][64];
     const int S = 3 * nst;
     const int half = blockIdx.y;
     const int r = blockIdx.z & 1, sh = blockIdx.z >> 1;
@@ -615,16 +660,26 @@ __global__ __launch_bounds__(256) void k1b_kernel(const uint32_t* const* __restr
     for (int d = 0; d < PD; ++d)
 #pragma unroll
         for (int v = 0; v < 4; ++v) abuf[d][v] = __builtin_nontemporal_load(A + (size_t)d * 256 + v * 64 + l);
-    for (int it = tid; it < CT * 256; it += 256) (&Bt[0][0][0][0])[it] = Q[it];
+#pragma unroll
+    for (int t = 0; t < D; ++t)
+#pragma unroll
+        for (int i = 0; i < CT; ++i) (&Bt[t][0][0][0])[tid + 256 * i] = Q[(size_t)t * CT * 256 + tid + 256 * i];
+    v4i breg[CT];
+#pragma unroll
+    for (int i = 0; i < CT; ++i) breg[i] = Q[(size_t)D * CT * 256 + tid + 256 * i];
     for (int kk0 = 0; kk0 < nk; kk0 += PD) {
 #pragma unroll
         for (int d = 0; d < PD; ++d) {
             const int kk = kk0 + d;
-            const int cur = d & 1;
             __syncthreads();
-            if (kk + 1 < nk)
-                for (int it = tid; it < CT * 256; it += 256)
-                    (&Bt[cur ^ 1][0][0][0])[it] = Q[(size_t)(kk + 1) * CT * 256 + it];
+            if (kk + D < nk) {
+#pragma unroll
+                for (int i = 0; i < CT; ++i) (&Bt[(d + D) % NS][0][0][0])[tid + 256 * i] = breg[i];
+            }
+            if (kk + D + 1 < nk) {
+#pragma unroll
+                for (int i = 0; i < CT; ++i) breg[i] = Q[(size_t)(kk + D + 1) * CT * 256 + tid + 256 * i];
+            }
             v4i a[4];
 #pragma unroll
             for (int v = 0; v < 4; ++v) a[v] = abuf[d][v];
@@ -636,7 +691,7 @@ __global__ __launch_bounds__(256) void k1b_kernel(const uint32_t* const* __restr
             for (int ct = 0; ct < CT; ++ct) {
                 v4i b[4];
 #pragma unroll
-                for (int v = 0; v < 4; ++v) b[v] = Bt[cur][ct][v][l];
+                for (int v = 0; v < 4; ++v) b[v] = Bt[d % NS][ct][v][l];
 #pragma unroll
                 for (int u = 0; u < 4; ++u)
 #pragma unroll

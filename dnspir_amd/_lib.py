@@ -25,7 +25,11 @@ ERRORS = {
     -8: "DNSPIR_E_HIP",
     -9: "DNSPIR_E_DECODE",
     -10: "DNSPIR_E_NOINIT",
+    -11: "DNSPIR_E_NOKEY",
+    -12: "DNSPIR_E_CLOSED",
 }
+E_NOKEY = -11
+QUERY_KEYLESS = 1
 
 
 class DnspirError(RuntimeError):
@@ -38,6 +42,29 @@ class Timing(ctypes.Structure):
     _fields_ = [("primary_us", ctypes.c_double), ("secondary_us", ctypes.c_double),
                 ("total_us", ctypes.c_double), ("bytes_streamed", ctypes.c_uint64),
                 ("k1_us", ctypes.c_double), ("k1_launches", ctypes.c_int)]
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("slots", ctypes.c_int), ("max_batch", ctypes.c_int), ("scratch_budget", ctypes.c_size_t),
+                ("key_cache_entries", ctypes.c_int)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int), ("db", ctypes.c_int), ("batch_size", ctypes.c_int),
+                ("key_cached", ctypes.c_int), ("resent", ctypes.c_int), ("submit_us", ctypes.c_double),
+                ("start_us", ctypes.c_double), ("done_us", ctypes.c_double)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("completed", ctypes.c_uint64), ("failed", ctypes.c_uint64), ("batches", ctypes.c_uint64),
+                ("key_hits", ctypes.c_uint64), ("key_inserts", ctypes.c_uint64), ("key_evictions", ctypes.c_uint64),
+                ("key_misses", ctypes.c_uint64), ("admission_waits", ctypes.c_uint64),
+                ("scratch_in_use_peak", ctypes.c_size_t), ("inflight_peak", ctypes.c_int)]
+
+
+class Arrival(ctypes.Structure):
+    _fields_ = [("db", ctypes.c_int), ("client", ctypes.c_uint64), ("arrival_us", ctypes.c_double),
+                ("query", ctypes.c_void_p), ("qlen", ctypes.c_size_t), ("keys", ctypes.c_void_p)]
 
 
 vp = ctypes.c_void_p
@@ -78,6 +105,18 @@ SIGNATURES = {
     "dnspir_rng_free": (None, [vp]),
     "dnspir_prepare_query": (ci, [vp, sz, sz, ci, ci, u8p, ctypes.POINTER(sz), u8p]),
     "dnspir_process_reply": (ci, [u8p, sz, u8p, u16p]),
+    "dnspir_client_keys": (ci, [vp, u8p, u8p]),
+    "dnspir_client_query": (ci, [vp, u8p, sz, sz, ci, ci, u8p, ctypes.POINTER(sz)]),
+    "dnspir_key_bytes": (sz, []),
+    "dnspir_batch_scratch_bytes": (sz, [ci, sz, ci]),
+    "dnspir_server_create": (ci, [ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]),
+    "dnspir_server_add_database": (ci, [vp, vp, sz, ip]),
+    "dnspir_server_max_batch": (ci, [vp, ci]),
+    "dnspir_server_submit": (ci, [vp, ci, ctypes.c_uint64, u8p, sz, ci, ctypes.POINTER(ctypes.c_uint64)]),
+    "dnspir_server_wait": (ci, [vp, ctypes.c_uint64, u8p, ctypes.POINTER(RequestInfo)]),
+    "dnspir_server_replay": (ci, [vp, ctypes.POINTER(Arrival), sz, u8p, ctypes.POINTER(RequestInfo)]),
+    "dnspir_server_get_stats": (ci, [vp, ctypes.POINTER(ServerStats)]),
+    "dnspir_server_destroy": (ci, [vp]),
 }
 
 _lib = None

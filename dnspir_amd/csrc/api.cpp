@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <stdio.h>
+#include <string.h>
 
 #include <algorithm>
 #include <chrono>
@@ -12,58 +13,32 @@
 #include <new>
 #include <vector>
 
-#include "../../include/dnspir.h"
 #include "dn_common.hpp"
+#include "engine.hpp"
 #include "host_math.hpp"
 #include "kernels.hpp"
 
 using namespace dn;
+using namespace dne;
 
 namespace {
+thread_local char g_last_error[256] = "";
+}  // namespace
 
-constexpr int A_BITS = 17, B_BITS = 12, GK_BITS = 50, QRY_BITS = 34;
-constexpr size_t REPLY_BYTES = 3 * (N * A_BITS / 8 + N * B_BITS / 8);
-constexpr size_t EXTRA_BYTES = (size_t)LANES * N * 8 + (size_t)LANES * N * 4;  // a_coef + a17
-constexpr size_t PTR_SLOTS = 2 * 2 * 256 + 1;  // two fleets x (slabs, a17s) + one scratch slot
-
-char g_last_error[256] = "";
-int hip_fail(hipError_t e, int line) {
+int dne::hip_fail(hipError_t e, int line) {
     snprintf(g_last_error, sizeof(g_last_error), "api.cpp:%d: %s", line, hipGetErrorString(e));
     return DNSPIR_E_HIP;
 }
-#define HIP_OK(x)                                         \
-    do {                                                  \
-        hipError_t e_ = (x);                              \
-        if (e_ != hipSuccess) return hip_fail(e_, __LINE__); \
-    } while (0)
 
-int loop_len(int l) { return l < N ? l : N / 2; }
-int nstreams(int l) { return l == N ? 2 : 1; }
+namespace {
+
+constexpr size_t EXTRA_BYTES = (size_t)LANES * N * 8 + (size_t)LANES * N * 4;  // a_coef + a17
+constexpr size_t PTR_SLOTS = 2 * 2 * 256 + 1;  // two fleets x (slabs, a17s) + one scratch slot
+
 bool valid_group(int l) {
     if (l <= 0 || l > N || (N % l) != 0) return false;
     return l == N || 2 * l <= N;  // base_pir.rs:150-154
 }
-size_t slab_bytes(int l) { return (size_t)6 * N * l * 32; }  // (GK_DIGITS+1)*2*N*l elements of 32 B
-
-// grow-only device buffer
-struct DevBuf {
-    void* p = nullptr;
-    size_t n = 0;
-    int ensure(size_t bytes) {
-        if (bytes <= n) return 0;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-        if (hipMalloc(&p, bytes) != hipSuccess) return DNSPIR_E_OOM;
-        n = bytes;
-        return 0;
-    }
-    template <class X>
-    X* as() const { return reinterpret_cast<X*>(p); }
-    ~DevBuf() {
-        if (p) (void)hipFree(p);
-    }
-};
 
 struct Global {
     std::mutex mu;
@@ -180,29 +155,7 @@ int build_tables(DevTables* T) {
 
 }  // namespace
 
-// ---- shard and context types --------------------------------------------------
-struct dnspir_shard {
-    int l, L, nst, seeds;
-    uint32_t* slab;
-    uint64_t* a_coef;  // [8][N] mod q, the get_a result
-    uint32_t* a17;     // [8][N] a mod-switched to 17 bits
-    bool has_db;
-};
-
-struct dnspir_ctx {
-    hipStream_t stream = nullptr;
-    bool own = false;
-    DevBuf se, M, te, aout, cout, qcoef, qs, qp, acc, outb, ql, acoef, bcoef, a17o, b12o, ptrs, idx, qbytes, rbytes;
-    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // start, primary end, end, k1 start, k1 end
-    int k1_launches = 0;
-    std::vector<const dnspir_shard*> cached[2];
-    std::vector<uint64_t> stage;
-    std::vector<long long> h_idx;  // host staging of the lane lists (outlives the async copy)
-};
-
-namespace {
-
-int ctx_init(dnspir_ctx* c, hipStream_t s) {
+int dne::ctx_init(dnspir_ctx* c, hipStream_t s) {
     if (s) c->stream = s;
     else {
         HIP_OK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
@@ -212,10 +165,21 @@ int ctx_init(dnspir_ctx* c, hipStream_t s) {
     return 0;
 }
 
-int ensure_init() {
+int dne::ctx_presize(dnspir_ctx* c, int max_nq) {
+    int rc;
+    if ((rc = c->ptrs.ensure(PTR_SLOTS * sizeof(void*)))) return rc;
+    if ((rc = c->idx.ensure((size_t)3 * max_nq * (sizeof(long long) + 2 * sizeof(int)) + 64))) return rc;
+    return c->keyp.ensure((size_t)2 * max_nq * sizeof(void*));
+}
+
+int dne::ensure_init() {
     if (g.device < 0 || !g.tables) return DNSPIR_E_NOINIT;
     return hipSetDevice(g.device) == hipSuccess ? 0 : DNSPIR_E_HIP;
 }
+
+const DevTables* dne::tables() { return g.tables; }
+
+namespace {
 
 dnspir_ctx* default_ctx() { return g.default_ctx; }
 
@@ -401,11 +365,23 @@ void fill_timing(dnspir_ctx* c, dnspir_timing* tm, double wall_us) {
     tm->k1_launches = c->k1_launches;
 }
 
-double now_us() {
+}  // namespace
+
+double dne::now_us() {
     return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-}  // namespace
+size_t dne::query_layout(int l_p, int l_s, int offs[6], int bits[6]) {
+    const int b[6] = {GK_BITS, GK_BITS, QRY_BITS, QRY_BITS, QRY_BITS, QRY_BITS};
+    const int pres[6] = {1, 1, 1, l_p == N, 1, l_s == N};
+    size_t off = 0;
+    for (int v = 0; v < 6; ++v) {
+        bits[v] = b[v];
+        offs[v] = pres[v] ? (int)off : -1;
+        if (pres[v]) off += dnh::condensed_size(b[v], N);
+    }
+    return off;
+}
 
 extern "C" {
 
@@ -550,8 +526,8 @@ static int perform_common(const dnspir_shard* sh, const uint64_t* qb, const uint
     if (rc) return rc;
     if (!sh->has_db) return DNSPIR_E_ARG;
     if (sh->nst == 2 && !qbc) return DNSPIR_E_CONJ;  // base_pir.rs:965
-    std::lock_guard<std::mutex> lk(g.mu);
     dnspir_ctx* c = default_ctx();
+    std::lock_guard<std::mutex> lk(c->mu);
     const hipStream_t s = c->stream;
     const int nq = split ? LANES : 1;
     if ((rc = c->qcoef.ensure((size_t)LANES * 6 * N * 8))) return rc;
@@ -675,7 +651,7 @@ int dnspir_double_pir(dnspir_ctx* c, const dnspir_shard* const* f0, size_t len0,
     if (len0 < len1) return DNSPIR_E_SHAPE;  // pir_wrapper.rs:362
     for (int sl = 0; sl < 2; ++sl)
         if (present[sl] && (rc = check_fleet(fleets[sl], lens[sl], l_p))) return rc;
-    std::lock_guard<std::mutex> lk(g.mu);
+    std::lock_guard<std::mutex> lk(c->mu);
     const double t0 = now_us();
     if ((rc = c->qcoef.ensure((size_t)LANES * 6 * N * 8))) return rc;
     const uint64_t* qs[2] = {q0, q1};
@@ -758,7 +734,7 @@ int dnspir_process_queries_device(dnspir_ctx* c, const dnspir_shard* const* f0, 
     int rc = ensure_init();
     if (rc) return rc;
     if (!c) c = default_ctx();
-    std::lock_guard<std::mutex> lk(g.mu);
+    std::lock_guard<std::mutex> lk(c->mu);
     return process_device_impl(c, f0, len0, f1, len1, q0, ql0, q1, ql1, r0, r1, tm, now_us());
 }
 
@@ -768,7 +744,7 @@ int dnspir_process_queries(dnspir_ctx* c, const dnspir_shard* const* f0, size_t 
     int rc = ensure_init();
     if (rc) return rc;
     if (!c) c = default_ctx();
-    std::lock_guard<std::mutex> lk(g.mu);
+    std::lock_guard<std::mutex> lk(c->mu);
     const double t0 = now_us();
     const size_t qcap = 80000;
     if ((rc = c->qbytes.ensure(2 * qcap))) return rc;
@@ -797,34 +773,69 @@ int dnspir_process_queries(dnspir_ctx* c, const dnspir_shard* const* f0, size_t 
     return 0;
 }
 
+}  // extern "C"
+
+size_t dne::batch_scratch_bytes(int l_p, size_t len, int nq, size_t qlen) {
+    dnspir_ctx probe;  // never initialised: only its buffers' sizes are used
+    Arena count;
+    count.cap = ~(size_t)0 >> 1;
+    probe.bind_arena(&count);
+    if (batch_reserve(&probe, l_p, len, nq, qlen)) return 0;
+    const size_t used = count.used;
+    probe.bind_arena(nullptr);
+    return used;
+}
+
+int dne::batch_reserve(dnspir_ctx* c, int l_p, size_t len, int nq, size_t qlen) {
+    const int l_s = dnspir_choose_index_group_secondary(len);
+    const int L_s = loop_len(l_s), nst_s = nstreams(l_s), nst_p = nstreams(l_p);
+    const int nqc = std::min(64, nq), nl = 3 * nq;
+    int rc;
+    if (qlen) {
+        if ((rc = c->qbytes.ensure((size_t)nq * qlen))) return rc;
+        if ((rc = c->rbytes.ensure((size_t)nq * REPLY_BYTES))) return rc;
+    }
+    const size_t sizes[] = {
+        (size_t)nq * 6 * N * 8,                      // qcoef
+        (size_t)nq * 6 * 2 * N * 4,                  // qs
+        (size_t)nq * 4 * 2 * N * 4,                  // qp
+        (size_t)nq * l_s * 3 * N * 2,                // se
+        qlimb_bytes(nqc, nst_p),                     // ql
+        (size_t)nqc * len * 2 * N * LANES * 4,       // outb
+        (size_t)nl * l_s * N * 4,                    // M
+        (size_t)nst_s * nl * L_s * 2 * N * 4,        // te
+        (size_t)nst_s * nl * 2 * N * 4,              // aout
+        (size_t)nst_s * nl * 2 * N * 4,              // cout
+        (size_t)nl * N * 8,                          // acoef
+        (size_t)nl * N * 8,                          // bcoef
+        (size_t)nl * N * 4,                          // a17o
+        (size_t)nl * N * 4,                          // b12o
+    };
+    DevBuf* bufs[] = {&c->qcoef, &c->qs, &c->qp, &c->se, &c->ql, &c->outb, &c->M,
+                      &c->te, &c->aout, &c->cout, &c->acoef, &c->bcoef, &c->a17o, &c->b12o};
+    for (size_t i = 0; i < sizeof(sizes) / sizeof(sizes[0]); ++i)
+        if ((rc = bufs[i]->ensure(sizes[i]))) return rc;
+    return 0;
+}
+
 // ---- batched queries -----------------------------------------------------------
 // nq independent queries against one fleet, each answered as process_query
 // (pir_wrapper.rs:328-337) but sharing one pass over the shards: the primary
 // inner product runs as one MFMA contraction per 64 queries; every query then
 // gets its own secondary database (3 lanes each) and all of those run
 // together. Query q's reply bytes equal its single-query reply.
-static int batch_impl(dnspir_ctx* c, const dnspir_shard* const* fleet, size_t len, const uint8_t* qdev, size_t qlen, int nq,
-                      uint8_t* rdev, dnspir_timing* tm, double t0) {
+int dne::batch_impl(dnspir_ctx* c, const dnspir_shard* const* fleet, size_t len, const uint8_t* qdev, size_t qlen, int nq,
+                    uint8_t* rdev, dnspir_timing* tm, double t0, const BatchKeys* keys) {
     int rc;
     if (!fleet || len == 0 || !fleet[0] || nq <= 0 || !qdev || !rdev) return DNSPIR_E_ARG;
     const int l_p = fleet[0]->l;
     if (loop_len(l_p) != N / 2) return DNSPIR_E_SHAPE;
     if ((rc = check_fleet(fleet, len, l_p))) return rc;
     const int l_s = dnspir_choose_index_group_secondary(len);
-    const int conj_p = l_p == N, conj_s = l_s == N;
-    int offs[6], bits[6] = {GK_BITS, GK_BITS, QRY_BITS, QRY_BITS, QRY_BITS, QRY_BITS};
-    int pres[6] = {1, 1, 1, conj_p, 1, conj_s};
-    size_t off = 0;
-    for (int v = 0; v < 6; ++v) {
-        offs[v] = pres[v] ? (int)off : -1;
-        if (pres[v]) off += dnh::condensed_size(bits[v], N);
-    }
-    if (qlen < off) return DNSPIR_E_SHORT_QUERY;
+    int offs[6], bits[6];
+    if (qlen < query_layout(l_p, l_s, offs, bits)) return DNSPIR_E_SHORT_QUERY;
     const hipStream_t s = c->stream;
-    if ((rc = c->qcoef.ensure((size_t)nq * 6 * N * 8))) return rc;
-    if ((rc = c->qs.ensure((size_t)nq * 6 * 2 * N * 4))) return rc;
-    if ((rc = c->qp.ensure((size_t)nq * 4 * 2 * N * 4))) return rc;
-    if ((rc = c->se.ensure((size_t)nq * l_s * 3 * N * 2))) return rc;
+    if ((rc = batch_reserve(c, l_p, len, nq, 0))) return rc;
     const uint32_t* const* slabs;
     const uint32_t* const* a17s;
     if ((rc = upload_fleet(c, 0, fleet, len, &slabs, &a17s))) return rc;
@@ -842,6 +853,18 @@ static int batch_impl(dnspir_ctx* c, const dnspir_shard* const* fleet, size_t le
     launch_decode_query(qdev, qlen, nq, offs, bits, c->qcoef.as<uint64_t>(), s);
     HIP_OK(hipEventRecord(c->ev[0], s));
     launch_qprep(c->qcoef.as<uint64_t>(), c->qs.as<uint32_t>(), c->qp.as<uint32_t>(), nq, g.tables, s);
+    if (keys && (keys->src || keys->dst)) {
+        // cached keys replace the decoded ones; fresh ones are stored
+        if ((rc = c->keyp.ensure((size_t)2 * nq * sizeof(void*)))) return rc;
+        c->h_keyp.assign((size_t)2 * nq, nullptr);
+        for (int q = 0; q < nq; ++q) {
+            if (keys->src) c->h_keyp[q] = keys->src[q];
+            if (keys->dst) c->h_keyp[nq + q] = keys->dst[q];
+        }
+        HIP_OK(hipMemcpyAsync(c->keyp.p, c->h_keyp.data(), (size_t)2 * nq * sizeof(void*), hipMemcpyHostToDevice, s));
+        launch_key_cache(c->keyp.as<const uint32_t*>(), c->keyp.as<uint32_t*>() + nq, nq, c->qs.as<uint32_t>(),
+                         c->qp.as<uint32_t>(), s);
+    }
     c->k1_launches = 0;
     for (int q0 = 0; q0 < nq; q0 += 64) {
         const int nqc = std::min(64, nq - q0);
@@ -873,27 +896,29 @@ static int batch_impl(dnspir_ctx* c, const dnspir_shard* const* fleet, size_t le
     return 0;
 }
 
-extern "C" int dnspir_process_batch_device(dnspir_ctx* c, const dnspir_shard* const* fleet, size_t len, const uint8_t* queries_dev,
+extern "C" {
+
+int dnspir_process_batch_device(dnspir_ctx* c, const dnspir_shard* const* fleet, size_t len, const uint8_t* queries_dev,
                                            size_t qlen, int nq, uint8_t* replies_dev, dnspir_timing* tm) {
     int rc = ensure_init();
     if (rc) return rc;
     if (!c) c = default_ctx();
-    std::lock_guard<std::mutex> lk(g.mu);
-    return batch_impl(c, fleet, len, queries_dev, qlen, nq, replies_dev, tm, now_us());
+    std::lock_guard<std::mutex> lk(c->mu);
+    return batch_impl(c, fleet, len, queries_dev, qlen, nq, replies_dev, tm, now_us(), nullptr);
 }
 
-extern "C" int dnspir_process_batch(dnspir_ctx* c, const dnspir_shard* const* fleet, size_t len, const uint8_t* queries,
+int dnspir_process_batch(dnspir_ctx* c, const dnspir_shard* const* fleet, size_t len, const uint8_t* queries,
                                     size_t qlen, int nq, uint8_t* replies, dnspir_timing* tm) {
     int rc = ensure_init();
     if (rc) return rc;
     if (!c) c = default_ctx();
     if (!queries || !replies || nq <= 0) return DNSPIR_E_ARG;
-    std::lock_guard<std::mutex> lk(g.mu);
+    std::lock_guard<std::mutex> lk(c->mu);
     const double t0 = now_us();
     if ((rc = c->qbytes.ensure((size_t)nq * qlen + 16))) return rc;
     if ((rc = c->rbytes.ensure((size_t)nq * REPLY_BYTES))) return rc;
     HIP_OK(hipMemcpyAsync(c->qbytes.p, queries, (size_t)nq * qlen, hipMemcpyHostToDevice, c->stream));
-    rc = batch_impl(c, fleet, len, c->qbytes.as<uint8_t>(), qlen, nq, c->rbytes.as<uint8_t>(), nullptr, t0);
+    rc = batch_impl(c, fleet, len, c->qbytes.as<uint8_t>(), qlen, nq, c->rbytes.as<uint8_t>(), nullptr, t0, nullptr);
     if (rc) return rc;
     HIP_OK(hipMemcpy(replies, c->rbytes.p, (size_t)nq * REPLY_BYTES, hipMemcpyDeviceToHost));
     fill_timing(c, tm, now_us() - t0);
@@ -910,21 +935,45 @@ struct dnspir_rng {
 dnspir_rng* dnspir_rng_new(const uint8_t seed[32]) { return seed ? new (std::nothrow) dnspir_rng(seed) : nullptr; }
 void dnspir_rng_free(dnspir_rng* r) { delete r; }
 
-int dnspir_prepare_query(dnspir_rng* rng, size_t pidx, size_t sidx, int lp, int ls, uint8_t* query, size_t* qlen,
-                         uint8_t sk_seed[32]) {
-    if (!rng || !query || !qlen || !sk_seed) return DNSPIR_E_ARG;
-    if (!valid_group(lp) || !valid_group(ls)) return DNSPIR_E_SHAPE;
-    if (pidx >= (size_t)lp * LANES || sidx >= (size_t)ls / LANES) return DNSPIR_E_ARG;
+}  // extern "C"
+
+namespace {
+
+void client_sk(const uint8_t sk_seed[32], std::vector<int64_t>& sk) {
+    sk.assign(N, 0);
+    dnh::ChaCha12 skr(sk_seed);
+    dnh::gen_sk(skr, 128, sk.data());
+}
+
+size_t emit_condensed(const uint64_t* x, int bits, uint8_t* out) {
+    std::vector<uint64_t> tmp(N);
+    for (int k = 0; k < N; ++k) tmp[k] = dnh::mod_switch_encode(x[k], bits);
+    return dnh::condense(bits, tmp.data(), N, out);
+}
+
+// prepare_query_general (pir_wrapper.rs:276-317), key part: sk seed drawn
+// from the rng (:289), Galois keys gk0 || gk1 (:291-293, :305-306)
+int client_keys(dnspir_rng* rng, uint8_t sk_seed[32], uint8_t* keys, size_t* n) {
     for (int i = 0; i < 32; ++i) sk_seed[i] = (uint8_t)(rng->r.next_u32() & 0xFFFF);
-    std::vector<int64_t> sk(N);
-    {
-        dnh::ChaCha12 skr(sk_seed);
-        dnh::gen_sk(skr, 128, sk.data());
-    }
+    std::vector<int64_t> sk;
+    client_sk(sk_seed, sk);
+    std::vector<uint64_t> gk(2 * N);
+    dnh::gen_gk_b(rng->r, sk.data(), gk.data());
+    *n = emit_condensed(gk.data(), GK_BITS, keys);
+    *n += emit_condensed(gk.data() + N, GK_BITS, keys + *n);
+    return 0;
+}
+
+// query part: primary X^(i*kp) [and conj], secondary X^(j*ks) [and conj]
+// (double_pir.rs:245-317, pir_wrapper.rs:294-315)
+int client_query(dnspir_rng* rng, const uint8_t sk_seed[32], size_t pidx, size_t sidx, int lp, int ls, uint8_t* query,
+                 size_t* qlen) {
+    std::vector<int64_t> sk;
+    client_sk(sk_seed, sk);
     const int kp = N / 2 / loop_len(lp), ks = N / 2 / loop_len(ls);
     const int64_t i = (int64_t)(pidx / LANES);
     const int64_t j = (int64_t)(sidx * LANES + pidx % LANES);
-    std::vector<uint64_t> gk(2 * N), ct(N), tmp(N);
+    std::vector<uint64_t> ct(N);
     std::vector<uint16_t> mono(N);
     auto set_mono = [&](int64_t e) {
         std::fill(mono.begin(), mono.end(), 0);
@@ -933,14 +982,6 @@ int dnspir_prepare_query(dnspir_rng* rng, size_t pidx, size_t sidx, int lp, int 
         if (e < N) mono[e] = 1;
         else mono[e - N] = (uint16_t)(dnh::T - 1);
     };
-    dnh::gen_gk_b(rng->r, sk.data(), gk.data());
-    size_t n = 0;
-    auto emit = [&](const uint64_t* x, int bits) {
-        for (int k = 0; k < N; ++k) tmp[k] = dnh::mod_switch_encode(x[k], bits);
-        n += dnh::condense(bits, tmp.data(), N, query + n);
-    };
-    emit(gk.data(), GK_BITS);
-    emit(gk.data() + N, GK_BITS);
     set_mono(i * kp);
     dnh::enc_sym_b(dnh::SEEDS[2], rng->r, sk.data(), mono.data(), ct.data());
     std::vector<uint64_t> pq = ct, pqc, sq, sqc;
@@ -957,11 +998,42 @@ int dnspir_prepare_query(dnspir_rng* rng, size_t pidx, size_t sidx, int lp, int 
         dnh::enc_sym_b(dnh::SEEDS[5], rng->r, sk.data(), mono.data(), ct.data());
         sqc = ct;
     }
-    emit(pq.data(), QRY_BITS);
-    if (lp == N) emit(pqc.data(), QRY_BITS);
-    emit(sq.data(), QRY_BITS);
-    if (ls == N) emit(sqc.data(), QRY_BITS);
+    size_t n = 0;
+    n += emit_condensed(pq.data(), QRY_BITS, query + n);
+    if (lp == N) n += emit_condensed(pqc.data(), QRY_BITS, query + n);
+    n += emit_condensed(sq.data(), QRY_BITS, query + n);
+    if (ls == N) n += emit_condensed(sqc.data(), QRY_BITS, query + n);
     *qlen = n;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dnspir_client_keys(dnspir_rng* rng, uint8_t sk_seed[32], uint8_t* keys) {
+    if (!rng || !sk_seed || !keys) return DNSPIR_E_ARG;
+    size_t n = 0;
+    return client_keys(rng, sk_seed, keys, &n);
+}
+
+int dnspir_client_query(dnspir_rng* rng, const uint8_t sk_seed[32], size_t pidx, size_t sidx, int lp, int ls,
+                        uint8_t* query, size_t* qlen) {
+    if (!rng || !sk_seed || !query || !qlen) return DNSPIR_E_ARG;
+    if (!valid_group(lp) || !valid_group(ls)) return DNSPIR_E_SHAPE;
+    if (pidx >= (size_t)lp * LANES || sidx >= (size_t)ls / LANES) return DNSPIR_E_ARG;
+    return client_query(rng, sk_seed, pidx, sidx, lp, ls, query, qlen);
+}
+
+int dnspir_prepare_query(dnspir_rng* rng, size_t pidx, size_t sidx, int lp, int ls, uint8_t* query, size_t* qlen,
+                         uint8_t sk_seed[32]) {
+    if (!rng || !query || !qlen || !sk_seed) return DNSPIR_E_ARG;
+    if (!valid_group(lp) || !valid_group(ls)) return DNSPIR_E_SHAPE;
+    if (pidx >= (size_t)lp * LANES || sidx >= (size_t)ls / LANES) return DNSPIR_E_ARG;
+    size_t nk = 0, nq = 0;
+    client_keys(rng, sk_seed, query, &nk);
+    client_query(rng, sk_seed, pidx, sidx, lp, ls, query + nk, &nq);
+    *qlen = nk + nq;
     return 0;
 }
 

@@ -1,0 +1,162 @@
+// Internal types shared by the C ABI (api.cpp) and the serving loop
+// (serve.cpp): shard and context objects, device scratch buffers, and the
+// batched query pipeline.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+
+#include <mutex>
+#include <vector>
+
+#include "../../include/dnspir.h"
+#include "dn_common.hpp"
+
+namespace dne {
+
+using dn::LANES;
+using dn::N;
+
+// wire widths (pir_wrapper.rs:57-72)
+constexpr int A_BITS = 17, B_BITS = 12, GK_BITS = 50, QRY_BITS = 34;
+constexpr size_t REPLY_BYTES = 3 * (N * A_BITS / 8 + N * B_BITS / 8);
+// the two Galois keys at the head of every query (pir_wrapper.rs:370-372)
+constexpr size_t KEY_BYTES = 2 * (N / 8) * GK_BITS;
+// a device-cached key: NTT rows of gk0, gk1 in both moduli, in slot order
+// (secondary phase) and in Galois-position order (primary phase)
+constexpr size_t KEY_ENTRY_WORDS = 2 * 2 * 2 * N;
+
+int hip_fail(hipError_t e, int line);
+#define HIP_OK(x)                                            \
+    do {                                                     \
+        hipError_t e_ = (x);                                 \
+        if (e_ != hipSuccess) return dne::hip_fail(e_, __LINE__); \
+    } while (0)
+
+inline int loop_len(int l) { return l < N ? l : N / 2; }
+inline int nstreams(int l) { return l == N ? 2 : 1; }
+inline size_t slab_bytes(int l) { return (size_t)6 * N * l * 32; }  // (GK_DIGITS+1)*2*N*l elements of 32 B
+
+// A bump region of device memory that a context's scratch buffers are carved
+// from (instead of hipMalloc) while it is bound. base == nullptr with a huge
+// cap only counts bytes (sizing pass).
+struct Arena {
+    char* base = nullptr;
+    size_t cap = 0, used = 0;
+};
+
+// grow-only device buffer, from hipMalloc or from the bound arena
+struct DevBuf {
+    void* p = nullptr;
+    size_t n = 0;
+    Arena* arena = nullptr;
+    int ensure(size_t bytes) {
+        if (bytes <= n) return 0;
+        if (arena) {
+            const size_t off = (arena->used + 255) & ~(size_t)255;
+            if (off + bytes > arena->cap) return DNSPIR_E_OOM;
+            p = arena->base + off;
+            n = bytes;
+            arena->used = off + bytes;
+            return 0;
+        }
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        if (hipMalloc(&p, bytes) != hipSuccess) return DNSPIR_E_OOM;
+        n = bytes;
+        return 0;
+    }
+    // forget the arena storage (the arena owns it)
+    void detach() {
+        if (arena) {
+            p = nullptr;
+            n = 0;
+        }
+    }
+    template <class X>
+    X* as() const {
+        return reinterpret_cast<X*>(p);
+    }
+    ~DevBuf() {
+        if (p && !arena) (void)hipFree(p);
+    }
+};
+
+// Per-query Galois-key sources of one batch: for query q, key_src[q] (a
+// cache entry to read its keys from instead of its bytes) and key_dst[q] (a
+// cache entry to store the keys decoded from its bytes), each may be null.
+struct BatchKeys {
+    const uint32_t* const* src = nullptr;
+    uint32_t* const* dst = nullptr;
+};
+
+int ensure_init();
+const dn::DevTables* tables();
+double now_us();
+
+// Offsets of the six query vectors in the wire format (pir_wrapper.rs:370-384):
+// gk0 gk1 (50 bits) || primary (34) [|| conj] || secondary (34) [|| conj];
+// offs[v] = -1 for an absent conjugate. Returns the total length.
+size_t query_layout(int l_p, int l_s, int offs[6], int bits[6]);
+
+}  // namespace dne
+
+struct dnspir_shard {
+    int l, L, nst, seeds;
+    uint32_t* slab;
+    uint64_t* a_coef;  // [8][N] mod q, the get_a result
+    uint32_t* a17;     // [8][N] a mod-switched to 17 bits
+    bool has_db;
+};
+
+struct dnspir_ctx {
+    std::mutex mu;  // one call at a time per context
+    hipStream_t stream = nullptr;
+    bool own = false;
+    // size-dependent scratch (arena-capable)
+    dne::DevBuf se, M, te, aout, cout, qcoef, qs, qp, acc, outb, ql, acoef, bcoef, a17o, b12o, qbytes, rbytes;
+    // small persistent buffers (pointer lists, lane lists)
+    dne::DevBuf ptrs, idx, keyp;
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // start, primary end, end, k1 start, k1 end
+    int k1_launches = 0;
+    std::vector<const dnspir_shard*> cached[2];
+    std::vector<uint64_t> stage;
+    std::vector<long long> h_idx;  // host staging of the lane lists (outlives the async copy)
+    std::vector<const void*> h_keyp;
+
+    dne::DevBuf* scratch_bufs[17] = {&se, &M, &te, &aout, &cout, &qcoef, &qs, &qp, &acc,
+                                     &outb, &ql, &acoef, &bcoef, &a17o, &b12o, &qbytes, &rbytes};
+    // carve all scratch from `a` from now on (nullptr = back to hipMalloc);
+    // storage the buffers held before is released or forgotten
+    void bind_arena(dne::Arena* a) {
+        for (auto* b : scratch_bufs) {
+            if (!b->arena && b->p) (void)hipFree(b->p);
+            b->p = nullptr;
+            b->n = 0;
+            b->arena = a;
+        }
+    }
+};
+
+namespace dne {
+
+int ctx_init(dnspir_ctx* c, hipStream_t s);
+// sizes the persistent pointer / lane-list buffers for batches of up to max_nq
+int ctx_presize(dnspir_ctx* c, int max_nq);
+
+// nq queries (packed at stride qlen in device memory, full wire format)
+// against one fleet; replies packed REPLY_BYTES each at rdev. Caller holds
+// the context's lock.
+int batch_impl(dnspir_ctx* c, const dnspir_shard* const* fleet, size_t len, const uint8_t* qdev, size_t qlen, int nq,
+               uint8_t* rdev, dnspir_timing* tm, double t0, const BatchKeys* keys);
+
+// device scratch bytes batch_impl takes from an arena for nq queries
+// against a fleet of `len` shards of order l_p, plus the query and reply
+// staging (qbytes, rbytes) of a serving slot
+size_t batch_scratch_bytes(int l_p, size_t len, int nq, size_t qlen);
+// carves the staging and every scratch buffer batch_impl needs
+int batch_reserve(dnspir_ctx* c, int l_p, size_t len, int nq, size_t qlen);
+
+}  // namespace dne

@@ -789,6 +789,21 @@ __global__ __launch_bounds__(256) void qprep_kernel(const uint64_t* __restrict__
     }
 }
 
+// Device cache of per-client Galois keys. Query q's key rows are rows 0-1
+// (gk0, gk1, both moduli) of its qs and qp blocks, 4N words each. An entry
+// holds [qs rows 0-1 | qp rows 0-1]. src[q] set: the entry replaces the rows
+// decoded from the query bytes; else dst[q] set: the rows are stored into it.
+__global__ __launch_bounds__(256) void key_cache_kernel(const uint32_t* const* __restrict__ src, uint32_t* const* __restrict__ dst,
+                                                        uint32_t* __restrict__ qs, uint32_t* __restrict__ qp) {
+    const int q = blockIdx.y, part = blockIdx.z;
+    u32x4* rows = reinterpret_cast<u32x4*>(part == 0 ? qs + (size_t)q * 6 * 2 * N : qp + (size_t)q * 4 * 2 * N);
+    const int i = blockIdx.x * 256 + threadIdx.x;  // one of N 16-byte words
+    const u32x4* s = reinterpret_cast<const u32x4*>(src[q]);
+    u32x4* d = reinterpret_cast<u32x4*>(dst[q]);
+    if (s) rows[i] = s[part * N + i];
+    else if (d) d[part * N + i] = rows[i];
+}
+
 // ---------------------------------------------------------------------------
 // K1: the primary inner product (main_pir_loop, base_pir.rs:1085-1247) in
 // closed form  c[g] = sum_i sum_arr DBrot_arr[i][g] * Q_arr[rot_i(g)].
@@ -1104,6 +1119,10 @@ void launch_chain_out(const uint32_t* a_out, const uint32_t* c_out, int nst, int
 
 void launch_qprep(const uint64_t* qcoef, uint32_t* qs, uint32_t* qp, int nq, const DevTables* T, hipStream_t s) {
     hipLaunchKernelGGL(qprep_kernel, dim3(6, 2, nq), dim3(256), 0, s, qcoef, qs, qp, T);
+}
+
+void launch_key_cache(const uint32_t* const* src, uint32_t* const* dst, int nq, uint32_t* qs, uint32_t* qp, hipStream_t s) {
+    hipLaunchKernelGGL(key_cache_kernel, dim3(N / 256, nq, 2), dim3(256), 0, s, src, dst, qs, qp);
 }
 
 void launch_k1(const uint32_t* const* slabs, int nshards, const uint32_t* qp, unsigned long long* acc, int L, int nst,

@@ -33,6 +33,9 @@ void launch_k1b(const uint32_t* const* slabs, int nshards, const uint32_t* qp, i
 void launch_chain_out(const uint32_t* a_out, const uint32_t* c_out, int nst, int nlanes, uint64_t* a_coef, uint64_t* b_coef,
                       uint32_t* a17, uint32_t* b12, const DevTables* T, hipStream_t s);
 void launch_qprep(const uint64_t* qcoef, uint32_t* qs, uint32_t* qp, int nq, const DevTables* T, hipStream_t s);
+// per-query Galois-key cache copies (src[q] -> key rows, else key rows -> dst[q]);
+// src, dst: device arrays of nq entry pointers (null = none)
+void launch_key_cache(const uint32_t* const* src, uint32_t* const* dst, int nq, uint32_t* qs, uint32_t* qp, hipStream_t s);
 void launch_k1(const uint32_t* const* slabs, int nshards, const uint32_t* qp, unsigned long long* acc, int L, int nst,
                int rows_per_block, hipStream_t s);
 void launch_k1_split(const uint32_t* const* slabs, int nshards, const uint32_t* qpl, unsigned long long* acc, int L, int nst,

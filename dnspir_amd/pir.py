@@ -14,7 +14,8 @@ import ctypes
 
 import numpy as np
 
-from ._lib import LANES, N, DnspirError, Timing, check, lib
+from ._lib import (LANES, N, QUERY_KEYLESS, Arrival, DnspirError, RequestInfo, ServerConfig, ServerStats, Timing, check,
+                   lib)
 
 _ptr = lambda a, t: a.ctypes.data_as(t)  # noqa: E731
 u8p = ctypes.POINTER(ctypes.c_uint8)
@@ -262,6 +263,106 @@ def prepare_query(rng, primary_idx, secondary_idx, l_primary, l_secondary):
     check(lib().dnspir_prepare_query(rng.h, primary_idx, secondary_idx, l_primary, l_secondary, _ptr(out, u8p),
                                      ctypes.byref(n), _ptr(sk, u8p)), "prepare_query")
     return bytes(out[:n.value]), bytes(sk)
+
+
+def key_bytes():
+    return lib().dnspir_key_bytes()
+
+
+def client_keys(rng):
+    """Key half of prepare_query: -> (key bytes, sk seed)."""
+    sk = np.zeros(32, np.uint8)
+    out = np.zeros(key_bytes(), np.uint8)
+    check(lib().dnspir_client_keys(rng.h, _ptr(sk, u8p), _ptr(out, u8p)), "client_keys")
+    return bytes(out), bytes(sk)
+
+
+def client_query(rng, sk_seed, primary_idx, secondary_idx, l_primary, l_secondary):
+    """Query half of prepare_query (no key prefix) for a client's long-lived key."""
+    sk = np.frombuffer(bytes(sk_seed), np.uint8).copy()
+    out = np.zeros(query_bytes(l_primary, l_secondary) + 16, np.uint8)
+    n = ctypes.c_size_t()
+    check(lib().dnspir_client_query(rng.h, _ptr(sk, u8p), primary_idx, secondary_idx, l_primary, l_secondary,
+                                    _ptr(out, u8p), ctypes.byref(n)), "client_query")
+    return bytes(out[:n.value])
+
+
+def batch_scratch_bytes(l_primary, num_shards, nq):
+    return lib().dnspir_batch_scratch_bytes(l_primary, num_shards, nq)
+
+
+class Server:
+    """Continuous-batching request loop over resident databases
+    (dnspir_server_*): slots, HBM scratch budget, device Galois-key cache."""
+
+    def __init__(self, slots=3, max_batch=64, scratch_budget=8 << 30, key_cache_entries=256):
+        init()
+        cfg = ServerConfig(slots, max_batch, scratch_budget, key_cache_entries)
+        h = ctypes.c_void_p()
+        check(lib().dnspir_server_create(ctypes.byref(cfg), ctypes.byref(h)), "dnspir_server_create")
+        self.h = h
+        self._fleets = []
+
+    def add_database(self, fleet):
+        arr, n = _fleet(fleet)
+        db = ctypes.c_int()
+        check(lib().dnspir_server_add_database(self.h, arr, n, ctypes.byref(db)), "server_add_database")
+        self._fleets.append(list(fleet))  # the shards must outlive the server
+        return db.value
+
+    def max_batch(self, db):
+        rc = lib().dnspir_server_max_batch(self.h, db)
+        if rc < 0:
+            raise DnspirError(rc, "server_max_batch")
+        return rc
+
+    def submit(self, db, client, query, keyless=False):
+        """Queues one query; returns a ticket (raises DnspirError, e.g.
+        DNSPIR_E_NOKEY for a keyless query whose keys are not cached)."""
+        q = np.frombuffer(bytes(query), np.uint8).copy()
+        t = ctypes.c_uint64()
+        check(lib().dnspir_server_submit(self.h, db, client, _ptr(q, u8p), q.size, QUERY_KEYLESS if keyless else 0,
+                                         ctypes.byref(t)), "server_submit")
+        return t.value
+
+    def wait(self, ticket):
+        """-> (reply bytes, RequestInfo)"""
+        out = np.zeros(reply_bytes(), np.uint8)
+        info = RequestInfo()
+        check(lib().dnspir_server_wait(self.h, ticket, _ptr(out, u8p), ctypes.byref(info)), "server_wait")
+        return bytes(out), info
+
+    def replay(self, arrivals):
+        """arrivals: list of (db, client, arrival_us, keyless query bytes, key
+        bytes). Submits each at its time (open loop), waits for all; returns
+        (replies as one uint8 array [n][reply_bytes], list of RequestInfo)."""
+        n = len(arrivals)
+        arr = (Arrival * n)()
+        keep = []
+        for i, (db, client, t_us, q, keys) in enumerate(arrivals):
+            qb = ctypes.create_string_buffer(bytes(q), len(q))
+            kb = ctypes.create_string_buffer(bytes(keys), len(keys)) if keys is not None else None
+            keep.append((qb, kb))
+            arr[i] = Arrival(db, client, t_us, ctypes.cast(qb, ctypes.c_void_p), len(q),
+                             ctypes.cast(kb, ctypes.c_void_p) if kb is not None else None)
+        replies = np.zeros((n, reply_bytes()), np.uint8)
+        infos = (RequestInfo * n)()
+        lib().dnspir_server_replay(self.h, arr, n, _ptr(replies, u8p), infos)
+        return replies, list(infos)
+
+    def stats(self):
+        st = ServerStats()
+        check(lib().dnspir_server_get_stats(self.h, ctypes.byref(st)), "server_get_stats")
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def close(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value:
+            lib().dnspir_server_destroy(h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
 
 
 def process_reply(reply, sk_seed):

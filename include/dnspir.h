@@ -33,6 +33,8 @@ extern "C" {
 #define DNSPIR_E_HIP (-8)         /* a HIP runtime call failed */
 #define DNSPIR_E_DECODE (-9)      /* reply digits out of range, double_pir.rs:88 */
 #define DNSPIR_E_NOINIT (-10)     /* dnspir_init not called / no device */
+#define DNSPIR_E_NOKEY (-11)      /* keyless query, but the client's Galois keys are not cached */
+#define DNSPIR_E_CLOSED (-12)     /* the server is shutting down */
 
 typedef struct dnspir_shard dnspir_shard;
 typedef struct dnspir_ctx dnspir_ctx;
@@ -140,6 +142,86 @@ int dnspir_choose_index_group_secondary(size_t num_shards);
 size_t dnspir_query_bytes(int l_primary, int l_secondary);
 size_t dnspir_reply_bytes(void);
 
+/* ---- serving: continuous batching over resident databases -----------------
+ * No counterpart in the reference, whose server answers each query on its
+ * connection thread (dns/server.rs:747-807 -> process_query). Requests queue
+ * per database; `slots` workers (one query context and HIP stream each) take
+ * up to max_batch waiting requests of the database with the oldest request
+ * whenever they are free, if the batch's device scratch fits in the free part
+ * of a `scratch_budget`-byte arena allocated once (admission by HBM budget;
+ * no hipMalloc while serving). Batches of different slots overlap on the
+ * device. Each reply is byte-identical to the query's process_queries reply.
+ * Per-client Galois keys stay in HBM (key_cache_entries clients, LRU): after
+ * one full query a client may send keyless ones (its query without the
+ * dnspir_key_bytes() key prefix, flag DNSPIR_QUERY_KEYLESS); once its entry is
+ * evicted they fail with DNSPIR_E_NOKEY and the client sends the full query. */
+typedef struct dnspir_server dnspir_server;
+
+#define DNSPIR_QUERY_KEYLESS 1
+
+typedef struct {
+    int slots;               /* batches on the device at once, one stream each (1..8) */
+    int max_batch;           /* queries per batch (1..128) */
+    size_t scratch_budget;   /* device bytes all in-flight batches may hold together */
+    int key_cache_entries;   /* clients whose Galois keys stay on the device (0 = none) */
+} dnspir_server_config;
+
+typedef struct {
+    int status;              /* 0 or DNSPIR_E_* */
+    int db;
+    int batch_size;          /* queries in the batch that answered the request */
+    int key_cached;          /* 1: the Galois keys came from the device cache */
+    int resent;              /* dnspir_server_replay: 1 = keys sent after DNSPIR_E_NOKEY */
+    double submit_us;        /* monotonic clock, microseconds */
+    double start_us;         /* batch admitted */
+    double done_us;          /* reply in host memory */
+} dnspir_request_info;
+
+typedef struct {
+    uint64_t completed, failed, batches;
+    uint64_t key_hits;        /* keyless queries served from the cache */
+    uint64_t key_inserts;     /* keys stored from full queries */
+    uint64_t key_evictions;
+    uint64_t key_misses;      /* keyless queries answered DNSPIR_E_NOKEY */
+    uint64_t admission_waits; /* times waiting requests had to wait for scratch */
+    size_t scratch_in_use_peak;
+    int inflight_peak;        /* most batches on the device at once */
+} dnspir_server_stats;
+
+/* one request of an open-loop load (dnspir_server_replay) */
+typedef struct {
+    int db;
+    uint64_t client;
+    double arrival_us;        /* submit time relative to the start of the replay */
+    const uint8_t* query;     /* keyless query bytes */
+    size_t qlen;
+    const uint8_t* keys;      /* the client's key bytes, sent after DNSPIR_E_NOKEY */
+} dnspir_arrival;
+
+/* length of the Galois-key prefix of a wire query (2 x 12,800 B) */
+size_t dnspir_key_bytes(void);
+/* device scratch of one batch of nq queries against num_shards shards of
+ * order l_primary (for sizing scratch_budget); 0 = unsupported shape */
+size_t dnspir_batch_scratch_bytes(int l_primary, size_t num_shards, int nq);
+int dnspir_server_create(const dnspir_server_config* cfg, dnspir_server** out);
+/* registers a fleet (half- or full-size shards) before the first submit */
+int dnspir_server_add_database(dnspir_server* s, const dnspir_shard* const* fleet, size_t len, int* db);
+/* the largest batch of database `db` whose scratch fits the budget */
+int dnspir_server_max_batch(dnspir_server* s, int db);
+/* queues one query (bytes copied); returns at once */
+int dnspir_server_submit(dnspir_server* s, int db, uint64_t client, const uint8_t* query, size_t qlen, int flags,
+                         uint64_t* ticket);
+/* blocks until the request is answered; reply: dnspir_reply_bytes() or NULL */
+int dnspir_server_wait(dnspir_server* s, uint64_t ticket, uint8_t* reply, dnspir_request_info* info);
+/* submits arrivals[i] at arrival_us (keyless first, full after DNSPIR_E_NOKEY),
+ * then waits for all; replies: n * dnspir_reply_bytes() or NULL */
+int dnspir_server_replay(dnspir_server* s, const dnspir_arrival* arrivals, size_t n, uint8_t* replies,
+                         dnspir_request_info* infos);
+int dnspir_server_get_stats(dnspir_server* s, dnspir_server_stats* out);
+/* stops the workers after their current batches; queued requests end with
+ * DNSPIR_E_CLOSED. Call after every dnspir_server_wait has returned. */
+int dnspir_server_destroy(dnspir_server* s);
+
 /* ---- client side (host only): prepare_query / process_reply --------------- */
 /* StdRng stand-in: ChaCha12 in rand_chacha 0.3 word order */
 dnspir_rng* dnspir_rng_new(const uint8_t seed[32]);
@@ -148,6 +230,13 @@ void dnspir_rng_free(dnspir_rng* rng);
  * dnspir_query_bytes(l_primary, l_secondary) bytes */
 int dnspir_prepare_query(dnspir_rng* rng, size_t primary_idx, size_t secondary_idx, int l_primary, int l_secondary,
                          uint8_t* query, size_t* qlen, uint8_t sk_seed[32]);
+/* The two halves of prepare_query for a client with long-lived keys: draws
+ * the secret-key seed and writes the Galois-key bytes (dnspir_key_bytes());
+ * then each dnspir_client_query writes one keyless query for that key.
+ * keys || client_query bytes equal prepare_query's bytes for the same rng. */
+int dnspir_client_keys(dnspir_rng* rng, uint8_t sk_seed[32], uint8_t* keys);
+int dnspir_client_query(dnspir_rng* rng, const uint8_t sk_seed[32], size_t primary_idx, size_t secondary_idx,
+                        int l_primary, int l_secondary, uint8_t* query, size_t* qlen);
 /* process_reply (pir_wrapper.rs:420-440): out [N] coefficients mod 1025 */
 int dnspir_process_reply(const uint8_t* reply, size_t rlen, const uint8_t sk_seed[32], uint16_t* out);
 

@@ -82,3 +82,36 @@ def test_device_calls_without_init_fail_cleanly():
     assert _lib.lib().dnspir_init(0) == -10
     h = __import__("ctypes").c_void_p()
     assert _lib.lib().dnspir_ctx_create(None, __import__("ctypes").byref(h)) == -10
+
+
+@pytest.mark.parametrize("shape", [(1024, 32), (2048, 2048), (1024, 256)])
+def test_client_keys_plus_query_equal_prepare_query(shape):
+    """client_keys || client_query consume the rng as prepare_query does."""
+    lp, ls = shape
+    r1, r2 = pir.Rng(bytes([9] * 32)), pir.Rng(bytes([9] * 32))
+    for pi, si in [(5, 0), (8 * lp - 1, ls // 8 - 1)]:
+        q, sk = pir.prepare_query(r1, pi, si, lp, ls)
+        k, sk2 = pir.client_keys(r2)
+        assert sk == sk2 and len(k) == pir.key_bytes() == 25600
+        assert k + pir.client_query(r2, sk2, pi, si, lp, ls) == q
+
+
+def test_batch_scratch_bytes():
+    """sizing of serving admission: grows with the batch, zero for shapes
+    the batched path does not take"""
+    b = [pir.batch_scratch_bytes(2048, 256, n) for n in (1, 2, 64, 65, 128)]
+    assert all(x > 0 for x in b) and b == sorted(b)
+    # the secondary database alone: 3 lanes x l_s columns x N u16 per query
+    assert b[1] - b[0] > 3 * 2048 * 2048 * 2
+    assert pir.batch_scratch_bytes(512, 4, 1) == 0      # rotated shard layout
+    assert pir.batch_scratch_bytes(1024, 0, 1) == 0
+    assert pir.batch_scratch_bytes(1024, 257, 1) == 0
+
+
+def test_server_needs_a_device():
+    import ctypes
+    cfg = _lib.ServerConfig(1, 1, 1 << 20, 0)
+    h = ctypes.c_void_p()
+    assert _lib.lib().dnspir_server_create(ctypes.byref(cfg), ctypes.byref(h)) == -10
+    bad = _lib.ServerConfig(0, 1, 1 << 20, 0)
+    assert _lib.lib().dnspir_server_create(ctypes.byref(bad), ctypes.byref(h)) == -5

@@ -100,17 +100,14 @@ def make_shards(pir, lp, count, seed0):
     return shards, entries
 
 
-def batched(pir, entries_total, sizes, reps):
+def batched(pir, shards, entries, entries_total, sizes, reps):
     """Batched serving at `entries_total` entries: every shard distinct and
     resident; batches of B concurrent clients (own keys each) through
     dnspir_process_batch_device; unbatched = the same clients one query at a
     time through dnspir_process_queries_device."""
     import torch
     lp, ls = pir.get_database_shape(entries_total)
-    nsh = pir.num_databases(entries_total)
-    t0 = time.perf_counter()
-    shards, entries = make_shards(pir, lp, nsh, 5000)
-    log(f"[bench] batched: {nsh} shards of l={lp} preprocessed in {time.perf_counter() - t0:.1f} s")
+    nsh = len(shards)
     bmax = max(sizes)
     idx_rng = np.random.default_rng(42)
     rng = pir.Rng(bytes([42] + [0] * 31))
@@ -161,8 +158,7 @@ def batched(pir, entries_total, sizes, reps):
         ops = 2 * 4 * 4 * macs
         mfma = {"int8_ops": ops, "k1_ms": b64["k1_ms"], "achieved_tops": round(ops / (b64["k1_ms"] * 1e-3) / 1e12, 1),
                 "peak_tops": 5000.0, "frac": round(ops / (b64["k1_ms"] * 1e-3) / 5.0e15, 4)}
-    del shards, entries
-    torch.cuda.empty_cache()
+    del ctx
     return {
         "workload": (f"{entries_total} entries = {nsh} shards x {8 * lp} entries (l={lp}), all resident; "
                      f"secondary l={ls}; B concurrent clients with their own keys"),
@@ -172,6 +168,106 @@ def batched(pir, entries_total, sizes, reps):
         "detail": detail,
         "mfma_int8_b64": mfma,
         "decrypt_ok": ok,
+    }
+
+
+def serving(pir, dbs, rates, seconds, n_clients, args):
+    """Serving mix: open-loop Poisson arrivals (seeded) at each rate against
+    the resident databases `dbs` = [(name, shards, entries, resident, share)],
+    n_clients clients with their own keys; keyless queries after the first
+    contact, full queries again after a key-cache eviction."""
+    from concurrent.futures import ThreadPoolExecutor
+    srv = pir.Server(slots=args.serve_slots, max_batch=args.serve_max_batch,
+                     scratch_budget=args.serve_budget_gib << 30, key_cache_entries=args.serve_key_cache)
+    ids = [srv.add_database(d[1]) for d in dbs]
+    shapes = []
+    for name, shards, entries, resident, share in dbs:
+        lp = shards[0].group_order
+        shapes.append((lp, pir.choose_index_group_secondary(len(shards)), len(shards)))
+    t0 = time.perf_counter()
+
+    def make_client(c):
+        rng = pir.Rng(bytes([7, c & 255, c >> 8] + [0] * 29))
+        keys, sk = pir.client_keys(rng)
+        r = np.random.default_rng(10_000 + c)
+        qs = []
+        for lp, ls, nsh in shapes:
+            pi, si = int(r.integers(0, 8 * lp)), int(r.integers(0, nsh))
+            qs.append((pi, si, pir.client_query(rng, sk, pi, si, lp, ls)))
+        return keys, sk, qs
+
+    with ThreadPoolExecutor(16) as ex:
+        clients = list(ex.map(make_client, range(n_clients)))
+    log(f"[bench] serving: {n_clients} clients' keys and queries in {time.perf_counter() - t0:.1f} s; "
+        f"max batch per db {[srv.max_batch(i) for i in ids]}")
+    share = np.array([d[4] for d in dbs], float)
+    share /= share.sum()
+    rng = np.random.default_rng(2024)
+    # warm the key cache and the slots
+    warm = [(ids[k % len(ids)], c, 1000.0 * k, clients[c][2][k % len(ids)][2], clients[c][0]) for k, c in
+            enumerate(range(min(64, n_clients)))]
+    srv.replay(warm)
+    results, sample_ok, sample_n = {}, True, 0
+    for rate in rates:
+        n = max(100, int(rate * seconds))
+        t = np.cumsum(rng.exponential(1e6 / rate, n))
+        who = rng.integers(0, n_clients, n)
+        tgt = rng.choice(len(dbs), n, p=share)
+        arr = [(ids[int(tgt[k])], int(who[k]), float(t[k]), clients[who[k]][2][tgt[k]][2], clients[who[k]][0])
+               for k in range(n)]
+        st0 = srv.stats()
+        t_start = time.perf_counter()
+        replies, infos = srv.replay(arr)
+        wall = time.perf_counter() - t_start
+        st1 = srv.stats()
+        bad = sum(1 for i in infos if i.status != 0)
+        lat = np.array([(i.done_us - i.submit_us) / 1e3 for i in infos])
+        wait = np.array([(i.start_us - i.submit_us) / 1e3 for i in infos])
+        span = (max(i.done_us for i in infos) - min(i.submit_us for i in infos)) / 1e6
+        per_db = {}
+        for k, (name, *_rest) in enumerate(dbs):
+            m = tgt == k
+            if m.any():
+                per_db[name] = {"n": int(m.sum()), "p50_ms": round(float(np.percentile(lat[m], 50)), 2),
+                                "p99_ms": round(float(np.percentile(lat[m], 99)), 2)}
+        # decryption check of a seeded 5% sample
+        pick = np.random.default_rng(int(rate)).choice(n, max(1, n // 20), replace=False)
+        for k in pick:
+            d = int(tgt[k])
+            pi, si, _ = clients[who[k]][2][d]
+            _, _, entries, resident, _ = dbs[d]
+            want = entries[si % resident][pi].cpu().numpy().astype(np.uint16)
+            got = pir.process_reply(replies[k].tobytes(), clients[who[k]][1])
+            sample_ok &= bool(np.array_equal(got, want))
+            sample_n += 1
+        batches = st1["batches"] - st0["batches"]
+        results[str(rate)] = {
+            "requests": n, "failed": bad, "offered_qps": round(n / (t[-1] / 1e6), 1),
+            "achieved_qps": round(n / span, 1), "wall_s": round(wall, 2),
+            "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
+            "queue_wait_p50_ms": round(float(np.percentile(wait, 50)), 2),
+            "mean_batch": round(float(np.mean([i.batch_size for i in infos])), 1),
+            "batches": int(batches),
+            "keys_resent": int(sum(i.resent for i in infos)),
+            "key_cache_hits": int(st1["key_hits"] - st0["key_hits"]),
+            "per_db": per_db,
+        }
+        log(f"[bench] serving {rate}/s: {results[str(rate)]}")
+    st = srv.stats()
+    srv.close()
+    return {
+        "workload": ("Poisson arrivals (seeded) against " +
+                     ", ".join(f"{name} ({len(sh)} shards of l={sh[0].group_order}, {res} resident)"
+                               for name, sh, _, res, _ in dbs) +
+                     f"; targets split {'/'.join(str(int(d[4] * 100)) for d in dbs)}; {n_clients} clients "
+                     f"with their own keys, device key cache of {args.serve_key_cache} (LRU)"),
+        "server": {"slots": args.serve_slots, "max_batch": args.serve_max_batch,
+                   "scratch_budget_gib": args.serve_budget_gib, "key_cache_entries": args.serve_key_cache,
+                   "scratch_peak_gib": round(st["scratch_in_use_peak"] / 2**30, 2),
+                   "inflight_peak": st["inflight_peak"], "admission_waits": st["admission_waits"],
+                   "key_evictions": st["key_evictions"]},
+        "rates": results,
+        "decrypt_sample": {"n": sample_n, "ok": sample_ok},
     }
 
 
@@ -189,6 +285,15 @@ def main():
     ap.add_argument("--batch-sizes", default="1,8,16,32,64,128")
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-rates", default="50,200,500,1000")
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--serve-clients", type=int, default=1000)
+    ap.add_argument("--serve-small-entries", type=int, default=32768)
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-max-batch", type=int, default=64)
+    ap.add_argument("--serve-budget-gib", type=int, default=48)
+    ap.add_argument("--serve-key-cache", type=int, default=768)
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (PMC calibration)")
     args = ap.parse_args()
@@ -275,10 +380,31 @@ def main():
     traffic = read_traffic(tag)
 
     batch = None
+    bshards = bentries = None
+    if not (args.no_batched and args.no_serving):
+        blp = pir.get_database_shape(args.batch_entries)[0]
+        bn = pir.num_databases(args.batch_entries)
+        t0 = time.perf_counter()
+        bshards, bentries = make_shards(pir, blp, bn, 5000)
+        log(f"[bench] {args.batch_entries} entries: {bn} shards of l={blp} preprocessed in "
+            f"{time.perf_counter() - t0:.1f} s")
     if not args.no_batched:
-        del shards, fleet, entries
-        torch.cuda.empty_cache()
-        batch = batched(pir, args.batch_entries, [int(x) for x in args.batch_sizes.split(",")], args.batch_reps)
+        batch = batched(pir, bshards, bentries, args.batch_entries,
+                        [int(x) for x in args.batch_sizes.split(",")], args.batch_reps)
+
+    serve = None
+    if not args.no_serving:
+        slp = pir.get_database_shape(args.serve_small_entries)[0]
+        sn = pir.num_databases(args.serve_small_entries)
+        sshards, sentries = make_shards(pir, slp, sn, 9000)
+        dbs = [(f"{args.serve_small_entries} entries", sshards, sentries, sn, 0.2),
+               (f"{args.batch_entries} entries", bshards, bentries, len(bshards), 0.3),
+               (f"{args.entries} entries", fleet, entries, resident, 0.5)]
+        serve = serving(pir, dbs, [int(x) for x in args.serve_rates.split(",")], args.serve_seconds,
+                        args.serve_clients, args)
+        del sshards, sentries
+    del bshards, bentries, shards, fleet, entries
+    torch.cuda.empty_cache()
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -335,6 +461,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "batched": batch,
+        "serving": serve,
     }
     print(json.dumps(out), flush=True)
 

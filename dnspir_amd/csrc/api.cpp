@@ -825,7 +825,7 @@ int dne::batch_reserve(dnspir_ctx* c, int l_p, size_t len, int nq, size_t qlen) 
 // gets its own secondary database (3 lanes each) and all of those run
 // together. Query q's reply bytes equal its single-query reply.
 int dne::batch_impl(dnspir_ctx* c, const dnspir_shard* const* fleet, size_t len, const uint8_t* qdev, size_t qlen, int nq,
-                    uint8_t* rdev, dnspir_timing* tm, double t0, const BatchKeys* keys) {
+                    uint8_t* rdev, dnspir_timing* tm, double t0, const BatchKeys* keys, bool sync) {
     int rc;
     if (!fleet || len == 0 || !fleet[0] || nq <= 0 || !qdev || !rdev) return DNSPIR_E_ARG;
     const int l_p = fleet[0]->l;
@@ -890,6 +890,7 @@ int dne::batch_impl(dnspir_ctx* c, const dnspir_shard* const* fleet, size_t len,
     launch_encode_reply(c->a17o.as<uint32_t>(), c->b12o.as<uint32_t>(), 0, nq, rdev, s);
     HIP_OK(hipEventRecord(c->ev[2], s));
     HIP_OK(hipGetLastError());
+    if (!sync) return 0;
     HIP_OK(hipStreamSynchronize(s));
     fill_timing(c, tm, now_us() - t0);
     if (tm) tm->bytes_streamed = (uint64_t)((nq + 63) / 64) * len * slab_bytes(l_p);

@@ -148,9 +148,11 @@ int ctx_presize(dnspir_ctx* c, int max_nq);
 
 // nq queries (packed at stride qlen in device memory, full wire format)
 // against one fleet; replies packed REPLY_BYTES each at rdev. Caller holds
-// the context's lock.
+// the context's lock. sync = false: returns once everything is enqueued on
+// the context's stream; c->ev[1] then marks the end of the primary phase
+// (the last read of the shards) and tm is not filled.
 int batch_impl(dnspir_ctx* c, const dnspir_shard* const* fleet, size_t len, const uint8_t* qdev, size_t qlen, int nq,
-               uint8_t* rdev, dnspir_timing* tm, double t0, const BatchKeys* keys);
+               uint8_t* rdev, dnspir_timing* tm, double t0, const BatchKeys* keys, bool sync = true);
 
 // device scratch bytes batch_impl takes from an arena for nq queries
 // against a fleet of `len` shards of order l_p, plus the query and reply

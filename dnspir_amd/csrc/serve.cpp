@@ -63,6 +63,7 @@ struct Database {
     size_t qlen = 0;           // full wire length
     int max_batch = 0;         // largest batch whose scratch fits the budget
     std::vector<size_t> need;  // need[nb]: scratch bytes of a batch of nb
+    bool streaming = false;    // a batch is in its primary phase (reading the shards)
 };
 
 struct KeyEntry {
@@ -220,7 +221,7 @@ struct dnspir_server {
                 cv_done.notify_all();
                 continue;
             }
-            if (tried[r->db] || !dispatchable(r)) {
+            if (tried[r->db] || dbs[r->db].streaming || !dispatchable(r)) {
                 ++it;
                 continue;
             }
@@ -244,6 +245,7 @@ struct dnspir_server {
             }
             b.db = r->db;
             b.bytes = d.need[nb];
+            d.streaming = true;
             b.reqs.clear();
             const double t = now_us();
             for (int i = 0; i < nb; ++i) {
@@ -298,8 +300,16 @@ struct dnspir_server {
             k.src = src.data();
             k.dst = dst.data();
             rc = batch_impl(c, d.fleet.data(), d.fleet.size(), c->qbytes.as<uint8_t>(), d.qlen, nb, c->rbytes.as<uint8_t>(),
-                            nullptr, now_us(), any_key ? &k : nullptr);
+                            nullptr, now_us(), any_key ? &k : nullptr, false);
         }
+        // the shards are free for the next batch of this database once the
+        // primary phase has read them; the secondary phase overlaps it
+        if (!rc && hipEventSynchronize(c->ev[1]) != hipSuccess) rc = DNSPIR_E_HIP;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            d.streaming = false;
+        }
+        cv_work.notify_all();
         if (!rc && (hipMemcpyAsync(sl->hr, c->rbytes.p, (size_t)nb * REPLY_BYTES, hipMemcpyDeviceToHost, c->stream) !=
                         hipSuccess ||
                     hipStreamSynchronize(c->stream) != hipSuccess))

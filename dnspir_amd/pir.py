@@ -338,13 +338,18 @@ class Server:
         (replies as one uint8 array [n][reply_bytes], list of RequestInfo)."""
         n = len(arrivals)
         arr = (Arrival * n)()
-        keep = []
+        bufs = {}  # one C copy per distinct bytes object
+
+        def cbuf(b):
+            if b is None:
+                return None
+            k = id(b)
+            if k not in bufs:
+                bufs[k] = (b, ctypes.create_string_buffer(bytes(b), len(b)))
+            return ctypes.cast(bufs[k][1], ctypes.c_void_p)
+
         for i, (db, client, t_us, q, keys) in enumerate(arrivals):
-            qb = ctypes.create_string_buffer(bytes(q), len(q))
-            kb = ctypes.create_string_buffer(bytes(keys), len(keys)) if keys is not None else None
-            keep.append((qb, kb))
-            arr[i] = Arrival(db, client, t_us, ctypes.cast(qb, ctypes.c_void_p), len(q),
-                             ctypes.cast(kb, ctypes.c_void_p) if kb is not None else None)
+            arr[i] = Arrival(db, client, t_us, cbuf(q), len(q), cbuf(keys))
         replies = np.zeros((n, reply_bytes()), np.uint8)
         infos = (RequestInfo * n)()
         lib().dnspir_server_replay(self.h, arr, n, _ptr(replies, u8p), infos)

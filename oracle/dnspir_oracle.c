@@ -923,6 +923,9 @@ static double now_us(void) {
     return (double)ts.tv_sec * 1e6 + (double)ts.tv_nsec * 1e-3;
 }
 
+static int g_reuse_repeated = 0;
+void orc_set_reuse_repeated(int on) { g_reuse_repeated = on; }
+
 int orc_double_pir(orc_shard* const* fleet0, size_t len0, orc_shard* const* fleet1, size_t len1,
                    const uint64_t* q0, const uint64_t* q1, int l_s, uint64_t* out0, uint64_t* out1,
                    double* primary_us, double* secondary_us) {
@@ -948,6 +951,20 @@ int orc_double_pir(orc_shard* const* fleet0, size_t len0, orc_shard* const* flee
         for (size_t i = 0; i < lens[sl]; ++i) {
             const orc_shard* sh = fleets[sl][i];
             if (sh->conj != s0->conj) return -2; /* double_pir.rs:160 */
+            if (g_reuse_repeated) {
+                /* a shard that repeats in the fleet gives the same plaintexts for the same query */
+                size_t prev = i;
+                for (size_t k = 0; k < i; ++k)
+                    if (fleets[sl][k] == sh) {
+                        prev = k;
+                        break;
+                    }
+                if (prev != i) {
+                    memcpy(parts[sl] + (i * LANES * CTXT_PARTS) * (size_t)N, parts[sl] + (prev * LANES * CTXT_PARTS) * (size_t)N,
+                           sizeof(uint16_t) * LANES * CTXT_PARTS * (size_t)N);
+                    continue;
+                }
+            }
             orc_shard_get_a(sh, a);
             int rc = orc_shard_perform_pir(sh, q + 2 * N, sh->conj ? q + 3 * N : NULL, q, b);
             if (rc) return rc;

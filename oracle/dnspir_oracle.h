@@ -80,6 +80,9 @@ int orc_shard_requires_conj(const orc_shard*);
  * secondary, secondary_conj (conj entries ignored when not required).
  * fleets: shard pointer lists (shards may repeat); q[s] NULL = empty slot.
  * out[s]: [3][2][N] (a then b per part), written only for present slots. */
+/* test speed-up: a shard repeated in a fleet reuses the plaintexts computed
+ * for its first occurrence (identical by construction); off by default */
+void orc_set_reuse_repeated(int on);
 int orc_double_pir(orc_shard* const* fleet0, size_t len0, orc_shard* const* fleet1, size_t len1,
                    const uint64_t* q0, const uint64_t* q1, int secondary_group_order,
                    uint64_t* out0, uint64_t* out1, double* primary_us, double* secondary_us);

@@ -81,6 +81,7 @@ def lib():
             "orc_prepare_query": (ctypes.c_int, [vp, sz, sz, ctypes.c_int, ctypes.c_int, u8p, ctypes.POINTER(sz), u8p]),
             "orc_process_reply": (ctypes.c_int, [u8p, sz, u8p, u16p]),
             "orc_set_threads": (None, [ctypes.c_int]),
+            "orc_set_reuse_repeated": (None, [ctypes.c_int]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -96,6 +97,12 @@ def ptr(a, t):
 
 def set_threads(n):
     lib().orc_set_threads(int(n))
+
+
+def set_reuse_repeated(on):
+    """Repeated shards of a fleet reuse their first occurrence's primary
+    result (a test speed-up; identical output)."""
+    lib().orc_set_reuse_repeated(1 if on else 0)
 
 
 def chacha_block(inp, rounds):

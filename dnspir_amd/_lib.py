@@ -62,6 +62,11 @@ class ServerStats(ctypes.Structure):
                 ("scratch_in_use_peak", ctypes.c_size_t), ("inflight_peak", ctypes.c_int)]
 
 
+class TuneEntry(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 48), ("column_tiles", ctypes.c_int), ("ms", ctypes.c_double),
+                ("exact", ctypes.c_int), ("chosen", ctypes.c_int)]
+
+
 class Arrival(ctypes.Structure):
     _fields_ = [("db", ctypes.c_int), ("client", ctypes.c_uint64), ("arrival_us", ctypes.c_double),
                 ("query", ctypes.c_void_p), ("qlen", ctypes.c_size_t), ("keys", ctypes.c_void_p)]
@@ -91,6 +96,7 @@ SIGNATURES = {
     "dnspir_ctx_create": (ci, [vp, ctypes.POINTER(vp)]),
     "dnspir_ctx_destroy": (ci, [vp]),
     "dnspir_ctx_stream": (vp, [vp]),
+    "dnspir_ctx_set_graphs": (ci, [vp, ci]),
     "dnspir_double_pir": (ci, [vp, vp, sz, vp, sz, u64p, u64p, u64p, u64p, ctypes.POINTER(Timing)]),
     "dnspir_process_queries": (ci, [vp, vp, sz, vp, sz, u8p, sz, u8p, sz, u8p, u8p, ctypes.POINTER(Timing)]),
     "dnspir_process_queries_device": (ci, [vp, vp, sz, vp, sz, vp, sz, vp, sz, vp, vp, ctypes.POINTER(Timing)]),
@@ -108,6 +114,7 @@ SIGNATURES = {
     "dnspir_client_keys": (ci, [vp, u8p, u8p]),
     "dnspir_client_query": (ci, [vp, u8p, sz, sz, ci, ci, u8p, ctypes.POINTER(sz)]),
     "dnspir_key_bytes": (sz, []),
+    "dnspir_autotune_batch": (ci, [vp, vp, sz, ci, ctypes.POINTER(TuneEntry), ci, ip]),
     "dnspir_batch_scratch_bytes": (sz, [ci, sz, ci]),
     "dnspir_server_create": (ci, [ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]),
     "dnspir_server_add_database": (ci, [vp, vp, sz, ip]),

@@ -322,6 +322,7 @@ int run_double(dnspir_ctx* c, const dnspir_shard* const* fleets[2], const size_t
     launch_qprep(c->qcoef.as<uint64_t>(), c->qs.as<uint32_t>(), c->qp.as<uint32_t>(), 2, g.tables, s);
     uint64_t streamed = 0;
     c->k1_launches = 0;
+    c->k1_timed = true;
     for (int sl = 0; sl < 2; ++sl) {
         if (!present[sl]) continue;
         const int len = (int)lens[sl];
@@ -354,9 +355,10 @@ int run_double(dnspir_ctx* c, const dnspir_shard* const* fleets[2], const size_t
 void fill_timing(dnspir_ctx* c, dnspir_timing* tm, double wall_us) {
     if (!tm) return;
     float a = 0, b = 0, k = 0;
-    (void)hipEventElapsedTime(&a, c->ev[0], c->ev[1]);
-    (void)hipEventElapsedTime(&b, c->ev[1], c->ev[2]);
-    (void)hipEventElapsedTime(&k, c->ev[3], c->ev[4]);
+    bool err = hipEventElapsedTime(&a, c->ev[0], c->ev[1]) != hipSuccess;
+    err |= hipEventElapsedTime(&b, c->ev[1], c->ev[2]) != hipSuccess;
+    if (c->k1_timed) err |= hipEventElapsedTime(&k, c->ev[3], c->ev[4]) != hipSuccess;
+    if (err) (void)hipGetLastError();  // a timing gap must not fail the next call
     tm->primary_us = a * 1000.0;
     tm->secondary_us = b * 1000.0;
     tm->total_us = wall_us;
@@ -609,6 +611,13 @@ int dnspir_ctx_destroy(dnspir_ctx* c) {
 
 void* dnspir_ctx_stream(dnspir_ctx* c) { return c ? (void*)c->stream : nullptr; }
 
+int dnspir_ctx_set_graphs(dnspir_ctx* c, int on) {
+    if (!c) return DNSPIR_E_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->graphs = on != 0;
+    return c->graphs_broken ? 1 : 0;
+}
+
 int dnspir_choose_index_group_secondary(size_t num_shards) {
     int log2l = 0;
     while (((size_t)1 << log2l) < num_shards) ++log2l;
@@ -825,7 +834,7 @@ int dne::batch_reserve(dnspir_ctx* c, int l_p, size_t len, int nq, size_t qlen) 
 // gets its own secondary database (3 lanes each) and all of those run
 // together. Query q's reply bytes equal its single-query reply.
 int dne::batch_impl(dnspir_ctx* c, const dnspir_shard* const* fleet, size_t len, const uint8_t* qdev, size_t qlen, int nq,
-                    uint8_t* rdev, dnspir_timing* tm, double t0, const BatchKeys* keys, bool sync) {
+                    uint8_t* rdev, dnspir_timing* tm, double t0, const BatchKeys* keys, bool sync, bool graph) {
     int rc;
     if (!fleet || len == 0 || !fleet[0] || nq <= 0 || !qdev || !rdev) return DNSPIR_E_ARG;
     const int l_p = fleet[0]->l;
@@ -850,10 +859,8 @@ int dne::batch_impl(dnspir_ctx* c, const dnspir_shard* const* fleet, size_t len,
     const long long* d_off;
     const int *d_ncols, *d_slot;
     if ((rc = upload_lanes(c, loff, ncols, slot, &d_off, &d_ncols, &d_slot))) return rc;
-    launch_decode_query(qdev, qlen, nq, offs, bits, c->qcoef.as<uint64_t>(), s);
-    HIP_OK(hipEventRecord(c->ev[0], s));
-    launch_qprep(c->qcoef.as<uint64_t>(), c->qs.as<uint32_t>(), c->qp.as<uint32_t>(), nq, g.tables, s);
-    if (keys && (keys->src || keys->dst)) {
+    const bool use_keys = keys && (keys->src || keys->dst);
+    if (use_keys) {
         // cached keys replace the decoded ones; fresh ones are stored
         if ((rc = c->keyp.ensure((size_t)2 * nq * sizeof(void*)))) return rc;
         c->h_keyp.assign((size_t)2 * nq, nullptr);
@@ -862,33 +869,86 @@ int dne::batch_impl(dnspir_ctx* c, const dnspir_shard* const* fleet, size_t len,
             if (keys->dst) c->h_keyp[nq + q] = keys->dst[q];
         }
         HIP_OK(hipMemcpyAsync(c->keyp.p, c->h_keyp.data(), (size_t)2 * nq * sizeof(void*), hipMemcpyHostToDevice, s));
-        launch_key_cache(c->keyp.as<const uint32_t*>(), c->keyp.as<uint32_t*>() + nq, nq, c->qs.as<uint32_t>(),
-                         c->qp.as<uint32_t>(), s);
     }
-    c->k1_launches = 0;
-    for (int q0 = 0; q0 < nq; q0 += 64) {
-        const int nqc = std::min(64, nq - q0);
-        bool ob = false;
-        if ((rc = run_primary(c, slabs, (int)len, l_p, c->qp.as<uint32_t>() + (size_t)q0 * 4 * 2 * N, nqc, &ob,
-                              q0 == 0 ? c->ev[3] : nullptr)))
-            return rc;
-        ++c->k1_launches;
-        if (q0 == 0) HIP_OK(hipEventRecord(c->ev[4], s));
-        EpiLaunch e;
-        e.outb = c->outb.as<uint32_t>();
-        e.nshards = (int)len;
-        e.nq = nqc;
-        e.a17s = a17s;
-        e.se = c->se.as<uint16_t>() + (size_t)q0 * l_s * 3 * N;
-        e.se_qstride = (size_t)l_s * 3 * N;
-        e.lpr = 3;
-        e.lane0_mult = 0;
-        launch_epi(e, g.tables, s);
+    // the device work of the batch, as two phases of kernels (timing events
+    // only when launched eagerly)
+    auto enqueue_primary = [&](bool events) -> int {
+        launch_decode_query(qdev, qlen, nq, offs, bits, c->qcoef.as<uint64_t>(), s);
+        launch_qprep(c->qcoef.as<uint64_t>(), c->qs.as<uint32_t>(), c->qp.as<uint32_t>(), nq, g.tables, s);
+        if (use_keys)
+            launch_key_cache(c->keyp.as<const uint32_t*>(), c->keyp.as<uint32_t*>() + nq, nq, c->qs.as<uint32_t>(),
+                             c->qp.as<uint32_t>(), s);
+        int r2;
+        for (int q0 = 0; q0 < nq; q0 += 64) {
+            const int nqc = std::min(64, nq - q0);
+            bool ob = false;
+            if ((r2 = run_primary(c, slabs, (int)len, l_p, c->qp.as<uint32_t>() + (size_t)q0 * 4 * 2 * N, nqc, &ob,
+                                  events && q0 == 0 ? c->ev[3] : nullptr)))
+                return r2;
+            if (events && q0 == 0) HIP_OK(hipEventRecord(c->ev[4], s));
+            EpiLaunch e;
+            e.outb = c->outb.as<uint32_t>();
+            e.nshards = (int)len;
+            e.nq = nqc;
+            e.a17s = a17s;
+            e.se = c->se.as<uint16_t>() + (size_t)q0 * l_s * 3 * N;
+            e.se_qstride = (size_t)l_s * 3 * N;
+            e.lpr = 3;
+            e.lane0_mult = 0;
+            launch_epi(e, g.tables, s);
+        }
+        return 0;
+    };
+    auto enqueue_secondary = [&]() -> int {
+        int r2;
+        if ((r2 = run_secondary(c, c->se.as<uint16_t>(), 3LL * N, d_off, d_ncols, d_slot, 3 * nq, l_s))) return r2;
+        launch_encode_reply(c->a17o.as<uint32_t>(), c->b12o.as<uint32_t>(), 0, nq, rdev, s);
+        return 0;
+    };
+    c->k1_launches = (nq + 63) / 64;
+    bool graphed = false;
+    if (graph && c->graphs && !c->graphs_broken) {
+        // everything the captured launches bake in
+        std::vector<uintptr_t> key = {(uintptr_t)qdev, qlen, (uintptr_t)nq, (uintptr_t)rdev, len, (uintptr_t)l_p,
+                                      (uintptr_t)use_keys, (uintptr_t)c->keyp.p, (uintptr_t)c->ptrs.p, (uintptr_t)c->idx.p,
+                                      (uintptr_t)k1b_get_choice(1, nstreams(l_p)), (uintptr_t)k1b_get_choice(2, nstreams(l_p))};
+        for (auto* b : c->scratch_bufs) key.push_back((uintptr_t)b->p);
+        if (!c->graph_exec[0] || key != c->graph_key) {
+            c->drop_graphs();
+            bool ok = true;
+            for (int ph = 0; ph < 2 && ok; ++ph) {
+                hipGraph_t gr = nullptr;
+                ok = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess;
+                const int r2 = ok ? (ph == 0 ? enqueue_primary(false) : enqueue_secondary()) : 0;
+                ok = (hipStreamEndCapture(s, &gr) == hipSuccess) && ok && r2 == 0 && gr;
+                if (ok) ok = hipGraphInstantiate(&c->graph_exec[ph], gr, nullptr, nullptr, 0) == hipSuccess;
+                if (gr) (void)hipGraphDestroy(gr);
+            }
+            if (ok) {
+                c->graph_key = key;
+            } else {
+                (void)hipGetLastError();
+                c->drop_graphs();
+                c->graphs_broken = true;  // capture is not available here: launch eagerly from now on
+            }
+        }
+        if (c->graph_exec[0] && c->graph_exec[1]) {
+            HIP_OK(hipEventRecord(c->ev[0], s));
+            HIP_OK(hipGraphLaunch(c->graph_exec[0], s));
+            HIP_OK(hipEventRecord(c->ev[1], s));
+            HIP_OK(hipGraphLaunch(c->graph_exec[1], s));
+            HIP_OK(hipEventRecord(c->ev[2], s));
+            graphed = true;
+        }
     }
-    HIP_OK(hipEventRecord(c->ev[1], s));
-    if ((rc = run_secondary(c, c->se.as<uint16_t>(), 3LL * N, d_off, d_ncols, d_slot, 3 * nq, l_s))) return rc;
-    launch_encode_reply(c->a17o.as<uint32_t>(), c->b12o.as<uint32_t>(), 0, nq, rdev, s);
-    HIP_OK(hipEventRecord(c->ev[2], s));
+    if (!graphed) {
+        HIP_OK(hipEventRecord(c->ev[0], s));
+        if ((rc = enqueue_primary(true))) return rc;
+        HIP_OK(hipEventRecord(c->ev[1], s));
+        if ((rc = enqueue_secondary())) return rc;
+        HIP_OK(hipEventRecord(c->ev[2], s));
+    }
+    c->k1_timed = !graphed;
     HIP_OK(hipGetLastError());
     if (!sync) return 0;
     HIP_OK(hipStreamSynchronize(s));
@@ -905,7 +965,7 @@ int dnspir_process_batch_device(dnspir_ctx* c, const dnspir_shard* const* fleet,
     if (rc) return rc;
     if (!c) c = default_ctx();
     std::lock_guard<std::mutex> lk(c->mu);
-    return batch_impl(c, fleet, len, queries_dev, qlen, nq, replies_dev, tm, now_us(), nullptr);
+    return batch_impl(c, fleet, len, queries_dev, qlen, nq, replies_dev, tm, now_us(), nullptr, true, true);
 }
 
 int dnspir_process_batch(dnspir_ctx* c, const dnspir_shard* const* fleet, size_t len, const uint8_t* queries,
@@ -919,11 +979,82 @@ int dnspir_process_batch(dnspir_ctx* c, const dnspir_shard* const* fleet, size_t
     if ((rc = c->qbytes.ensure((size_t)nq * qlen + 16))) return rc;
     if ((rc = c->rbytes.ensure((size_t)nq * REPLY_BYTES))) return rc;
     HIP_OK(hipMemcpyAsync(c->qbytes.p, queries, (size_t)nq * qlen, hipMemcpyHostToDevice, c->stream));
-    rc = batch_impl(c, fleet, len, c->qbytes.as<uint8_t>(), qlen, nq, c->rbytes.as<uint8_t>(), nullptr, t0, nullptr);
+    rc = batch_impl(c, fleet, len, c->qbytes.as<uint8_t>(), qlen, nq, c->rbytes.as<uint8_t>(), nullptr, t0, nullptr, true, true);
     if (rc) return rc;
     HIP_OK(hipMemcpy(replies, c->rbytes.p, (size_t)nq * REPLY_BYTES, hipMemcpyDeviceToHost));
     fill_timing(c, tm, now_us() - t0);
     if (tm) tm->bytes_streamed = (uint64_t)((nq + 63) / 64) * len * slab_bytes(fleet[0]->l);
+    return 0;
+}
+
+// ---- autotuner of the batched inner product --------------------------------------
+int dnspir_autotune_batch(dnspir_ctx* c, const dnspir_shard* const* fleet, size_t len, int nq, dnspir_tune_entry* entries,
+                          int max_entries, int* n_entries) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (!c) c = default_ctx();
+    if (!fleet || len == 0 || !fleet[0] || nq < 3 || nq > 64) return DNSPIR_E_ARG;
+    const int l_p = fleet[0]->l;
+    if (loop_len(l_p) != N / 2) return DNSPIR_E_SHAPE;
+    if ((rc = check_fleet(fleet, len, l_p))) return rc;
+    std::lock_guard<std::mutex> lk(c->mu);
+    const hipStream_t s = c->stream;
+    const int nst = nstreams(l_p), ct = nq > 32 ? 2 : 1;
+    const size_t qlb = qlimb_bytes(nq, nst), ob = (size_t)nq * len * 2 * N * LANES * 4;
+    if ((rc = c->ql.ensure(qlb)) || (rc = c->outb.ensure(ob)) || (rc = c->acc.ensure(ob + 256))) return rc;
+    const uint32_t* const* slabs;
+    const uint32_t* const* a17s;
+    if ((rc = upload_fleet(c, 0, fleet, len, &slabs, &a17s))) return rc;
+    // random int8 digits are valid operands (|digit| <= 128 bounds the sums)
+    launch_fill_hash(c->ql.as<uint32_t>(), qlb / 4, 12345u, s);
+    uint32_t* ref = c->acc.as<uint32_t>();
+    unsigned* flag = reinterpret_cast<unsigned*>(c->acc.as<char>() + ob);
+    hipEvent_t e0 = c->ev[3], e1 = c->ev[4];
+    int n = 0, first = -1, best = -1;
+    double best_ms = 1e30;
+    for (int v = 0; v < k1b_variant_count(); ++v) {
+        if (k1b_variant_ct(v) != ct) continue;
+        uint32_t* out = first < 0 ? ref : c->outb.as<uint32_t>();
+        launch_k1b_variant(v, slabs, (int)len, c->ql.as<uint32_t>(), out, nst, nq, s);  // warm-up
+        HIP_OK(hipEventRecord(e0, s));
+        const int reps = 3;
+        for (int k = 0; k < reps; ++k) launch_k1b_variant(v, slabs, (int)len, c->ql.as<uint32_t>(), out, nst, nq, s);
+        HIP_OK(hipEventRecord(e1, s));
+        unsigned h_flag = 0;
+        if (first >= 0) {
+            launch_fill_u32(flag, 1, 0, s);
+            launch_diff(out, ref, ob / 4, flag, s);
+            HIP_OK(hipMemcpyAsync(&h_flag, flag, 4, hipMemcpyDeviceToHost, s));
+        }
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipStreamSynchronize(s));
+        float ms = 0;
+        HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+        const double per = ms / reps;
+        const bool exact = h_flag == 0;
+        if (first < 0) first = v;
+        if (exact && per < best_ms) {
+            best_ms = per;
+            best = v;
+        }
+        if (entries && n < max_entries) {
+            dnspir_tune_entry& t = entries[n];
+            memset(&t, 0, sizeof(t));
+            snprintf(t.name, sizeof(t.name), "%s", k1b_variant_name(v));
+            t.column_tiles = ct;
+            t.ms = per;
+            t.exact = exact ? 1 : 0;
+        }
+        ++n;
+    }
+    if (best < 0) return DNSPIR_E_HIP;
+    k1b_set_choice(ct, nst, best);
+    if (entries) {
+        int k = 0;
+        for (int v = 0; v < k1b_variant_count() && k < max_entries; ++v)
+            if (k1b_variant_ct(v) == ct) entries[k++].chosen = v == best ? 1 : 0;
+    }
+    if (n_entries) *n_entries = std::min(n, max_entries);
     return 0;
 }
 

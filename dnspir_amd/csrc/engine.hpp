@@ -125,6 +125,19 @@ struct dnspir_ctx {
     std::vector<uint64_t> stage;
     std::vector<long long> h_idx;  // host staging of the lane lists (outlives the async copy)
     std::vector<const void*> h_keyp;
+    bool k1_timed = true;  // ev[3]/ev[4] bracket the first inner-product launch
+    // batched pipeline captured as two HIP graphs (primary, secondary phase),
+    // replayed while its key matches
+    bool graphs = true;
+    bool graphs_broken = false;  // capture failed once: stay eager
+    hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+    std::vector<uintptr_t> graph_key;
+    void drop_graphs() {
+        for (auto& g : graph_exec)
+            if (g) (void)hipGraphExecDestroy(g), g = nullptr;
+        graph_key.clear();
+    }
+    ~dnspir_ctx() { drop_graphs(); }
 
     dne::DevBuf* scratch_bufs[17] = {&se, &M, &te, &aout, &cout, &qcoef, &qs, &qp, &acc,
                                      &outb, &ql, &acoef, &bcoef, &a17o, &b12o, &qbytes, &rbytes};
@@ -150,9 +163,10 @@ int ctx_presize(dnspir_ctx* c, int max_nq);
 // against one fleet; replies packed REPLY_BYTES each at rdev. Caller holds
 // the context's lock. sync = false: returns once everything is enqueued on
 // the context's stream; c->ev[1] then marks the end of the primary phase
-// (the last read of the shards) and tm is not filled.
+// (the last read of the shards) and tm is not filled. graph = true (and the
+// context's graphs on): the device work runs as a captured HIP graph.
 int batch_impl(dnspir_ctx* c, const dnspir_shard* const* fleet, size_t len, const uint8_t* qdev, size_t qlen, int nq,
-               uint8_t* rdev, dnspir_timing* tm, double t0, const BatchKeys* keys, bool sync = true);
+               uint8_t* rdev, dnspir_timing* tm, double t0, const BatchKeys* keys, bool sync = true, bool graph = false);
 
 // device scratch bytes batch_impl takes from an arena for nq queries
 // against a fleet of `len` shards of order l_p, plus the query and reply

@@ -9,6 +9,8 @@
 //                streaming inner product reads it at the output position
 //                (the Galois scatter of base_pir.rs:1085-1247 made static)
 //   lane-major temporaries [stream][lane][row][...][r][slot] during encoding
+#include <atomic>
+
 #include "dn_common.hpp"
 #include "kernels.hpp"
 
@@ -728,6 +730,234 @@ __global__ __launch_bounds__(256) void k1b_kernel(const uint32_t* const* __restr
     }
 }
 
+// The shard operand through a global-address-space pointer: the loads are
+// global (not flat) loads, so waiting for them never waits for LDS traffic.
+typedef __attribute__((address_space(1))) const v4i gv4i;
+
+// k1b with the query tiles loaded PD steps ahead into a register ring, like
+// the shard tiles, and written to a 2-stage LDS buffer one step before use:
+// waiting for a query tile never waits for a younger shard prefetch (vector
+// memory counters retire in order). Loads are unconditional (clamped index).
+template <int CT, int PD>
+__global__ __launch_bounds__(256) void k1b_r(const uint32_t* const* __restrict__ slabs, const v4i* __restrict__ ql,
+                                             uint32_t* __restrict__ outb, int nst, int nq, size_t q_stride) {
+    __shared__ v4i Bt[2][CT][4][64];
+    const int S = 3 * nst;
+    const int half = blockIdx.y;
+    const int r = blockIdx.z & 1, sh = blockIdx.z >> 1;
+    const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
+    const int g = blockIdx.x * 4 + wave;
+    gv4i* A = (gv4i*)(slabs[sh]) + (((size_t)(r * 2 + half) * (HALFN / 4) + g) * S) * 32 * 256 + l;
+    gv4i* Q = (gv4i*)ql + ((size_t)(r * 2 + half) * S) * 32 * CT * 256 + tid;
+    const int nk = S * 32;
+    v16i acc[7][CT];
+#pragma unroll
+    for (int w = 0; w < 7; ++w)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+            for (int e = 0; e < 16; ++e) acc[w][ct][e] = 0;
+    v4i abuf[PD][4];
+    v4i bq[PD][CT];
+    // step 0's query tile straight to LDS; steps 1..PD in the register ring
+#pragma unroll
+    for (int i = 0; i < CT; ++i) (&Bt[0][0][0][0])[tid + 256 * i] = Q[256 * i];
+#pragma unroll
+    for (int d = 0; d < PD; ++d) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) abuf[d][v] = __builtin_nontemporal_load(A + (size_t)d * 256 + v * 64);
+        const int kb = min(d + 1, nk - 1);
+#pragma unroll
+        for (int i = 0; i < CT; ++i) bq[d][i] = Q[(size_t)kb * CT * 256 + 256 * i];
+    }
+    for (int kk0 = 0; kk0 < nk; kk0 += PD) {
+#pragma unroll
+        for (int d = 0; d < PD; ++d) {
+            const int kk = kk0 + d;
+            __syncthreads();  // tile kk visible; tile kk-1's buffer free
+            // tile kk+1 (loaded PD steps ago) -> LDS
+#pragma unroll
+            for (int i = 0; i < CT; ++i) (&Bt[(kk + 1) & 1][0][0][0])[tid + 256 * i] = bq[d][i];
+            v4i a[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) a[v] = abuf[d][v];
+            v4i b[CT][4];
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) b[ct][v] = Bt[kk & 1][ct][v][l];
+            const int ka = min(kk + PD, nk - 1), kb = min(kk + PD + 1, nk - 1);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) abuf[d][v] = __builtin_nontemporal_load(A + (size_t)ka * 256 + v * 64);
+#pragma unroll
+            for (int i = 0; i < CT; ++i) bq[d][i] = Q[(size_t)kb * CT * 256 + 256 * i];
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v)
+                        acc[u + v][ct] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[u], b[ct][v], acc[u + v][ct], 0, 0, 0);
+        }
+    }
+    const uint32_t p = mod_p(r);
+    const int K = mod_k(r);
+    uint32_t cw[7];
+    {
+        uint64_t x = 1;
+#pragma unroll
+        for (int w = 0; w < 7; ++w) {
+            cw[w] = (uint32_t)(x % p);
+            x = (x % p) * 256;
+        }
+    }
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct) {
+        const int q = ct * 32 + (l & 31);
+        if (q >= nq) continue;
+        uint32_t* o = outb + (size_t)q * q_stride + ((size_t)sh * 2 + r) * N * LANES;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            int64_t t = 0;
+#pragma unroll
+            for (int w = 0; w < 7; ++w) t += (int64_t)acc[w][ct][e] * cw[w];
+            const int row = 8 * (e >> 2) + 4 * (l >> 5) + (e & 3);
+            const int a = g * 4 + (row >> 3), lane8 = row & 7;
+            o[(half * HALFN + a) * LANES + lane8] = from_i64(t, p, K);
+        }
+    }
+}
+
+
+// the same with U steps unrolled per loop iteration (U a multiple of PD)
+template <int CT, int PD, int U>
+__global__ __launch_bounds__(256) void k1b_u(const uint32_t* const* __restrict__ slabs, const v4i* __restrict__ ql,
+                                             uint32_t* __restrict__ outb, int nst, int nq, size_t q_stride) {
+    __shared__ v4i Bt[2][CT][4][64];
+    const int S = 3 * nst;
+    const int half = blockIdx.y;
+    const int r = blockIdx.z & 1, sh = blockIdx.z >> 1;
+    const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
+    const int g = blockIdx.x * 4 + wave;
+    gv4i* A = (gv4i*)(slabs[sh]) + (((size_t)(r * 2 + half) * (HALFN / 4) + g) * S) * 32 * 256 + l;
+    gv4i* Q = (gv4i*)ql + ((size_t)(r * 2 + half) * S) * 32 * CT * 256 + tid;
+    const int nk = S * 32;
+    v16i acc[7][CT];
+#pragma unroll
+    for (int w = 0; w < 7; ++w)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+            for (int e = 0; e < 16; ++e) acc[w][ct][e] = 0;
+    v4i abuf[PD][4];
+    v4i bq[PD][CT];
+    // step 0's query tile straight to LDS; steps 1..PD in the register ring
+#pragma unroll
+    for (int i = 0; i < CT; ++i) (&Bt[0][0][0][0])[tid + 256 * i] = Q[256 * i];
+#pragma unroll
+    for (int d = 0; d < PD; ++d) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) abuf[d][v] = __builtin_nontemporal_load(A + (size_t)d * 256 + v * 64);
+        const int kb = min(d + 1, nk - 1);
+#pragma unroll
+        for (int i = 0; i < CT; ++i) bq[d][i] = Q[(size_t)kb * CT * 256 + 256 * i];
+    }
+    static_assert(U % PD == 0, "PD divides U");
+    for (int kk0 = 0; kk0 < nk; kk0 += U) {
+#pragma unroll
+        for (int dd = 0; dd < U; ++dd) {
+            const int d = dd % PD;
+            const int kk = kk0 + dd;
+            __syncthreads();  // tile kk visible; tile kk-1's buffer free
+            // tile kk+1 (loaded PD steps ago) -> LDS
+#pragma unroll
+            for (int i = 0; i < CT; ++i) (&Bt[(kk + 1) & 1][0][0][0])[tid + 256 * i] = bq[d][i];
+            v4i a[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) a[v] = abuf[d][v];
+            v4i b[CT][4];
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) b[ct][v] = Bt[kk & 1][ct][v][l];
+            const int ka = min(kk + PD, nk - 1), kb = min(kk + PD + 1, nk - 1);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) abuf[d][v] = __builtin_nontemporal_load(A + (size_t)ka * 256 + v * 64);
+#pragma unroll
+            for (int i = 0; i < CT; ++i) bq[d][i] = Q[(size_t)kb * CT * 256 + 256 * i];
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v)
+                        acc[u + v][ct] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[u], b[ct][v], acc[u + v][ct], 0, 0, 0);
+        }
+    }
+    const uint32_t p = mod_p(r);
+    const int K = mod_k(r);
+    uint32_t cw[7];
+    {
+        uint64_t x = 1;
+#pragma unroll
+        for (int w = 0; w < 7; ++w) {
+            cw[w] = (uint32_t)(x % p);
+            x = (x % p) * 256;
+        }
+    }
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct) {
+        const int q = ct * 32 + (l & 31);
+        if (q >= nq) continue;
+        uint32_t* o = outb + (size_t)q * q_stride + ((size_t)sh * 2 + r) * N * LANES;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            int64_t t = 0;
+#pragma unroll
+            for (int w = 0; w < 7; ++w) t += (int64_t)acc[w][ct][e] * cw[w];
+            const int row = 8 * (e >> 2) + 4 * (l >> 5) + (e & 3);
+            const int a = g * 4 + (row >> 3), lane8 = row & 7;
+            o[(half * HALFN + a) * LANES + lane8] = from_i64(t, p, K);
+        }
+    }
+}
+
+
+// Candidates for the batched inner product, per column-tile count. All give
+// identical results (exact integer arithmetic); the autotuner times them.
+typedef void (*K1bFn)(const uint32_t* const*, const v4i*, uint32_t*, int, int, size_t);
+struct K1bVariant {
+    const char* name;
+    int ct;
+    K1bFn fn;
+};
+static const K1bVariant k_k1b_variants[] = {
+    {"lds ring, prefetch 4", 1, k1b_kernel<1, 4>}, {"register ring, prefetch 2", 1, k1b_r<1, 2>},
+    {"register ring, prefetch 3", 1, k1b_r<1, 3>}, {"register ring, prefetch 4", 1, k1b_r<1, 4>},
+    {"register ring, prefetch 4, unroll 32", 1, k1b_u<1, 4, 32>},
+    {"lds ring, prefetch 4", 2, k1b_kernel<2, 4>}, {"lds ring, prefetch 8", 2, k1b_kernel<2, 8>},
+    {"register ring, prefetch 2", 2, k1b_r<2, 2>}, {"register ring, prefetch 3", 2, k1b_r<2, 3>},
+    {"register ring, prefetch 4", 2, k1b_r<2, 4>}, {"register ring, prefetch 4, unroll 8", 2, k1b_u<2, 4, 8>},
+    {"register ring, prefetch 4, unroll 32", 2, k1b_u<2, 4, 32>},
+};
+constexpr int K1B_NVAR = sizeof(k_k1b_variants) / sizeof(k_k1b_variants[0]);
+// chosen variant per (column tiles, streams); -1 = default (register ring, prefetch 4)
+static std::atomic<int> g_k1b_choice[3][3] = {{{-1}, {-1}, {-1}}, {{-1}, {-1}, {-1}}, {{-1}, {-1}, {-1}}};
+
+int k1b_variant_count() { return K1B_NVAR; }
+const char* k1b_variant_name(int i) { return i >= 0 && i < K1B_NVAR ? k_k1b_variants[i].name : ""; }
+int k1b_variant_ct(int i) { return i >= 0 && i < K1B_NVAR ? k_k1b_variants[i].ct : 0; }
+void k1b_set_choice(int ct, int nst, int i) {
+    if (ct >= 1 && ct <= 2 && nst >= 1 && nst <= 2) g_k1b_choice[ct][nst] = i;
+}
+int k1b_get_choice(int ct, int nst) {
+    int i = (ct >= 1 && ct <= 2 && nst >= 1 && nst <= 2) ? g_k1b_choice[ct][nst].load() : -1;
+    if (i >= 0 && i < K1B_NVAR && k_k1b_variants[i].ct == ct) return i;
+    for (int k = 0; k < K1B_NVAR; ++k)
+        if (k_k1b_variants[k].ct == ct && k_k1b_variants[k].fn == (ct == 1 ? (K1bFn)k1b_r<1, 4> : (K1bFn)k1b_r<2, 4>)) return k;
+    return 0;
+}
+
 // sum of the chains' a (and c) over the streams -> inverse NTT -> CRT ->
 // coefficients mod q and their 17-bit (a) / 12-bit (b) mod switch.
 // One block per lane index. a_out/c_out [st][li][r][slot].
@@ -1104,12 +1334,40 @@ void launch_k1b(const uint32_t* const* slabs, int nshards, const uint32_t* qp, i
         hipLaunchKernelGGL((k1d_kernel<1, 2>), dim3(HALFN / 16, 2, nshards * 2), dim3(256), 0, s, slabs, ql, outb, nst, qstride);
     else if (nq == 2)
         hipLaunchKernelGGL((k1d_kernel<2, 2>), dim3(HALFN / 16, 2, nshards * 2), dim3(256), 0, s, slabs, ql, outb, nst, qstride);
-    else if (CT == 1)
-        hipLaunchKernelGGL((k1b_kernel<1, 4>), dim3(HALFN / 16, 2, nshards * 2), dim3(256), 0, s, slabs, (const v4i*)ql, outb,
-                           nst, nq, qstride);
     else
-        hipLaunchKernelGGL((k1b_kernel<2, 4>), dim3(HALFN / 16, 2, nshards * 2), dim3(256), 0, s, slabs, (const v4i*)ql, outb,
-                           nst, nq, qstride);
+        launch_k1b_variant(k1b_get_choice(CT, nst), slabs, nshards, ql, outb, nst, nq, s);
+}
+
+void launch_k1b_variant(int i, const uint32_t* const* slabs, int nshards, const uint32_t* ql, uint32_t* outb, int nst, int nq,
+                        hipStream_t s) {
+    const size_t qstride = (size_t)nshards * 2 * N * LANES;
+    hipLaunchKernelGGL(k_k1b_variants[i].fn, dim3(HALFN / 16, 2, nshards * 2), dim3(256), 0, s, slabs, (const v4i*)ql, outb, nst,
+                       nq, qstride);
+}
+
+__global__ void fill_hash_kernel(uint32_t* p, size_t n, uint32_t seed) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        uint64_t x = (i + 1) * 0x9E3779B97F4A7C15ull ^ seed;
+        x ^= x >> 31;
+        x *= 0xBF58476D1CE4E5B9ull;
+        x ^= x >> 27;
+        p[i] = (uint32_t)(x >> 16);
+    }
+}
+
+__global__ void diff_kernel(const uint32_t* a, const uint32_t* b, size_t n, unsigned* flag) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        if (a[i] != b[i]) {
+            atomicOr(flag, 1u);
+            return;
+        }
+}
+
+void launch_fill_hash(uint32_t* p, size_t n, uint32_t seed, hipStream_t s) {
+    hipLaunchKernelGGL(fill_hash_kernel, dim3(2048), dim3(256), 0, s, p, n, seed);
+}
+void launch_diff(const uint32_t* a, const uint32_t* b, size_t n, unsigned* flag, hipStream_t s) {
+    hipLaunchKernelGGL(diff_kernel, dim3(2048), dim3(256), 0, s, a, b, n, flag);
 }
 
 void launch_chain_out(const uint32_t* a_out, const uint32_t* c_out, int nst, int nlanes, uint64_t* a_coef, uint64_t* b_coef,

@@ -30,6 +30,18 @@ size_t qlimb_bytes(int nq, int nst);
 // main_start (optional) is recorded right before the inner-product launch
 void launch_k1b(const uint32_t* const* slabs, int nshards, const uint32_t* qp, int nq, uint32_t* ql, uint32_t* outb, int nst,
                 hipStream_t s, hipEvent_t main_start = nullptr);
+// batched inner-product candidates (all exact), chosen per (column tiles, streams)
+int k1b_variant_count();
+const char* k1b_variant_name(int i);
+int k1b_variant_ct(int i);
+void k1b_set_choice(int ct, int nst, int i);
+int k1b_get_choice(int ct, int nst);
+// variant i on query limbs already in ql (qlimb layout for column tiles k1b_variant_ct(i))
+void launch_k1b_variant(int i, const uint32_t* const* slabs, int nshards, const uint32_t* ql, uint32_t* outb, int nst, int nq,
+                        hipStream_t s);
+void launch_fill_hash(uint32_t* p, size_t n, uint32_t seed, hipStream_t s);
+// *flag |= 1 where a and b differ
+void launch_diff(const uint32_t* a, const uint32_t* b, size_t n, unsigned* flag, hipStream_t s);
 void launch_chain_out(const uint32_t* a_out, const uint32_t* c_out, int nst, int nlanes, uint64_t* a_coef, uint64_t* b_coef,
                       uint32_t* a17, uint32_t* b12, const DevTables* T, hipStream_t s);
 void launch_qprep(const uint64_t* qcoef, uint32_t* qs, uint32_t* qp, int nq, const DevTables* T, hipStream_t s);

@@ -14,8 +14,8 @@ import ctypes
 
 import numpy as np
 
-from ._lib import (LANES, N, QUERY_KEYLESS, Arrival, DnspirError, RequestInfo, ServerConfig, ServerStats, Timing, check,
-                   lib)
+from ._lib import (LANES, N, QUERY_KEYLESS, Arrival, DnspirError, RequestInfo, ServerConfig, ServerStats, Timing,
+                   TuneEntry, check, lib)
 
 _ptr = lambda a, t: a.ctypes.data_as(t)  # noqa: E731
 u8p = ctypes.POINTER(ctypes.c_uint8)
@@ -122,6 +122,14 @@ class Context:
     def stream(self):
         return lib().dnspir_ctx_stream(self.h)
 
+    def set_graphs(self, on):
+        """Batched calls replay a captured HIP graph (default on); returns
+        True if capture failed on this context (eager launches)."""
+        rc = lib().dnspir_ctx_set_graphs(self.h, 1 if on else 0)
+        if rc < 0:
+            raise DnspirError(rc, "ctx_set_graphs")
+        return rc == 1
+
     def __del__(self):
         h = getattr(self, "h", None)
         if h is not None and h.value:
@@ -216,6 +224,19 @@ def process_batch_device(fleet, queries_dev, qlen, nq, replies_dev, ctx=None):
                                             ctypes.c_void_p(replies_dev.data_ptr()), ctypes.byref(tm)),
           "process_batch_device")
     return tm
+
+
+def autotune_batch(fleet, nq, ctx=None):
+    """dnspir_autotune_batch: times the batched inner-product candidates for
+    batches of nq queries on this fleet, keeps the fastest exact one; returns
+    [{name, column_tiles, ms, exact, chosen}]."""
+    init()
+    f, n = _fleet(fleet)
+    ents = (TuneEntry * 32)()
+    cnt = ctypes.c_int()
+    check(lib().dnspir_autotune_batch(ctx.h if ctx else None, f, n, nq, ents, 32, ctypes.byref(cnt)), "autotune_batch")
+    return [{"name": e.name.decode(), "column_tiles": e.column_tiles, "ms": round(e.ms, 4), "exact": bool(e.exact),
+             "chosen": bool(e.chosen)} for e in ents[:cnt.value]]
 
 
 def get_database_shape(num_entries):

@@ -96,6 +96,11 @@ int dnspir_shard_perform_pir_split(const dnspir_shard* shard, const uint64_t* qb
 int dnspir_ctx_create(void* stream, dnspir_ctx** out);
 int dnspir_ctx_destroy(dnspir_ctx* ctx);
 void* dnspir_ctx_stream(dnspir_ctx* ctx);
+/* Batched calls (dnspir_process_batch*) run their device work as a captured
+ * HIP graph, replayed while shapes and buffers stay the same (default on).
+ * Returns 1 if capture failed once on this context (it then launches
+ * eagerly), else 0. */
+int dnspir_ctx_set_graphs(dnspir_ctx* ctx, int on);
 
 /* ---- two-level composition: perform_batched_double_pir (double_pir.rs:126-228)
  * fleets: random-access shard lists (shards may repeat). q0/q1: query blocks
@@ -133,6 +138,22 @@ int dnspir_process_batch(dnspir_ctx* ctx, const dnspir_shard* const* fleet, size
 /* Same with queries and replies in device memory. */
 int dnspir_process_batch_device(dnspir_ctx* ctx, const dnspir_shard* const* fleet, size_t len, const uint8_t* queries_dev,
                                 size_t qlen, int nq, uint8_t* replies_dev, dnspir_timing* timing);
+/* Autotuner of the batched inner product: times every candidate kernel
+ * (query-tile staging, prefetch depth, unrolling) for batches of nq queries
+ * (3..64) on this fleet with random operands, checks that all of them give
+ * identical results, and makes the fastest the one used for that batch class
+ * (column tiles = nq > 32 ? 2 : 1, conjugated streams or not) from then on.
+ * The limb split is not tuned: 4 x 4 signed base-256 digits is the only exact
+ * int8 split of 26-bit residues. entries: up to max_entries results. */
+typedef struct {
+    char name[48];
+    int column_tiles;
+    double ms;               /* mean device time of one launch */
+    int exact;               /* 1: output identical to the first candidate's */
+    int chosen;
+} dnspir_tune_entry;
+int dnspir_autotune_batch(dnspir_ctx* ctx, const dnspir_shard* const* fleet, size_t len, int nq, dnspir_tune_entry* entries,
+                          int max_entries, int* n_entries);
 /* get_database_shape (pir_wrapper.rs:219-231): index-group orders */
 void dnspir_database_shape(size_t num_entries, int* l_primary, int* l_secondary);
 /* num_databases (pir_wrapper.rs:238-241) */

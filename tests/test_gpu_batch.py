@@ -109,3 +109,57 @@ def test_batch_rejects_mixed_sizes_and_short(pir):
     with pytest.raises(DnspirError) as ei:
         pir.process_batch(g, [qs[0][2][:-8], qs[1][2][:-8]])
     assert ei.value.code == -1
+
+
+@pytest.mark.parametrize("nq", [10, 40])
+def test_autotune_candidates_are_exact(pir, nq):
+    """Every batched inner-product candidate gives identical results; the chosen
+    one then answers batches byte-identically to single queries."""
+    g, _, e = _fleet(pir, 1024, 3)
+    res = pir.autotune_batch(g, nq)
+    _say(f"[batch] autotune nq={nq}: {res}")
+    assert len(res) >= 4 and all(r["exact"] for r in res)
+    assert sum(r["chosen"] for r in res) == 1
+    qs = _queries(pir, nq, 1024, 3, 90 + nq)
+    reps, _ = pir.process_batch(g, [q for _, _, q, _ in qs])
+    for k in (0, nq - 1):
+        single, _ = pir.process_queries(g, None, qs[k][2], None)
+        assert reps[k] == single[0]
+    for k, (pi, si, q, sk) in enumerate(qs):
+        assert np.array_equal(pir.process_reply(reps[k], sk), e[si][pi]), k
+
+
+def test_batch_graph_replays_match_eager(pir):
+    """The batched pipeline replayed as a captured HIP graph: every replay,
+    with eager work and synchronisation between replays and the query bytes
+    changed in place, equals the eager launch for the same queries."""
+    import torch
+    g, _, e = _fleet(pir, 1024, 3)
+    sets = [_queries(pir, 12, 1024, 3, s) for s in (55, 56)]
+    qlen = len(sets[0][0][2])
+    rb = pir.reply_bytes()
+    host = [np.frombuffer(b"".join(q for _, _, q, _ in qs), np.uint8).copy() for qs in sets]
+    qdev = torch.from_numpy(host[0]).cuda()
+    rdev = torch.zeros(12 * rb, dtype=torch.uint8, device="cuda")
+    ctx = pir.Context()
+    ctx.set_graphs(False)
+    eager = []
+    for h in host:
+        qdev.copy_(torch.from_numpy(h))
+        pir.process_batch_device(g, qdev, qlen, 12, rdev, ctx=ctx)
+        eager.append(rdev.cpu().numpy().copy())
+    assert not np.array_equal(eager[0], eager[1])
+    assert ctx.set_graphs(True) is False
+    for rep in range(6):
+        k = rep % 2
+        qdev.copy_(torch.from_numpy(host[k]))
+        rdev.zero_()
+        torch.cuda.synchronize()
+        _say(f"[batch] graph replay {rep}")
+        pir.process_batch_device(g, qdev, qlen, 12, rdev, ctx=ctx)
+        assert np.array_equal(rdev.cpu().numpy(), eager[k]), rep
+        pi, si, q, sk = sets[k][rep]
+        single, _ = pir.process_queries(g, None, q, None)
+        assert single[0] == eager[k][rep * rb:(rep + 1) * rb].tobytes()
+        assert np.array_equal(pir.process_reply(single[0], sk), e[si][pi])
+    assert ctx.set_graphs(True) is False  # capture worked: no eager fallback

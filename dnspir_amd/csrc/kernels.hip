@@ -262,34 +262,75 @@ __global__ __launch_bounds__(256) void k5_col_ntt_kernel(const uint16_t* __restr
 // in place in M. Row s: twist by psi^(-coset j), DFT over j, then
 // new[i] = evaluation at psi^(-z 5^i) (main half) / psi^(z 5^(i-N/2)) (conj
 // half), z = e_s * k mod 2N.
-constexpr int K5_TS = 8;
+// Stages top, top-1, ..., top-R+1 of the DIF transform over the l rows of a
+// tile [l][TS] (TS interleaved columns), in registers per group of 2^R
+// elements; lazy values in [0, 2p), Shoup products by psi^-(2N/2^(lh+1) j).
+template <int TS, int R>
+__device__ __forceinline__ void k5_dif_pass(uint32_t* tile, int logl, int top, const DevTables* __restrict__ T, int tid) {
+    constexpr int G = 1 << R;
+    const int sh = top - R + 1;  // group stride 2^sh in rows
+    const int ngroups = (1 << logl) / G * TS;
+    for (int g = tid; g < ngroups; g += 256) {
+        const int d = g % TS, gi = g / TS;
+        const int jlow = gi & ((1 << sh) - 1), blk = gi >> sh;
+        const int base = (blk << (top + 1)) + jlow;
+        uint32_t v[G];
+#pragma unroll
+        for (int k = 0; k < G; ++k) v[k] = tile[(base + (k << sh)) * TS + d];
+#pragma unroll
+        for (int t = 0; t < R; ++t) {
+            const int lh = top - t, hk = G >> (t + 1);
+            const int tws = M2N >> (lh + 1);
+#pragma unroll
+            for (int k = 0; k < G; ++k) {
+                if (k & hk) continue;
+                const int j = (base + (k << sh)) & ((1 << lh) - 1);
+                const int e = (M2N - ((tws * j) & (M2N - 1))) & (M2N - 1);
+                bf_gs(v[k], v[k + hk], T->psipow[0][e], T->psipowq[0][e], P1);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < G; ++k) tile[(base + (k << sh)) * TS + d] = v[k];
+    }
+}
+
+template <int TS>
 __global__ __launch_bounds__(256) void k5_rows_kernel(uint32_t* __restrict__ M, int l, int logl, int kq, int conj,
                                                       const DevTables* __restrict__ T) {
     extern __shared__ uint32_t tile[];  // [l][TS]
-    const int s0 = blockIdx.x * K5_TS, li = blockIdx.y;
+    const int s0 = blockIdx.x * TS, li = blockIdx.y;
     uint32_t* base = M + (size_t)li * l * N;
-    const int tid = threadIdx.x, nt = blockDim.x;
-    const int step = M2N / l;
-    for (int idx = tid; idx < l * K5_TS; idx += nt) {
-        const int j = idx / K5_TS, d = idx % K5_TS;
+    const int tid = threadIdx.x;
+    const int lstep = LOGN + 1 - logl;  // 2N / l = 2^lstep
+    for (int idx = tid; idx < l * TS; idx += 256) {
+        const int j = idx / TS, d = idx % TS;
         const int s = s0 + d;
         const int z = (T->slot_exp[s] * kq) & (M2N - 1);
-        const int coset = z % step;
+        const int coset = z & ((1 << lstep) - 1);
         const int e = (M2N - ((coset * j) & (M2N - 1))) & (M2N - 1);
         tile[idx] = mul_shoup(base[(size_t)j * N + s], T->psipow[0][e], T->psipowq[0][e], P1);
     }
     __syncthreads();
-    dft_dif_p1(tile, logl, K5_TS, T, tid, nt);
-    for (int idx = tid; idx < l * K5_TS; idx += nt) {
-        const int i = idx / K5_TS, d = idx % K5_TS;
+    int top = logl - 1;
+    while (top >= 2) {
+        k5_dif_pass<TS, 3>(tile, logl, top, T, tid);
+        __syncthreads();
+        top -= 3;
+    }
+    if (top == 1) k5_dif_pass<TS, 2>(tile, logl, top, T, tid);
+    else if (top == 0) k5_dif_pass<TS, 1>(tile, logl, top, T, tid);
+    if (top >= 0) __syncthreads();
+    for (int idx = tid; idx < l * TS; idx += 256) {
+        const int i = idx / TS, d = idx % TS;
         const int s = s0 + d;
         const int z = (T->slot_exp[s] * kq) & (M2N - 1);
-        const int coset = z % step;
+        const int coset = z & ((1 << lstep) - 1);
         int y;
         if (conj && i >= N / 2) y = (z * T->pow5[i - N / 2]) & (M2N - 1);
         else y = ((M2N - z) * T->pow5[i]) & (M2N - 1);
-        const int u = (((M2N - y) & (M2N - 1)) - coset) / step;  // natural DFT index
-        base[(size_t)i * N + s] = tile[brv_bits(u, logl) * K5_TS + d];
+        const int u = (((M2N - y) & (M2N - 1)) - coset) >> lstep;  // natural DFT index
+        const uint32_t v = tile[brv_bits(u, logl) * TS + d];
+        base[(size_t)i * N + s] = v >= P1 ? v - P1 : v;
     }
 }
 
@@ -1206,7 +1247,13 @@ void launch_k5(const uint16_t* entries, const long long* ent_off, long long col_
     for (int x = 1; x < (int)T_PLAIN; ++x)
         if ((x * l) % (int)T_PLAIN == 1) { scale = x; break; }
     hipLaunchKernelGGL(k5_col_ntt_kernel, dim3(l, nlanes), dim3(256), 0, s, entries, ent_off, col_stride, M, l, ncols_nonzero, T);
-    hipLaunchKernelGGL(k5_rows_kernel, dim3(N / K5_TS, nlanes), dim3(256), (size_t)l * K5_TS * 4, s, M, l, logl, kq, conj, T);
+    // tiles of 64 KiB: TS slots x l rows, TS * 4 bytes contiguous per row
+    if (l <= 512)
+        hipLaunchKernelGGL(k5_rows_kernel<32>, dim3(N / 32, nlanes), dim3(256), (size_t)l * 32 * 4, s, M, l, logl, kq, conj, T);
+    else if (l == 1024)
+        hipLaunchKernelGGL(k5_rows_kernel<16>, dim3(N / 16, nlanes), dim3(256), (size_t)l * 16 * 4, s, M, l, logl, kq, conj, T);
+    else
+        hipLaunchKernelGGL(k5_rows_kernel<8>, dim3(N / 8, nlanes), dim3(256), (size_t)l * 8 * 4, s, M, l, logl, kq, conj, T);
     hipLaunchKernelGGL(k5_cols_final_kernel, dim3(l, nlanes), dim3(256), 0, s, M, te_tmp, l, L, conj, nlanes, scale, T);
 }
 

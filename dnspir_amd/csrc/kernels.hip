@@ -128,7 +128,18 @@ __device__ __forceinline__ void bf_gs(uint32_t& x, uint32_t& y, uint32_t w, uint
     y = shoup_lazy(u - v + p2, w, wq, p);
 }
 
+// LDS index of element j in a transform buffer. Padded buffers (N + N/8
+// words) insert 4 words after every 32: the radix-8 pass at stride 4 then
+// hits 32 distinct banks per half-wave instead of 4, and 4-aligned groups
+// stay 16-byte aligned.
+constexpr int NPAD = N + N / 8;
+template <bool PAD>
+__device__ __forceinline__ int lx(int j) {
+    return PAD ? j + ((j >> 5) << 2) : j;
+}
+
 // one radix-8 forward pass (stages 3P..3P+2) for group g of 256
+template <bool PAD = false>
 __device__ __forceinline__ void fwd_pass8(uint32_t* buf, const uint32_t* tw, const uint32_t* twq, uint32_t p, int P, int g) {
     const int m = 1 << (3 * P);
     const int q = N >> (3 * P + 3);
@@ -136,7 +147,7 @@ __device__ __forceinline__ void fwd_pass8(uint32_t* buf, const uint32_t* tw, con
     const int j0 = i * 8 * q + r;
     uint32_t v[8];
 #pragma unroll
-    for (int c = 0; c < 8; ++c) v[c] = buf[j0 + c * q];
+    for (int c = 0; c < 8; ++c) v[c] = buf[lx<PAD>(j0 + c * q)];
     {
         const uint32_t w = tw[m + i], wq = twq[m + i];
 #pragma unroll
@@ -154,23 +165,26 @@ __device__ __forceinline__ void fwd_pass8(uint32_t* buf, const uint32_t* tw, con
         bf_ct(v[2 * h], v[2 * h + 1], w, wq, p);
     }
 #pragma unroll
-    for (int c = 0; c < 8; ++c) buf[j0 + c * q] = v[c];
+    for (int c = 0; c < 8; ++c) buf[lx<PAD>(j0 + c * q)] = v[c];
 }
 // last forward pass: stages 9 and 10 for the two radix-4 groups g, g + 256
+template <bool PAD = false>
 __device__ __forceinline__ void fwd_pass4(uint32_t* buf, const uint32_t* tw, const uint32_t* twq, uint32_t p, int g) {
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
         const int i = g + 256 * k;
-        uint32_t v0 = buf[4 * i], v1 = buf[4 * i + 1], v2 = buf[4 * i + 2], v3 = buf[4 * i + 3];
+        const int b4 = lx<PAD>(4 * i);
+        uint32_t v0 = buf[b4], v1 = buf[b4 + 1], v2 = buf[b4 + 2], v3 = buf[b4 + 3];
         const uint32_t w = tw[512 + i], wq = twq[512 + i];
         bf_ct(v0, v2, w, wq, p);
         bf_ct(v1, v3, w, wq, p);
         bf_ct(v0, v1, tw[1024 + 2 * i], twq[1024 + 2 * i], p);
         bf_ct(v2, v3, tw[1025 + 2 * i], twq[1025 + 2 * i], p);
-        buf[4 * i] = v0; buf[4 * i + 1] = v1; buf[4 * i + 2] = v2; buf[4 * i + 3] = v3;
+        buf[b4] = v0; buf[b4 + 1] = v1; buf[b4 + 2] = v2; buf[b4 + 3] = v3;
     }
 }
 // one radix-8 inverse pass (stages 3P+2..3P)
+template <bool PAD = false>
 __device__ __forceinline__ void inv_pass8(uint32_t* buf, const uint32_t* tw, const uint32_t* twq, uint32_t p, int P, int g) {
     const int m = 1 << (3 * P);
     const int q = N >> (3 * P + 3);
@@ -178,7 +192,7 @@ __device__ __forceinline__ void inv_pass8(uint32_t* buf, const uint32_t* tw, con
     const int j0 = i * 8 * q + r;
     uint32_t v[8];
 #pragma unroll
-    for (int c = 0; c < 8; ++c) v[c] = buf[j0 + c * q];
+    for (int c = 0; c < 8; ++c) v[c] = buf[lx<PAD>(j0 + c * q)];
 #pragma unroll
     for (int h = 0; h < 4; ++h) {
         const uint32_t w = tw[4 * m + 4 * i + h], wq = twq[4 * m + 4 * i + h];
@@ -196,40 +210,47 @@ __device__ __forceinline__ void inv_pass8(uint32_t* buf, const uint32_t* tw, con
         for (int c = 0; c < 4; ++c) bf_gs(v[c], v[c + 4], w, wq, p);
     }
 #pragma unroll
-    for (int c = 0; c < 8; ++c) buf[j0 + c * q] = v[c];
+    for (int c = 0; c < 8; ++c) buf[lx<PAD>(j0 + c * q)] = v[c];
 }
 
 // first inverse pass: stages 10 and 9 for the radix-4 groups g, g + 256
+template <bool PAD = false>
 __device__ __forceinline__ void inv_pass4(uint32_t* buf, const uint32_t* tw, const uint32_t* twq, uint32_t p, int g) {
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
         const int i = g + 256 * k;
-        uint32_t v0 = buf[4 * i], v1 = buf[4 * i + 1], v2 = buf[4 * i + 2], v3 = buf[4 * i + 3];
+        const int b4 = lx<PAD>(4 * i);
+        uint32_t v0 = buf[b4], v1 = buf[b4 + 1], v2 = buf[b4 + 2], v3 = buf[b4 + 3];
         bf_gs(v0, v1, tw[1024 + 2 * i], twq[1024 + 2 * i], p);
         bf_gs(v2, v3, tw[1025 + 2 * i], twq[1025 + 2 * i], p);
         const uint32_t w = tw[512 + i], wq = twq[512 + i];
         bf_gs(v0, v2, w, wq, p);
         bf_gs(v1, v3, w, wq, p);
-        buf[4 * i] = v0; buf[4 * i + 1] = v1; buf[4 * i + 2] = v2; buf[4 * i + 3] = v3;
+        buf[b4] = v0; buf[b4 + 1] = v1; buf[b4 + 2] = v2; buf[b4 + 3] = v3;
     }
 }
 // whole-block transforms by 256 threads (g = thread index) on an LDS array;
 // inputs in [0, 2p), outputs in [0, p); the inverse leaves the N factor in
+template <bool PAD = false>
 __device__ void ntt256_fwd(uint32_t* buf, const uint32_t* tw, const uint32_t* twq, uint32_t p, int g) {
     for (int P = 0; P < 3; ++P) {
-        fwd_pass8(buf, tw, twq, p, P, g);
+        fwd_pass8<PAD>(buf, tw, twq, p, P, g);
         __syncthreads();
     }
-    fwd_pass4(buf, tw, twq, p, g);
+    fwd_pass4<PAD>(buf, tw, twq, p, g);
     __syncthreads();
-    for (int j = g; j < N; j += 256) buf[j] = buf[j] >= p ? buf[j] - p : buf[j];
+    for (int j = g; j < N; j += 256) {
+        const int x = lx<PAD>(j);
+        buf[x] = buf[x] >= p ? buf[x] - p : buf[x];
+    }
     __syncthreads();
 }
+template <bool PAD = false>
 __device__ void ntt256_inv_unscaled(uint32_t* buf, const uint32_t* tw, const uint32_t* twq, uint32_t p, int g) {
-    inv_pass4(buf, tw, twq, p, g);
+    inv_pass4<PAD>(buf, tw, twq, p, g);
     __syncthreads();
     for (int P = 2; P >= 0; --P) {
-        inv_pass8(buf, tw, twq, p, P, g);
+        inv_pass8<PAD>(buf, tw, twq, p, P, g);
         __syncthreads();
     }
 }
@@ -244,7 +265,7 @@ __device__ void ntt256_inv_unscaled(uint32_t* buf, const uint32_t* tw, const uin
 __global__ __launch_bounds__(256) void k5_col_ntt_kernel(const uint16_t* __restrict__ entries, const long long* __restrict__ ent_off,
                                                          long long col_stride, uint32_t* __restrict__ M, int l,
                                                          const int* __restrict__ ncols_nonzero, const DevTables* __restrict__ T) {
-    __shared__ uint32_t a[N];
+    __shared__ uint32_t a[NPAD];
     const int j = blockIdx.x, li = blockIdx.y;
     uint32_t* out = M + ((size_t)li * l + j) * N;
     if (j >= ncols_nonzero[li]) {
@@ -252,10 +273,10 @@ __global__ __launch_bounds__(256) void k5_col_ntt_kernel(const uint16_t* __restr
         return;
     }
     const uint16_t* src = entries + ent_off[li] + (long long)j * col_stride;
-    for (int s = threadIdx.x; s < N; s += blockDim.x) a[s] = src[s];
+    for (int s = threadIdx.x; s < N; s += blockDim.x) a[lx<true>(s)] = src[s];
     __syncthreads();
-    ntt256_fwd(a, T->psi[0], T->psiq[0], P1, threadIdx.x);
-    for (int s = threadIdx.x; s < N; s += blockDim.x) out[s] = a[s];
+    ntt256_fwd<true>(a, T->psi[0], T->psiq[0], P1, threadIdx.x);
+    for (int s = threadIdx.x; s < N; s += blockDim.x) out[s] = a[lx<true>(s)];
 }
 
 // step 3 (coset_ntt, base_pir.rs:397-444) for a tile of TS slots of one lane,
@@ -341,7 +362,7 @@ __global__ __launch_bounds__(256) void k5_rows_kernel(uint32_t* __restrict__ M, 
 __global__ __launch_bounds__(256) void k5_cols_final_kernel(const uint32_t* __restrict__ M, uint32_t* __restrict__ te_tmp,
                                                             int l, int L, int conj, int nlanes, int scale,
                                                             const DevTables* __restrict__ T) {
-    __shared__ uint32_t a[2][N];
+    __shared__ uint32_t a[2][NPAD];
     const int c = blockIdx.x, li = blockIdx.y;
     const int tid = threadIdx.x, nt = blockDim.x;
     int j;
@@ -349,11 +370,12 @@ __global__ __launch_bounds__(256) void k5_cols_final_kernel(const uint32_t* __re
     else j = l - 1 - c;
     const uint32_t* col = M + ((size_t)li * l + j) * N;
     const int au = T->inv5pow[j & (HALFN - 1)];
-    for (int s = tid; s < N; s += nt) a[0][s] = col[T->exp_slot[(au * T->slot_exp[s]) & (M2N - 1)]];
+    for (int s = tid; s < N; s += nt) a[0][lx<true>(s)] = col[T->exp_slot[(au * T->slot_exp[s]) & (M2N - 1)]];
     __syncthreads();
-    ntt256_inv_unscaled(a[0], T->ipsi[0], T->ipsiq[0], P1, tid);
+    ntt256_inv_unscaled<true>(a[0], T->ipsi[0], T->ipsiq[0], P1, tid);
     for (int s = tid; s < N; s += nt) {
-        uint32_t u = a[0][s];
+        const int xs = lx<true>(s);
+        uint32_t u = a[0][xs];
         u = u >= P1 ? u - P1 : u;
         int32_t x = (int32_t)mul_shoup(u, T->ninv[0], T->ninvq[0], P1);
         if (x > (int32_t)(P1 / 2)) x -= (int32_t)P1;
@@ -361,18 +383,18 @@ __global__ __launch_bounds__(256) void k5_cols_final_kernel(const uint32_t* __re
         if (xt < 0) xt += T_PLAIN;
         int32_t y = (int32_t)(((int64_t)xt * scale) % T_PLAIN);
         if (y > (int32_t)(T_PLAIN / 2)) y -= (int32_t)T_PLAIN;
-        a[0][s] = y < 0 ? (uint32_t)(y + (int32_t)P1) : (uint32_t)y;
-        a[1][s] = y < 0 ? (uint32_t)(y + (int32_t)P2) : (uint32_t)y;
+        a[0][xs] = y < 0 ? (uint32_t)(y + (int32_t)P1) : (uint32_t)y;
+        a[1][xs] = y < 0 ? (uint32_t)(y + (int32_t)P2) : (uint32_t)y;
     }
     __syncthreads();
-    ntt256_fwd(a[0], T->psi[0], T->psiq[0], P1, tid);
-    ntt256_fwd(a[1], T->psi[1], T->psiq[1], P2, tid);
+    ntt256_fwd<true>(a[0], T->psi[0], T->psiq[0], P1, tid);
+    ntt256_fwd<true>(a[1], T->psi[1], T->psiq[1], P2, tid);
     const int st = c < L ? 0 : 1;
     const int row = c < L ? c : c - L;
     uint32_t* dst = te_tmp + (((size_t)(st * nlanes + li) * L + row) * 2) * N;
     for (int s = tid; s < N; s += nt) {
-        dst[s] = a[0][s];
-        dst[N + s] = a[1][s];
+        dst[s] = a[0][lx<true>(s)];
+        dst[N + s] = a[1][lx<true>(s)];
     }
 }
 
@@ -402,7 +424,7 @@ __global__ __launch_bounds__(1024) void chain_kernel(const uint32_t* __restrict_
     __shared__ uint32_t tw[2][2][N], twq[2][2][N];  // [fwd/inv][r][k]
     __shared__ uint32_t sa[2][N];
     __shared__ uint32_t sc[2][2][N];                // [buffer][r][slot]
-    __shared__ uint32_t xb[4][N];
+    __shared__ uint32_t xb[4][NPAD];  // transform buffers, padded index lx<true>
     const int li = blockIdx.x, st = blockIdx.y;
     const int tid = threadIdx.x;
     for (int k = tid; k < N; k += 1024) {
@@ -470,20 +492,22 @@ __global__ __launch_bounds__(1024) void chain_kernel(const uint32_t* __restrict_
                 bf_gs(v0, v2, w, wq, p);
                 bf_gs(v1, v3, w, wq, p);
                 uint32_t* b = xb[rA];
-                b[4 * ii] = v0; b[4 * ii + 1] = v1; b[4 * ii + 2] = v2; b[4 * ii + 3] = v3;
+                const int b4 = lx<true>(4 * ii);
+                b[b4] = v0; b[b4 + 1] = v1; b[b4 + 2] = v2; b[b4 + 3] = v3;
             }
         }
         __syncthreads();
 #pragma unroll 1
         for (int P = 2; P >= 0; --P) {
-            if (tid < 512) inv_pass8(xb[rA], tw[1][rA], twq[1][rA], mod_p(rA), P, gA);
+            if (tid < 512) inv_pass8<true>(xb[rA], tw[1][rA], twq[1][rA], mod_p(rA), P, gA);
             __syncthreads();
         }
         // ---- B: CRT, centered lift, digits (base_pir.rs:544-570) ----
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
             const int j = tid + 1024 * k;
-            uint32_t x1 = xb[0][j], x2 = xb[1][j];
+            const int xj = lx<true>(j);
+            uint32_t x1 = xb[0][xj], x2 = xb[1][xj];
             x1 = x1 >= P1 ? x1 - P1 : x1;
             x2 = x2 >= P2 ? x2 - P2 : x2;
             const uint32_t e1 = mul_mod(x1, NINV_Q2INV_P1, P1, 26);
@@ -496,10 +520,10 @@ __global__ __launch_bounds__(1024) void chain_kernel(const uint32_t* __restrict_
             const int32_t hi2 = (int32_t)((hi + (1ll << (LOG2_B - 1))) >> LOG2_B);
             const int32_t lo2 = (int32_t)(hi - ((int64_t)hi2 << LOG2_B));   // |lo2| <= 2^24
             uint32_t t;
-            t = (uint32_t)(lo + (int32_t)(2 * P1)); t = t >= 2 * P1 ? t - 2 * P1 : t; xb[0][j] = t >= P1 ? t - P1 : t;
-            t = (uint32_t)(lo + (int32_t)(2 * P2)); t = t >= 2 * P2 ? t - 2 * P2 : t; xb[1][j] = t >= P2 ? t - P2 : t;
-            t = (uint32_t)(lo2 + (int32_t)P1); xb[2][j] = t >= P1 ? t - P1 : t;
-            t = (uint32_t)(lo2 + (int32_t)P2); xb[3][j] = t >= P2 ? t - P2 : t;
+            t = (uint32_t)(lo + (int32_t)(2 * P1)); t = t >= 2 * P1 ? t - 2 * P1 : t; xb[0][xj] = t >= P1 ? t - P1 : t;
+            t = (uint32_t)(lo + (int32_t)(2 * P2)); t = t >= 2 * P2 ? t - 2 * P2 : t; xb[1][xj] = t >= P2 ? t - P2 : t;
+            t = (uint32_t)(lo2 + (int32_t)P1); xb[2][xj] = t >= P1 ? t - P1 : t;
+            t = (uint32_t)(lo2 + (int32_t)P2); xb[3][xj] = t >= P2 ? t - P2 : t;
         }
         __syncthreads();
         // ---- C: forward NTTs of the four digit vectors ----
@@ -507,10 +531,10 @@ __global__ __launch_bounds__(1024) void chain_kernel(const uint32_t* __restrict_
             const uint32_t p = mod_p(rC);
 #pragma unroll 1
             for (int P = 0; P < 3; ++P) {
-                fwd_pass8(xb[q4], tw[0][rC], twq[0][rC], p, P, gC);
+                fwd_pass8<true>(xb[q4], tw[0][rC], twq[0][rC], p, P, gC);
                 __syncthreads();
             }
-            fwd_pass4(xb[q4], tw[0][rC], twq[0][rC], p, gC);
+            fwd_pass4<true>(xb[q4], tw[0][rC], twq[0][rC], p, gC);
         }
         __syncthreads();
         // ---- D: update a (and c) per slot ----
@@ -519,7 +543,7 @@ __global__ __launch_bounds__(1024) void chain_kernel(const uint32_t* __restrict_
             const int s = tid + 1024 * (k >> 1), r = k & 1;
             const uint32_t p = mod_p(r);
             const int K = mod_k(r);
-            const uint32_t d0 = xb[r][s], d1 = xb[2 + r][s];  // [0, 2p)
+            const uint32_t d0 = xb[r][lx<true>(s)], d1 = xb[2 + r][lx<true>(s)];  // [0, 2p)
             sa[r][s] = red64((uint64_t)d0 * cg0[k] + (uint64_t)d1 * cg1[k] + (uint64_t)te[k] * cqa[k], p, K);
             if (SEC) {
                 const uint64_t cv = (uint64_t)sc[cur][r][sig_upd[k >> 1]] + (uint64_t)d0 * cb0[k] + (uint64_t)d1 * cb1[k] +

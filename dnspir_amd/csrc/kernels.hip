@@ -104,7 +104,7 @@ __device__ __forceinline__ int brv_bits(int v, int bits) {
 
 // ---------------------------------------------------------------------------
 // register-radix NTT passes (radix 8, radix 4 for stages 9-10) over an LDS
-// array; Harvey lazy values in [0, 2p), Shoup multiplications by twiddles
+// array; Harvey lazy values ([0, 4p) forward, [0, 2p) inverse), Shoup products by twiddles
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t red2p(uint32_t a, uint32_t p2) {  // [0, 4p) -> [0, 2p)
     const uint32_t b = a - p2;
@@ -113,12 +113,13 @@ __device__ __forceinline__ uint32_t red2p(uint32_t a, uint32_t p2) {  // [0, 4p)
 __device__ __forceinline__ uint32_t shoup_lazy(uint32_t a, uint32_t w, uint32_t wq, uint32_t p) {  // -> [0, 2p)
     return a * w - __umulhi(a, wq) * p;
 }
-// forward CT butterfly, inputs/outputs in [0, 2p)
+// forward CT butterfly (Harvey): inputs and outputs in [0, 4p)
 __device__ __forceinline__ void bf_ct(uint32_t& x, uint32_t& y, uint32_t w, uint32_t wq, uint32_t p) {
-    const uint32_t t = shoup_lazy(y, w, wq, p);
     const uint32_t p2 = 2 * p;
-    y = red2p(x - t + p2, p2);
-    x = red2p(x + t, p2);
+    const uint32_t u = red2p(x, p2);
+    const uint32_t t = shoup_lazy(y, w, wq, p);
+    x = u + t;
+    y = u - t + p2;
 }
 // inverse GS butterfly, inputs/outputs in [0, 2p)
 __device__ __forceinline__ void bf_gs(uint32_t& x, uint32_t& y, uint32_t w, uint32_t wq, uint32_t p) {
@@ -230,7 +231,8 @@ __device__ __forceinline__ void inv_pass4(uint32_t* buf, const uint32_t* tw, con
     }
 }
 // whole-block transforms by 256 threads (g = thread index) on an LDS array;
-// inputs in [0, 2p), outputs in [0, p); the inverse leaves the N factor in
+// inputs in [0, 2p), outputs in [0, p) (forward) / [0, 2p) (inverse, which
+// leaves the N factor in)
 template <bool PAD = false>
 __device__ void ntt256_fwd(uint32_t* buf, const uint32_t* tw, const uint32_t* twq, uint32_t p, int g) {
     for (int P = 0; P < 3; ++P) {
@@ -241,7 +243,8 @@ __device__ void ntt256_fwd(uint32_t* buf, const uint32_t* tw, const uint32_t* tw
     __syncthreads();
     for (int j = g; j < N; j += 256) {
         const int x = lx<PAD>(j);
-        buf[x] = buf[x] >= p ? buf[x] - p : buf[x];
+        uint32_t v = red2p(buf[x], 2 * p);
+        buf[x] = v >= p ? v - p : v;
     }
     __syncthreads();
 }
@@ -543,7 +546,7 @@ __global__ __launch_bounds__(1024) void chain_kernel(const uint32_t* __restrict_
             const int s = tid + 1024 * (k >> 1), r = k & 1;
             const uint32_t p = mod_p(r);
             const int K = mod_k(r);
-            const uint32_t d0 = xb[r][lx<true>(s)], d1 = xb[2 + r][lx<true>(s)];  // [0, 2p)
+            const uint32_t d0 = xb[r][lx<true>(s)], d1 = xb[2 + r][lx<true>(s)];  // [0, 4p)
             sa[r][s] = red64((uint64_t)d0 * cg0[k] + (uint64_t)d1 * cg1[k] + (uint64_t)te[k] * cqa[k], p, K);
             if (SEC) {
                 const uint64_t cv = (uint64_t)sc[cur][r][sig_upd[k >> 1]] + (uint64_t)d0 * cb0[k] + (uint64_t)d1 * cb1[k] +
@@ -554,8 +557,9 @@ __global__ __launch_bounds__(1024) void chain_kernel(const uint32_t* __restrict_
                 uint32_t* dst = ip_tmp + (((size_t)(st * nlanes + li) * L + i) * 6) * N;
                 const int gp = gpos_upd[k >> 1];
                 dst[(0 * 2 + r) * N + gp] = te[k];
-                dst[(1 * 2 + r) * N + gp] = d0 >= p ? d0 - p : d0;
-                dst[(2 * 2 + r) * N + gp] = d1 >= p ? d1 - p : d1;
+                const uint32_t e0 = red2p(d0, 2 * p), e1 = red2p(d1, 2 * p);
+                dst[(1 * 2 + r) * N + gp] = e0 >= p ? e0 - p : e0;
+                dst[(2 * 2 + r) * N + gp] = e1 >= p ? e1 - p : e1;
             }
         }
         cur ^= 1;
@@ -1201,29 +1205,38 @@ struct EpiArgs {
     uint64_t* bcoef;                  // [shard][lane][N] (query index = lane when diag), or null
     int diag;                         // query q = lane (perform_pir_split)
 };
-__global__ __launch_bounds__(256) void epi_primary_kernel(EpiArgs A, const DevTables* __restrict__ T) {
-    __shared__ uint32_t w[2][N];
+__global__ __launch_bounds__(512) void epi_primary_kernel(EpiArgs A, const DevTables* __restrict__ T) {
+    __shared__ uint32_t w[2][NPAD];
     const int sh = blockIdx.x, lane = blockIdx.y;
     const int q = A.diag ? lane : blockIdx.z;
-    const int tid = threadIdx.x, nt = blockDim.x;
-    for (int s = tid; s < N; s += nt) {
+    const int tid = threadIdx.x;
+    for (int s = tid; s < N; s += 512) {
         const int g = T->slot_gpos[s];
+        const int x = lx<true>(s);
         if (A.outb) {
             const uint32_t* o = A.outb + (size_t)q * A.q_stride + (size_t)sh * 2 * N * LANES;
-            w[0][s] = o[(size_t)g * LANES + lane];
-            w[1][s] = o[((size_t)N + g) * LANES + lane];
+            w[0][x] = o[(size_t)g * LANES + lane];
+            w[1][x] = o[((size_t)N + g) * LANES + lane];
         } else {
-            w[0][s] = red64(A.acc[(((size_t)sh * 2 + 0) * N + g) * LANES + lane], P1, 26);
-            w[1][s] = red64(A.acc[(((size_t)sh * 2 + 1) * N + g) * LANES + lane], P2, 25);
+            w[0][x] = red64(A.acc[(((size_t)sh * 2 + 0) * N + g) * LANES + lane], P1, 26);
+            w[1][x] = red64(A.acc[(((size_t)sh * 2 + 1) * N + g) * LANES + lane], P2, 25);
         }
     }
     __syncthreads();
     {
-        const int half = tid >= nt / 2;
-        ntt_inv(w[half], T, half, tid - half * (nt / 2), nt / 2);
+        // one modulus per 256 threads; N^-1 is folded into the CRT below
+        const int r = tid >> 8;
+        ntt256_inv_unscaled<true>(w[r], T->ipsi[r], T->ipsiq[r], mod_p(r), tid & 255);
     }
-    for (int s = tid; s < N; s += nt) {
-        const uint64_t b = crt(w[0][s], w[1][s]);
+    for (int s = tid; s < N; s += 512) {
+        const int x = lx<true>(s);
+        uint32_t x1 = w[0][x], x2 = w[1][x];
+        x1 = x1 >= P1 ? x1 - P1 : x1;
+        x2 = x2 >= P2 ? x2 - P2 : x2;
+        const uint32_t e1 = mul_mod(x1, NINV_Q2INV_P1, P1, 26);
+        const uint32_t e2 = mul_mod(x2, NINV_Q1INV_P2, P2, 25);
+        uint64_t b = (uint64_t)e1 * P2 + (uint64_t)e2 * P1;
+        b = b >= Q ? b - Q : b;
         if (A.bcoef) A.bcoef[((size_t)sh * LANES + lane) * N + s] = b;
         if (A.se) {
             const uint32_t b12 = mod_switch_enc(b, 12);
@@ -1480,7 +1493,7 @@ void launch_epi(const EpiLaunch& e, const DevTables* T, hipStream_t s) {
     A.lane0_mult = e.lane0_mult;
     A.bcoef = e.bcoef;
     A.diag = e.diag;
-    hipLaunchKernelGGL(epi_primary_kernel, dim3(e.nshards, LANES, e.diag ? 1 : e.nq), dim3(256), 0, s, A, T);
+    hipLaunchKernelGGL(epi_primary_kernel, dim3(e.nshards, LANES, e.diag ? 1 : e.nq), dim3(512), 0, s, A, T);
 }
 
 // ---------------------------------------------------------------------------

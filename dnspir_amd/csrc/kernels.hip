@@ -235,6 +235,7 @@ __device__ __forceinline__ void inv_pass4(uint32_t* buf, const uint32_t* tw, con
 // leaves the N factor in)
 template <bool PAD = false>
 __device__ void ntt256_fwd(uint32_t* buf, const uint32_t* tw, const uint32_t* twq, uint32_t p, int g) {
+#pragma unroll
     for (int P = 0; P < 3; ++P) {
         fwd_pass8<PAD>(buf, tw, twq, p, P, g);
         __syncthreads();
@@ -252,6 +253,7 @@ template <bool PAD = false>
 __device__ void ntt256_inv_unscaled(uint32_t* buf, const uint32_t* tw, const uint32_t* twq, uint32_t p, int g) {
     inv_pass4<PAD>(buf, tw, twq, p, g);
     __syncthreads();
+#pragma unroll
     for (int P = 2; P >= 0; --P) {
         inv_pass8<PAD>(buf, tw, twq, p, P, g);
         __syncthreads();
@@ -473,12 +475,6 @@ __global__ __launch_bounds__(1024) void chain_kernel(const uint32_t* __restrict_
     int cur = 0;
 #pragma unroll 1
     for (int i = 0; i < L; ++i) {
-        // TE row of this step (used in phase D)
-        uint32_t te[4];
-        const uint32_t* te_row = te_base + (size_t)i * 2 * N;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) te[k] = te_row[(k & 1) * N + tid + 1024 * (k >> 1)];
-
         // ---- A: inverse NTTs of sigma_5(a) ----
         if (tid < 512) {
             const uint32_t p = mod_p(rA);
@@ -529,10 +525,15 @@ __global__ __launch_bounds__(1024) void chain_kernel(const uint32_t* __restrict_
             t = (uint32_t)(lo2 + (int32_t)P2); xb[3][xj] = t >= P2 ? t - P2 : t;
         }
         __syncthreads();
+        // TE row of this step (used in phase D; loaded here so phase C hides it)
+        uint32_t te[4];
+        const uint32_t* te_row = te_base + (size_t)i * 2 * N;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) te[k] = te_row[(k & 1) * N + tid + 1024 * (k >> 1)];
         // ---- C: forward NTTs of the four digit vectors ----
         {
             const uint32_t p = mod_p(rC);
-#pragma unroll 1
+#pragma unroll  // constant pass index (the inverse loop above stays rolled: 128 VGPRs)
             for (int P = 0; P < 3; ++P) {
                 fwd_pass8<true>(xb[q4], tw[0][rC], twq[0][rC], p, P, gC);
                 __syncthreads();

@@ -993,6 +993,89 @@ __global__ __launch_bounds__(256) void k1b_u(const uint32_t* const* __restrict__
 }
 
 
+// Two column tiles on separate waves: a block of 8 waves covers 4 output
+// groups x 2 tiles of 32 queries, so each wave holds 7 accumulators (two
+// waves per SIMD fit) and the two waves of a group read the same shard
+// blocks at about the same time (the second read hits L1/L2; with NT the
+// loads are non-temporal). Query tiles as in k1b_r; U steps per iteration.
+template <int PD, int U, bool NT>
+__global__ __launch_bounds__(512) void k1b_pair(const uint32_t* const* __restrict__ slabs, const v4i* __restrict__ ql,
+                                                uint32_t* __restrict__ outb, int nst, int nq, size_t q_stride) {
+    constexpr int CT = 2;
+    static_assert(U % PD == 0, "PD divides U");
+    __shared__ v4i Bt[2][CT][4][64];
+    const int S = 3 * nst;
+    const int half = blockIdx.y;
+    const int r = blockIdx.z & 1, sh = blockIdx.z >> 1;
+    const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
+    const int ct = wave >> 2;
+    const int g = blockIdx.x * 4 + (wave & 3);
+    gv4i* A = (gv4i*)(slabs[sh]) + (((size_t)(r * 2 + half) * (HALFN / 4) + g) * S) * 32 * 256 + l;
+    gv4i* Q = (gv4i*)ql + ((size_t)(r * 2 + half) * S) * 32 * CT * 256 + tid;
+    const int nk = S * 32;
+    v16i acc[7];
+#pragma unroll
+    for (int w = 0; w < 7; ++w)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) acc[w][e] = 0;
+    v4i abuf[PD][4];
+    v4i bq[PD];
+    (&Bt[0][0][0][0])[tid] = Q[0];
+#pragma unroll
+    for (int d = 0; d < PD; ++d) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) abuf[d][v] = NT ? __builtin_nontemporal_load(A + (size_t)d * 256 + v * 64) : A[(size_t)d * 256 + v * 64];
+        bq[d] = Q[(size_t)min(d + 1, nk - 1) * CT * 256];
+    }
+    for (int kk0 = 0; kk0 < nk; kk0 += U) {
+#pragma unroll
+        for (int dd = 0; dd < U; ++dd) {
+            const int d = dd % PD;
+            const int kk = kk0 + dd;
+            __syncthreads();  // tile kk visible; tile kk-1's buffer free
+            (&Bt[(kk + 1) & 1][0][0][0])[tid] = bq[d];
+            v4i a[4], b[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) a[v] = abuf[d][v];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) b[v] = Bt[kk & 1][ct][v][l];
+            const int ka = min(kk + PD, nk - 1), kb = min(kk + PD + 1, nk - 1);
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                abuf[d][v] = NT ? __builtin_nontemporal_load(A + (size_t)ka * 256 + v * 64) : A[(size_t)ka * 256 + v * 64];
+            bq[d] = Q[(size_t)kb * CT * 256];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    acc[u + v] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[u], b[v], acc[u + v], 0, 0, 0);
+        }
+    }
+    const uint32_t p = mod_p(r);
+    const int K = mod_k(r);
+    uint32_t cw[7];
+    {
+        uint64_t x = 1;
+#pragma unroll
+        for (int w = 0; w < 7; ++w) {
+            cw[w] = (uint32_t)(x % p);
+            x = (x % p) * 256;
+        }
+    }
+    const int q = ct * 32 + (l & 31);
+    if (q >= nq) return;
+    uint32_t* o = outb + (size_t)q * q_stride + ((size_t)sh * 2 + r) * N * LANES;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        int64_t t = 0;
+#pragma unroll
+        for (int w = 0; w < 7; ++w) t += (int64_t)acc[w][e] * cw[w];
+        const int row = 8 * (e >> 2) + 4 * (l >> 5) + (e & 3);
+        const int a = g * 4 + (row >> 3), lane8 = row & 7;
+        o[(half * HALFN + a) * LANES + lane8] = from_i64(t, p, K);
+    }
+}
+
 // Candidates for the batched inner product, per column-tile count. All give
 // identical results (exact integer arithmetic); the autotuner times them.
 typedef void (*K1bFn)(const uint32_t* const*, const v4i*, uint32_t*, int, int, size_t);
@@ -1000,6 +1083,7 @@ struct K1bVariant {
     const char* name;
     int ct;
     K1bFn fn;
+    int threads = 256;
 };
 static const K1bVariant k_k1b_variants[] = {
     {"lds ring, prefetch 4", 1, k1b_kernel<1, 4>}, {"register ring, prefetch 2", 1, k1b_r<1, 2>},
@@ -1009,6 +1093,10 @@ static const K1bVariant k_k1b_variants[] = {
     {"register ring, prefetch 2", 2, k1b_r<2, 2>}, {"register ring, prefetch 3", 2, k1b_r<2, 3>},
     {"register ring, prefetch 4", 2, k1b_r<2, 4>}, {"register ring, prefetch 4, unroll 8", 2, k1b_u<2, 4, 8>},
     {"register ring, prefetch 4, unroll 32", 2, k1b_u<2, 4, 32>},
+    {"tile per wave, prefetch 4", 2, k1b_pair<4, 4, false>, 512},
+    {"tile per wave, prefetch 3, unroll 6", 2, k1b_pair<3, 6, false>, 512},
+    {"tile per wave, prefetch 2, unroll 4", 2, k1b_pair<2, 4, false>, 512},
+    {"tile per wave, non-temporal, prefetch 4", 2, k1b_pair<4, 4, true>, 512},
 };
 constexpr int K1B_NVAR = sizeof(k_k1b_variants) / sizeof(k_k1b_variants[0]);
 // chosen variant per (column tiles, streams); -1 = default (register ring, prefetch 4)
@@ -1426,8 +1514,8 @@ void launch_k1b(const uint32_t* const* slabs, int nshards, const uint32_t* qp, i
 void launch_k1b_variant(int i, const uint32_t* const* slabs, int nshards, const uint32_t* ql, uint32_t* outb, int nst, int nq,
                         hipStream_t s) {
     const size_t qstride = (size_t)nshards * 2 * N * LANES;
-    hipLaunchKernelGGL(k_k1b_variants[i].fn, dim3(HALFN / 16, 2, nshards * 2), dim3(256), 0, s, slabs, (const v4i*)ql, outb, nst,
-                       nq, qstride);
+    hipLaunchKernelGGL(k_k1b_variants[i].fn, dim3(HALFN / 16, 2, nshards * 2), dim3(k_k1b_variants[i].threads), 0, s, slabs,
+                       (const v4i*)ql, outb, nst, nq, qstride);
 }
 
 __global__ void fill_hash_kernel(uint32_t* p, size_t n, uint32_t seed) {

@@ -82,6 +82,106 @@ def cpu_baseline(lp, nshards, ls, query_block, threads):
     }
 
 
+# bench_wrapped_pir's literal (primary index, shard) pairs, pir_wrapper.rs:562
+REF_INDICES = [(0, 0), (1, 0), (0, 1), (1, 1), (2, 2), (1, 2), (2, 0), (2, 1), (2, 2), (2742, 128), (923, 255),
+               (1389, 129), (200, 200), (4000, 255), (3000, 0), (3000, 1)]
+
+
+def small_cases(pir, full_shard, full_entries):
+    """The two small configurations on the GPU, host query/reply bytes as the
+    reference's process_queries call (pir_wrapper.rs:577):
+    - 32768 entries = 4 half shards, 2 resident; the reference bench's 16
+      index pairs with the client rng seeded 0 (pir_wrapper.rs:562-566);
+    - one full shard (16384 entries), 32 seeded indices.
+    Returns (report, data kept for the CPU leg)."""
+    res, keep = {}, {}
+    lp, ls = pir.get_database_shape(32768)
+    n = pir.num_databases(32768)
+    shards, entries = make_shards(pir, lp, 2, 7000)
+    fleet = [shards[k % 2] for k in range(n)]
+    rng = pir.Rng(bytes([0] + [0] * 31))
+    qs = [(i, j % n) + pir.prepare_query(rng, i, j % n, lp, ls) for i, j in REF_INDICES]
+    ctx = pir.Context()
+    pir.process_queries(fleet, None, qs[0][2], None, ctx=ctx)
+    prim, sec, tot, reps, ok = [], [], [], [], True
+    for i, j, q, sk in qs:
+        r, tm = pir.process_queries(fleet, None, q, None, ctx=ctx)
+        prim.append(tm.primary_us / 1e3)
+        sec.append(tm.secondary_us / 1e3)
+        tot.append(tm.total_us / 1e3)
+        reps.append(r[0])
+        ok &= bool(np.array_equal(pir.process_reply(r[0], sk), entries[j % 2][i].cpu().numpy().astype(np.uint16)))
+    res["cpu_case_32768"] = {
+        "workload": f"32768 entries = {n} half shards (2 resident), secondary l={ls}; the reference bench's 16 index pairs",
+        "ms_per_query_mean": round(float(np.mean(tot)), 3), "ms_per_query_std": round(float(np.std(tot)), 3),
+        "primary_ms_mean": round(float(np.mean(prim)), 3), "primary_ms_std": round(float(np.std(prim)), 3),
+        "secondary_ms_mean": round(float(np.mean(sec)), 3), "secondary_ms_std": round(float(np.std(sec)), 3),
+        "decrypt_ok": ok}
+    keep["cpu_case_32768"] = ([e.cpu().numpy().astype(np.uint16) for e in entries], lp, ls, n, qs, reps)
+    del shards, entries, fleet
+    # one full shard
+    ls1 = pir.choose_index_group_secondary(1)
+    rng = pir.Rng(bytes([2] + [0] * 31))
+    idx = np.random.default_rng(2)
+    qs = []
+    for _ in range(32):
+        i = int(idx.integers(0, 8 * 2048))
+        qs.append((i, 0) + pir.prepare_query(rng, i, 0, 2048, ls1))
+    pir.process_queries([full_shard], None, qs[0][2], None, ctx=ctx)
+    tot, k1, reps, ok = [], [], [], True
+    for i, j, q, sk in qs:
+        r, tm = pir.process_queries([full_shard], None, q, None, ctx=ctx)
+        tot.append(tm.total_us / 1e3)
+        k1.append(tm.k1_us / 1e3)
+        reps.append(r[0])
+        ok &= bool(np.array_equal(pir.process_reply(r[0], sk), full_entries[i].cpu().numpy().astype(np.uint16)))
+    k1_ms = float(np.mean(k1))
+    res["one_full_shard"] = {
+        "workload": f"one full shard (16384 entries, conjugated primary query), secondary l={ls1}; 32 seeded indices",
+        "ms_per_query_mean": round(float(np.mean(tot)), 3), "k1_ms": round(k1_ms, 4),
+        "k1_gbs": round(805306368 / (k1_ms * 1e-3) / 1e9, 1), "decrypt_ok": ok}
+    keep["one_full_shard"] = (full_entries.cpu().numpy().astype(np.uint16), qs, reps)
+    del ctx
+    return res, keep
+
+
+def cpu_cases(keep, threads):
+    """The oracle on this host's cores for the two small configurations:
+    mean/std of its primary and secondary phases and a byte comparison of
+    every GPU reply with the oracle's reply for the same query."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as orc
+    orc.set_threads(threads)
+    out = {}
+    ents, lp, ls, n, qs, reps = keep["cpu_case_32768"]
+    res = []
+    for e in ents:
+        sh = orc.Shard(11, lp)
+        sh.set_db(e)
+        res.append(sh)
+    fleet = [res[k % 2] for k in range(n)]
+    prim, sec, same = [], [], 0
+    for (i, j, q, sk), rep in zip(qs, reps):
+        orep, pu, su = orc.process_queries(fleet, None, q, None)
+        prim.append(pu / 1e3)
+        sec.append(su / 1e3)
+        same += int(orep[0] == rep)
+    out["cpu_case_32768"] = {"primary_ms_mean": round(float(np.mean(prim)), 2), "primary_ms_std": round(float(np.std(prim)), 2),
+                             "secondary_ms_mean": round(float(np.mean(sec)), 2), "secondary_ms_std": round(float(np.std(sec)), 2),
+                             "gpu_replies_byte_identical": f"{same}/{len(qs)}"}
+    e, qs, reps = keep["one_full_shard"]
+    sh = orc.Shard(11, 2048)
+    sh.set_db(e)
+    ms, same = [], 0
+    for (i, j, q, sk), rep in zip(qs, reps):
+        orep, pu, su = orc.process_queries([sh], None, q, None)
+        ms.append((pu + su) / 1e3)
+        same += int(orep[0] == rep)
+    out["one_full_shard"] = {"ms_per_query_mean": round(float(np.mean(ms)), 2),
+                             "gpu_replies_byte_identical": f"{same}/{len(qs)}"}
+    return out
+
+
 def make_shards(pir, lp, count, seed0):
     """`count` shards of seeded uniform 2560-byte entries (device generated)."""
     import torch
@@ -387,6 +487,9 @@ def main():
     if not ok:
         log("[bench] DECRYPTION CHECK FAILED")
 
+    small, small_keep = small_cases(pir, shards[0], entries[0])
+    log(f"[bench] small configurations: {small}")
+
     k1_ms = float(np.mean(k1))
     achieved = bytes_streamed / (k1_ms * 1e-3) / 1e9
     tag = "2p22" if args.entries == 1 << 22 else f"e{args.entries}"
@@ -432,6 +535,7 @@ def main():
             block[5] = 0
         log(f"[bench] CPU baseline on {threads} threads")
         cpu = cpu_baseline(lp, nshards, ls, block, threads)
+        cpu["small_configurations"] = cpu_cases(small_keep, threads)
 
     out = {
         "metric": METRIC,
@@ -473,6 +577,7 @@ def main():
             "bytes_per_launch": bytes_streamed,
         },
         "cpu_baseline": cpu,
+        "small_configurations": small,
         "batched": batch,
         "serving": serve,
     }

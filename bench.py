@@ -288,11 +288,10 @@ def serving(pir, dbs, rates, seconds, n_clients, args):
     """Serving mix: open-loop Poisson arrivals (seeded) at each rate against
     the resident databases `dbs` = [(name, shards, entries, resident, share)],
     n_clients clients with their own keys; keyless queries after the first
-    contact, full queries again after a key-cache eviction."""
+    contact, full queries again after a key-cache eviction. The sweep runs
+    with the batching window `--serve-window`; the top rate is repeated with
+    no window for comparison."""
     from concurrent.futures import ThreadPoolExecutor
-    srv = pir.Server(slots=args.serve_slots, max_batch=args.serve_max_batch,
-                     scratch_budget=args.serve_budget_gib << 30, key_cache_entries=args.serve_key_cache)
-    ids = [srv.add_database(d[1]) for d in dbs]
     shapes = []
     for name, shards, entries, resident, share in dbs:
         lp = shards[0].group_order
@@ -311,17 +310,23 @@ def serving(pir, dbs, rates, seconds, n_clients, args):
 
     with ThreadPoolExecutor(16) as ex:
         clients = list(ex.map(make_client, range(n_clients)))
-    log(f"[bench] serving: {n_clients} clients' keys and queries in {time.perf_counter() - t0:.1f} s; "
-        f"max batch per db {[srv.max_batch(i) for i in ids]}")
+    log(f"[bench] serving: {n_clients} clients' keys and queries in {time.perf_counter() - t0:.1f} s")
     share = np.array([d[4] for d in dbs], float)
     share /= share.sum()
-    rng = np.random.default_rng(2024)
-    # warm the key cache and the slots
-    warm = [(ids[k % len(ids)], c, 1000.0 * k, clients[c][2][k % len(ids)][2], clients[c][0]) for k, c in
-            enumerate(range(min(64, n_clients)))]
-    srv.replay(warm)
-    results, sample_ok, sample_n = {}, True, 0
-    for rate in rates:
+    sample = {"n": 0, "ok": True}
+
+    def new_server(window):
+        srv = pir.Server(slots=args.serve_slots, max_batch=args.serve_max_batch,
+                         scratch_budget=args.serve_budget_gib << 30, key_cache_entries=args.serve_key_cache,
+                         batch_window=window)
+        ids = [srv.add_database(d[1]) for d in dbs]
+        # warm the key cache, the slots and the pass-time estimates
+        warm = [(ids[k % len(ids)], c, 1000.0 * k, clients[c][2][k % len(ids)][2], clients[c][0]) for k, c in
+                enumerate(range(min(64, n_clients)))]
+        srv.replay(warm)
+        return srv, ids
+
+    def run_rate(srv, ids, rate, rng):
         n = max(100, int(rate * seconds))
         t = np.cumsum(rng.exponential(1e6 / rate, n))
         who = rng.integers(0, n_clients, n)
@@ -342,7 +347,8 @@ def serving(pir, dbs, rates, seconds, n_clients, args):
             m = tgt == k
             if m.any():
                 per_db[name] = {"n": int(m.sum()), "p50_ms": round(float(np.percentile(lat[m], 50)), 2),
-                                "p99_ms": round(float(np.percentile(lat[m], 99)), 2)}
+                                "p99_ms": round(float(np.percentile(lat[m], 99)), 2),
+                                "mean_batch": round(float(np.mean([infos[i].batch_size for i in np.nonzero(m)[0]])), 1)}
         # decryption check of a seeded 5% sample
         pick = np.random.default_rng(int(rate)).choice(n, max(1, n // 20), replace=False)
         for k in pick:
@@ -351,36 +357,49 @@ def serving(pir, dbs, rates, seconds, n_clients, args):
             _, _, entries, resident, _ = dbs[d]
             want = entries[si % resident][pi].cpu().numpy().astype(np.uint16)
             got = pir.process_reply(replies[k].tobytes(), clients[who[k]][1])
-            sample_ok &= bool(np.array_equal(got, want))
-            sample_n += 1
-        batches = st1["batches"] - st0["batches"]
-        results[str(rate)] = {
-            "requests": n, "failed": bad, "offered_qps": round(n / (t[-1] / 1e6), 1),
+            sample["ok"] &= bool(np.array_equal(got, want))
+            sample["n"] += 1
+        return {
+            "requests": n, "failed": bad, "offered_qps": round(float(n / (t[-1] / 1e6)), 1),
             "achieved_qps": round(n / span, 1), "wall_s": round(wall, 2),
             "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
             "queue_wait_p50_ms": round(float(np.percentile(wait, 50)), 2),
             "mean_batch": round(float(np.mean([i.batch_size for i in infos])), 1),
-            "batches": int(batches),
+            "batches": int(st1["batches"] - st0["batches"]),
             "keys_resent": int(sum(i.resent for i in infos)),
             "key_cache_hits": int(st1["key_hits"] - st0["key_hits"]),
             "per_db": per_db,
         }
-        log(f"[bench] serving {rate}/s: {results[str(rate)]}")
+
+    srv, ids = new_server(args.serve_window)
+    rng = np.random.default_rng(2024)
+    results = {}
+    for rate in rates:
+        results[str(rate)] = run_rate(srv, ids, rate, rng)
+        log(f"[bench] serving {rate}/s (window {args.serve_window}): {results[str(rate)]}")
     st = srv.stats()
     srv.close()
+    compare = None
+    if args.serve_window > 0 and rates:
+        srv0, ids0 = new_server(0.0)
+        compare = {"rate": max(rates), "batch_window": 0.0,
+                   "result": run_rate(srv0, ids0, max(rates), np.random.default_rng(2024 + 1))}
+        log(f"[bench] serving {max(rates)}/s (no window): {compare['result']}")
+        srv0.close()
     return {
         "workload": ("Poisson arrivals (seeded) against " +
                      ", ".join(f"{name} ({len(sh)} shards of l={sh[0].group_order}, {res} resident)"
                                for name, sh, _, res, _ in dbs) +
                      f"; targets split {'/'.join(str(int(d[4] * 100)) for d in dbs)}; {n_clients} clients "
                      f"with their own keys, device key cache of {args.serve_key_cache} (LRU)"),
-        "server": {"slots": args.serve_slots, "max_batch": args.serve_max_batch,
+        "server": {"slots": args.serve_slots, "max_batch": args.serve_max_batch, "batch_window": args.serve_window,
                    "scratch_budget_gib": args.serve_budget_gib, "key_cache_entries": args.serve_key_cache,
                    "scratch_peak_gib": round(st["scratch_in_use_peak"] / 2**30, 2),
                    "inflight_peak": st["inflight_peak"], "admission_waits": st["admission_waits"],
                    "key_evictions": st["key_evictions"]},
         "rates": results,
-        "decrypt_sample": {"n": sample_n, "ok": sample_ok},
+        "top_rate_without_window": compare,
+        "decrypt_sample": sample,
     }
 
 
@@ -407,6 +426,8 @@ def main():
     ap.add_argument("--serve-max-batch", type=int, default=64)
     ap.add_argument("--serve-budget-gib", type=int, default=48)
     ap.add_argument("--serve-key-cache", type=int, default=768)
+    ap.add_argument("--serve-window", type=float, default=1.0,
+                    help="batching window in units of a database's primary-pass time (0 = none)")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (PMC calibration)")
     args = ap.parse_args()

@@ -46,7 +46,7 @@ class Timing(ctypes.Structure):
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("slots", ctypes.c_int), ("max_batch", ctypes.c_int), ("scratch_budget", ctypes.c_size_t),
-                ("key_cache_entries", ctypes.c_int)]
+                ("key_cache_entries", ctypes.c_int), ("batch_window", ctypes.c_double)]
 
 
 class RequestInfo(ctypes.Structure):

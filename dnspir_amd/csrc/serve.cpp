@@ -64,6 +64,7 @@ struct Database {
     int max_batch = 0;         // largest batch whose scratch fits the budget
     std::vector<size_t> need;  // need[nb]: scratch bytes of a batch of nb
     bool streaming = false;    // a batch is in its primary phase (reading the shards)
+    double pass_us = 0;        // running mean of the primary phase's duration (0 = none yet)
 };
 
 struct KeyEntry {
@@ -157,6 +158,7 @@ struct dnspir_server {
     size_t max_qlen = 0;
     int inflight = 0;
     bool waiting_for_memory = false;
+    double wake_at = 0;  // earliest time a held-back batch becomes due (0 = none)
     dnspir_server_stats st{};
 
     // ---- key cache (caller holds mu) ----
@@ -230,6 +232,18 @@ struct dnspir_server {
             std::vector<std::list<Request*>::iterator> pick;
             for (auto jt = it; jt != pending.end() && (int)pick.size() < d.max_batch; ++jt)
                 if ((*jt)->db == r->db && dispatchable(*jt)) pick.push_back(jt);
+            // batching window: while other batches run, a database whose pass
+            // streams for long lets its oldest request wait up to window x
+            // pass time for more requests (each pass costs the same HBM bytes
+            // whatever its batch size); an idle GPU starts at once
+            if (inflight > 0 && cfg.batch_window > 0 && d.pass_us > 0 && (int)pick.size() < d.max_batch) {
+                const double due = r->t_submit + cfg.batch_window * d.pass_us;
+                if (now_us() < due) {
+                    wake_at = wake_at == 0 ? due : std::min(wake_at, due);
+                    ++it;
+                    continue;
+                }
+            }
             const size_t room = fl.largest();
             int nb = (int)pick.size();
             while (nb > 0 && d.need[nb] > room) --nb;
@@ -269,6 +283,7 @@ struct dnspir_server {
     int run(Slot* sl, Batch& b) {
         Database& d = dbs[b.db];
         const int nb = (int)b.reqs.size();
+        const double t_dispatch = now_us();
         Arena ar;
         ar.base = arena + b.off;
         ar.cap = b.bytes;
@@ -308,6 +323,10 @@ struct dnspir_server {
         {
             std::lock_guard<std::mutex> lk(mu);
             d.streaming = false;
+            if (!rc) {
+                const double dt = now_us() - t_dispatch;
+                d.pass_us = d.pass_us == 0 ? dt : 0.8 * d.pass_us + 0.2 * dt;
+            }
         }
         cv_work.notify_all();
         if (!rc && (hipMemcpyAsync(sl->hr, c->rbytes.p, (size_t)nb * REPLY_BYTES, hipMemcpyDeviceToHost, c->stream) !=
@@ -342,7 +361,17 @@ struct dnspir_server {
         std::unique_lock<std::mutex> lk(mu);
         for (;;) {
             Batch b;
-            while (!stop && !form(b)) cv_work.wait(lk);
+            for (;;) {
+                wake_at = 0;
+                if (stop || form(b)) break;
+                if (wake_at > 0) {
+                    const auto until = std::chrono::steady_clock::time_point(std::chrono::duration_cast<std::chrono::steady_clock::duration>(
+                        std::chrono::duration<double, std::micro>(wake_at)));
+                    cv_work.wait_until(lk, until);
+                } else {
+                    cv_work.wait(lk);
+                }
+            }
             if (stop) break;
             ++inflight;
             st.inflight_peak = std::max(st.inflight_peak, inflight);
@@ -441,6 +470,7 @@ int dnspir_server_create(const dnspir_server_config* cfg, dnspir_server** out) {
     if (!cfg || !out) return DNSPIR_E_ARG;
     if (cfg->slots < 1 || cfg->slots > 8 || cfg->max_batch < 1 || cfg->max_batch > 128 || cfg->key_cache_entries < 0)
         return DNSPIR_E_ARG;
+    if (!(cfg->batch_window >= 0 && cfg->batch_window <= 1000)) return DNSPIR_E_ARG;
     int rc = ensure_init();
     if (rc) return rc;
     std::unique_ptr<dnspir_server> s(new (std::nothrow) dnspir_server);

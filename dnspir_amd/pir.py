@@ -316,9 +316,9 @@ class Server:
     """Continuous-batching request loop over resident databases
     (dnspir_server_*): slots, HBM scratch budget, device Galois-key cache."""
 
-    def __init__(self, slots=3, max_batch=64, scratch_budget=8 << 30, key_cache_entries=256):
+    def __init__(self, slots=3, max_batch=64, scratch_budget=8 << 30, key_cache_entries=256, batch_window=0.0):
         init()
-        cfg = ServerConfig(slots, max_batch, scratch_budget, key_cache_entries)
+        cfg = ServerConfig(slots, max_batch, scratch_budget, key_cache_entries, batch_window)
         h = ctypes.c_void_p()
         check(lib().dnspir_server_create(ctypes.byref(cfg), ctypes.byref(h)), "dnspir_server_create")
         self.h = h

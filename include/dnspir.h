@@ -185,6 +185,9 @@ typedef struct {
     int max_batch;           /* queries per batch (1..128) */
     size_t scratch_budget;   /* device bytes all in-flight batches may hold together */
     int key_cache_entries;   /* clients whose Galois keys stay on the device (0 = none) */
+    double batch_window;     /* while other batches run, a request may wait up to this many times
+                                its database's measured primary-pass time for more requests to
+                                join its batch (0 = start whenever a slot is free) */
 } dnspir_server_config;
 
 typedef struct {

@@ -171,3 +171,29 @@ def test_replay_open_loop_poisson(pir, dbs):
     _say(f"[serve] replay p50 {np.percentile(lat, 50):.2f} ms p99 {np.percentile(lat, 99):.2f} ms, "
          f"resent {resent}, stats {srv.stats()}")
     srv.close()
+
+
+def test_batching_window_keeps_replies_exact(pir, dbs):
+    """With a batching window the server holds a busy database's requests back
+    to fill larger batches; every reply stays the query's own."""
+    srv = pir.Server(slots=3, max_batch=16, scratch_budget=4 << 30, key_cache_entries=16, batch_window=2.0)
+    ids = [srv.add_database(d[0]) for d in dbs]
+    clients = [Client(pir, 400 + c) for c in range(8)]
+    r = np.random.default_rng(9)
+    arr, want = [], []
+    t = 0.0
+    for k in range(48):
+        c = clients[k % 8]
+        d = int(r.choice(3, p=[0.2, 0.3, 0.5]))
+        pi, si, q = c.query(dbs[d], 700 + k)
+        t += float(r.exponential(300.0))  # ~3300 requests/s: the GPU stays busy
+        arr.append((ids[d], c.cid, t, q, c.keys))
+        want.append((c, d, pi, si))
+    replies, infos = srv.replay(arr)
+    for k, (c, d, pi, si) in enumerate(want):
+        assert infos[k].status == 0, k
+        assert np.array_equal(pir.process_reply(replies[k].tobytes(), c.sk), dbs[d][1][si][pi]), k
+    st = srv.stats()
+    _say(f"[serve] window: {st['batches']} batches for 48 requests")
+    assert st["completed"] == 48 and st["batches"] < 48
+    srv.close()

@@ -115,3 +115,5 @@ def test_server_needs_a_device():
     assert _lib.lib().dnspir_server_create(ctypes.byref(cfg), ctypes.byref(h)) == -10
     bad = _lib.ServerConfig(0, 1, 1 << 20, 0)
     assert _lib.lib().dnspir_server_create(ctypes.byref(bad), ctypes.byref(h)) == -5
+    bad = _lib.ServerConfig(1, 1, 1 << 20, 0, -1.0)
+    assert _lib.lib().dnspir_server_create(ctypes.byref(bad), ctypes.byref(h)) in (-5, -10)

@@ -41,7 +41,7 @@ class DnspirError(RuntimeError):
 class Timing(ctypes.Structure):
     _fields_ = [("primary_us", ctypes.c_double), ("secondary_us", ctypes.c_double),
                 ("total_us", ctypes.c_double), ("bytes_streamed", ctypes.c_uint64),
-                ("k1_us", ctypes.c_double), ("k1_launches", ctypes.c_int)]
+                ("k1_us", ctypes.c_double), ("k1_launches", ctypes.c_int), ("ref_bytes", ctypes.c_uint64)]
 
 
 class ServerConfig(ctypes.Structure):

@@ -320,7 +320,7 @@ int run_double(dnspir_ctx* c, const dnspir_shard* const* fleets[2], const size_t
 
     HIP_OK(hipEventRecord(c->ev[0], s));
     launch_qprep(c->qcoef.as<uint64_t>(), c->qs.as<uint32_t>(), c->qp.as<uint32_t>(), 2, g.tables, s);
-    uint64_t streamed = 0;
+    uint64_t streamed = 0, ref_streamed = 0;
     c->k1_launches = 0;
     c->k1_timed = true;
     for (int sl = 0; sl < 2; ++sl) {
@@ -343,12 +343,16 @@ int run_double(dnspir_ctx* c, const dnspir_shard* const* fleets[2], const size_t
         e.lane0_mult = 0;
         launch_epi(e, g.tables, s);
         streamed += (uint64_t)len * slab_bytes(l_p);
+        ref_streamed += (uint64_t)len * ref_slab_bytes(l_p);
     }
     HIP_OK(hipEventRecord(c->ev[1], s));
     if ((rc = run_secondary(c, c->se.as<uint16_t>(), (long long)LANES * N, d_off, d_ncols, d_slot, (int)off.size(), l_s)))
         return rc;
     HIP_OK(hipEventRecord(c->ev[2], s));
-    if (tm) tm->bytes_streamed = streamed;
+    if (tm) {
+        tm->bytes_streamed = streamed;
+        tm->ref_bytes = ref_streamed;
+    }
     return 0;
 }
 
@@ -778,6 +782,7 @@ int dnspir_process_queries(dnspir_ctx* c, const dnspir_shard* const* f0, size_t 
     if (tm) {
         int lp = f0[0]->l;
         tm->bytes_streamed = (uint64_t)((q0 ? len0 : 0) + (q1 ? len1 : 0)) * slab_bytes(lp);
+        tm->ref_bytes = (uint64_t)((q0 ? len0 : 0) + (q1 ? len1 : 0)) * ref_slab_bytes(lp);
     }
     return 0;
 }
@@ -953,7 +958,10 @@ int dne::batch_impl(dnspir_ctx* c, const dnspir_shard* const* fleet, size_t len,
     if (!sync) return 0;
     HIP_OK(hipStreamSynchronize(s));
     fill_timing(c, tm, now_us() - t0);
-    if (tm) tm->bytes_streamed = (uint64_t)((nq + 63) / 64) * len * slab_bytes(l_p);
+    if (tm) {
+        tm->bytes_streamed = (uint64_t)((nq + 63) / 64) * len * slab_bytes(l_p);
+        tm->ref_bytes = (uint64_t)((nq + 63) / 64) * len * ref_slab_bytes(l_p);
+    }
     return 0;
 }
 
@@ -983,7 +991,10 @@ int dnspir_process_batch(dnspir_ctx* c, const dnspir_shard* const* fleet, size_t
     if (rc) return rc;
     HIP_OK(hipMemcpy(replies, c->rbytes.p, (size_t)nq * REPLY_BYTES, hipMemcpyDeviceToHost));
     fill_timing(c, tm, now_us() - t0);
-    if (tm) tm->bytes_streamed = (uint64_t)((nq + 63) / 64) * len * slab_bytes(fleet[0]->l);
+    if (tm) {
+        tm->bytes_streamed = (uint64_t)((nq + 63) / 64) * len * slab_bytes(fleet[0]->l);
+        tm->ref_bytes = (uint64_t)((nq + 63) / 64) * len * ref_slab_bytes(fleet[0]->l);
+    }
     return 0;
 }
 

@@ -21,6 +21,9 @@ constexpr uint32_t P2 = 33550337u;   // 2^25 - 2^12 + 1
 constexpr uint64_t Q = (uint64_t)P1 * P2;
 constexpr uint32_t T_PLAIN = 1025;   // plaintext modulus t
 constexpr int LOG2_B = 26;           // gadget base
+// one block of the shard slab's limb-block layout: 1024 residues as three
+// int8 digit planes (3 x 1024 B) and 64 words of 2-bit top digits
+constexpr int MB_BLOCK_WORDS = 832;
 
 __host__ __device__ constexpr uint32_t mod_p(int r) { return r == 0 ? P1 : P2; }
 __host__ __device__ constexpr int mod_k(int r) { return r == 0 ? 26 : 25; }

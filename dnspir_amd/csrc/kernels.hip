@@ -604,15 +604,33 @@ __global__ __launch_bounds__(256) void relayout_rot_kernel(const uint32_t* __res
     }
 }
 
-// Output-major limb-block layout (L = N/2), read by k1b_kernel. Logically
-// M2[r][half][a][s][j][lane] = row i = (a - j + L - 1) mod N/2 at Galois
-// position (half, j): the closed form of main_pir_loop as a plain matrix,
-// out[a] = sum_(s,j) M2[a][s][j] Q_s[j] (s = stream*3 + array).
-// Stored as 4 KiB blocks, one per (r, half, a-group g of 4, s, j-chunk c of 32):
-// byte v*1024 + l*16 + kk = signed base-256 digit v of x (balanced digits,
-// (x + 0x808080) ^ 0x808080), lane l = h*32 + (a%4)*8 + lane8, j = 32c + 16h + kk:
-// exactly the operand a 64-lane wave feeds to v_mfma_i32_32x32x32_i8.
+// Output-major limb-block layout (L = N/2), read by the k1b/k1d kernels.
+// Logically M2[r][half][a][s][j][lane] = row i = (a - j + L - 1) mod N/2 at
+// Galois position (half, j): the closed form of main_pir_loop as a plain
+// matrix, out[a] = sum_(s,j) M2[a][s][j] Q_s[j] (s = stream*3 + array).
+//
+// Each residue x in [0, p) is stored as 26 bits: the representative
+// x' = x, or x - p when x + 0x808080 >= 2^25, written as balanced base-256
+// digits x' = d0 + 2^8 d1 + 2^16 d2 + 2^24 t with d0..d2 in [-128, 127] and
+// t in {-2, -1, 0, 1} (p < 2^26), and t kept biased as u = t + 2 in 2 bits.
+// So x = d0 + 2^8 d1 + 2^16 d2 + 2^24 u - 2^25 (mod p), and the constant
+// -2^25 sum_k Q[k] of every output is added per query at recombination.
+//
+// Stored as MB_BLOCK_BYTES blocks, one per (r, half, a-group g of 4, s,
+// j-chunk c of 32): byte v*1024 + l*16 + kk = digit d_v (v < 3) of the value
+// at lane l = h*32 + (a%4)*8 + lane8, j = 32c + 16h + kk (exactly the A
+// operand a 64-lane wave feeds to v_mfma_i32_32x32x32_i8), then at byte
+// 3072 + 4l one word holding that lane's 16 top digits u, value kk at bit
+// 8*(kk%4) + 2*(kk/4), so (word >> 2m) & 0x03030303 is the int8 vector of
+// kk = 4m..4m+3.
 __device__ __forceinline__ uint32_t limbs8(uint32_t x) { return (x + 0x00808080u) ^ 0x00808080u; }
+
+// the representative whose top balanced digit lies in [-2, 1], as its 4
+// balanced digits (two's complement bytes)
+__device__ __forceinline__ uint32_t limbs8_top2(uint32_t x, uint32_t p) {
+    x = x >= p ? x - p : x;
+    return limbs8(x + 0x00808080u >= (1u << 25) ? x - p : x);
+}
 
 __global__ __launch_bounds__(256) void relayout_mb_kernel(const uint32_t* __restrict__ rows_tmp, uint32_t* __restrict__ slab,
                                                           int nst) {
@@ -626,28 +644,32 @@ __global__ __launch_bounds__(256) void relayout_mb_kernel(const uint32_t* __rest
     const int a0 = at * 32, j0 = c * 32;
     const int ibase = (a0 - j0 - 31 + (HALFN - 1)) & (HALFN - 1);  // row of (a0, j0 + 31)
     const int tid = threadIdx.x;
+    const uint32_t p = mod_p(r);
     for (int idx = tid; idx < 63 * LANES * 32; idx += 256) {
         const int jl = idx & 31, lane = (idx >> 5) & 7, ri = idx >> 8;
         const int i = (ibase + ri) & (HALFN - 1);
         tile[ri][lane][jl] = rows_tmp[(((size_t)(st * LANES + lane) * HALFN + i) * 6 + arr * 2 + r) * N + half * HALFN + j0 + jl];
     }
     __syncthreads();
-    // 8 blocks (a-groups) x 64 lanes; each item writes the 4 limb chunks of one lane
+    // 8 blocks (a-groups) x 64 lanes; each item writes the 3 digit chunks and
+    // the top-digit word of one lane
     for (int it = tid; it < 8 * 64; it += 256) {
         const int gi = it >> 6, l = it & 63;
         const int h = l >> 5, ai = (l >> 3) & 3, lane8 = l & 7;
         const int a = a0 + gi * 4 + ai;
         uint32_t w[16];
+        uint32_t top = 0;
 #pragma unroll
         for (int kk = 0; kk < 16; ++kk) {
             const int jl = 16 * h + kk;
             // row i = a - j + L-1  ->  tile row (a - a0) - jl + 31
-            w[kk] = limbs8(tile[(a - a0) - jl + 31][lane8][jl]);
+            w[kk] = limbs8_top2(tile[(a - a0) - jl + 31][lane8][jl], p);
+            top |= (((w[kk] >> 24) + 2u) & 3u) << (8 * (kk & 3) + 2 * (kk >> 2));
         }
         const int g = a0 / 4 + gi;
-        uint32_t* blk = slab + ((((size_t)(r * 2 + half) * (HALFN / 4) + g) * S + s) * 32 + c) * 1024;
+        uint32_t* blk = slab + ((((size_t)(r * 2 + half) * (HALFN / 4) + g) * S + s) * 32 + c) * MB_BLOCK_WORDS;
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
+        for (int v = 0; v < 3; ++v) {
             u32x4 o;
             o.x = ((w[0] >> (8 * v)) & 255) | (((w[1] >> (8 * v)) & 255) << 8) | (((w[2] >> (8 * v)) & 255) << 16) | (((w[3] >> (8 * v)) & 255) << 24);
             o.y = ((w[4] >> (8 * v)) & 255) | (((w[5] >> (8 * v)) & 255) << 8) | (((w[6] >> (8 * v)) & 255) << 16) | (((w[7] >> (8 * v)) & 255) << 24);
@@ -655,11 +677,12 @@ __global__ __launch_bounds__(256) void relayout_mb_kernel(const uint32_t* __rest
             o.w = ((w[12] >> (8 * v)) & 255) | (((w[13] >> (8 * v)) & 255) << 8) | (((w[14] >> (8 * v)) & 255) << 16) | (((w[15] >> (8 * v)) & 255) << 24);
             *reinterpret_cast<u32x4*>(blk + v * 256 + l * 4) = o;
         }
+        blk[768 + l] = top;
     }
 }
 
-// Query vectors in Galois order (qp [q][4][r][gpos]) -> B operands of
-// k1b_kernel: ql[r][half][s][c][ct][v][l] (16 bytes each): limb v of
+// Query vectors in Galois order (qp [q][4][r][gpos]) -> B operands of the
+// k1b kernels: ql[r][half][s][c][ct][v][l] (16 bytes each): limb v of
 // Q_b[s][j], b = 32 ct + l % 32, j = 32c + 16 (l / 32) + kk; zero for b >= nq.
 __global__ __launch_bounds__(256) void qlimb_kernel(const uint32_t* __restrict__ qp, uint32_t* __restrict__ ql, int nq, int nst,
                                                     int CT) {
@@ -691,225 +714,118 @@ __global__ __launch_bounds__(256) void qlimb_kernel(const uint32_t* __restrict__
     }
 }
 
+// The per-query constant of the biased top digits: corr[(r*2+half)][b] =
+// -2^25 sum_(s,j) Q_b[s][j] mod p_r over the contraction of (r, half)
+// (zero for b >= nq). Written behind the limb tiles in the ql buffer.
+__global__ __launch_bounds__(256) void qcorr_kernel(const uint32_t* __restrict__ qp, uint32_t* __restrict__ corr, int nq, int nst,
+                                                    int ncols) {
+    __shared__ unsigned long long part[256];
+    const int rh = blockIdx.x, b = blockIdx.y;
+    const int half = rh & 1, r = rh >> 1;
+    const int tid = threadIdx.x;
+    unsigned long long sum = 0;
+    if (b < nq) {
+        for (int s = 0; s < 3 * nst; ++s) {
+            const int st = s / 3, arr = s % 3;
+            const int vq = arr == 0 ? 2 + st : arr - 1;
+            const uint32_t* v = qp + (((size_t)b * 4 + vq) * 2 + r) * N + half * HALFN;
+            for (int j = tid; j < HALFN; j += 256) sum += v[j];
+        }
+    }
+    part[tid] = sum;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w) part[tid] += part[tid + w];
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const uint64_t p = mod_p(r);
+        const uint64_t m = ((1ull << 25) % p) * (part[0] % p) % p;
+        corr[(size_t)rh * ncols + b] = (uint32_t)((p - m) % p);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // K1 (single, split and batched queries) on the limb-block layout: for each
 // (shard, r, half) the product OUT[(a, lane)][b] = sum_(s,j) M2[(a,lane)][(s,j)] Q_b[(s,j)]
-// on int8 MFMA with exact limb decomposition: both operands are split into 4
-// signed base-256 digits (|digit| <= 128), the 16 digit products are summed in
-// int32 per weight w = u + v (|sum| <= 4 * 6144 * 2^14 < 2^29), and
-// OUT = sum_w 2^(8w) ACC_w mod p. One wave per a-group (32 output rows),
-// CT column tiles of 32 queries, A blocks prefetched PD K-steps ahead, B tiles
-// shared through LDS (double buffered). Writes outb[q][shard][r][gpos][lane].
+// on int8 MFMA with exact limb decomposition: the shard operand is split into
+// 3 signed base-256 digits and the biased top digit u in [0, 3], the query
+// into 4 signed digits; the 16 digit products are summed in int32 per weight
+// w = u + v (|sum| <= 4 * 6144 * 2^14 < 2^29), and
+// OUT = sum_w 2^(8w) ACC_w + corr_b mod p. One wave per a-group (32 output
+// rows), CT column tiles of 32 queries. Writes outb[q][shard][r][gpos][lane].
 // ---------------------------------------------------------------------------
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
-template <int CT, int PD>
-__global__ __launch_bounds__(256) void k1b_kernel(const uint32_t* const* __restrict__ slabs, const v4i* __restrict__ ql,
-                                                  uint32_t* __restrict__ outb, int nst, int nq, size_t q_stride) {
-    // B tiles of the next PD-1 K-steps sit in an LDS ring of PD stages; the
-    // tile PD steps ahead travels through registers (loaded one step earlier)
-    constexpr int NS = PD;
-    constexpr int D = NS - 1;
-    __shared__ v4i Bt[NS][CT][4][64];
-    const int S = 3 * nst;
-    const int half = blockIdx.y;
-    const int r = blockIdx.z & 1, sh = blockIdx.z >> 1;
-    const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
-    const int g = blockIdx.x * 4 + wave;
-    const v4i* A = reinterpret_cast<const v4i*>(slabs[sh]) + (((size_t)(r * 2 + half) * (HALFN / 4) + g) * S) * 32 * 256;
-    const v4i* Q = ql + ((size_t)(r * 2 + half) * S) * 32 * CT * 256;
-    const int nk = S * 32;
-    v16i acc[7][CT];
-#pragma unroll
-    for (int w = 0; w < 7; ++w)
-#pragma unroll
-        for (int ct = 0; ct < CT; ++ct)
-#pragma unroll
-            for (int e = 0; e < 16; ++e) acc[w][ct][e] = 0;
-    v4i abuf[PD][4];
-#pragma unroll
-    for (int d = 0; d < PD; ++d)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) abuf[d][v] = __builtin_nontemporal_load(A + (size_t)d * 256 + v * 64 + l);
-#pragma unroll
-    for (int t = 0; t < D; ++t)
-#pragma unroll
-        for (int i = 0; i < CT; ++i) (&Bt[t][0][0][0])[tid + 256 * i] = Q[(size_t)t * CT * 256 + tid + 256 * i];
-    v4i breg[CT];
-#pragma unroll
-    for (int i = 0; i < CT; ++i) breg[i] = Q[(size_t)D * CT * 256 + tid + 256 * i];
-    for (int kk0 = 0; kk0 < nk; kk0 += PD) {
-#pragma unroll
-        for (int d = 0; d < PD; ++d) {
-            const int kk = kk0 + d;
-            __syncthreads();
-            if (kk + D < nk) {
-#pragma unroll
-                for (int i = 0; i < CT; ++i) (&Bt[(d + D) % NS][0][0][0])[tid + 256 * i] = breg[i];
-            }
-            if (kk + D + 1 < nk) {
-#pragma unroll
-                for (int i = 0; i < CT; ++i) breg[i] = Q[(size_t)(kk + D + 1) * CT * 256 + tid + 256 * i];
-            }
-            v4i a[4];
-#pragma unroll
-            for (int v = 0; v < 4; ++v) a[v] = abuf[d][v];
-            if (kk + PD < nk) {
-#pragma unroll
-                for (int v = 0; v < 4; ++v) abuf[d][v] = __builtin_nontemporal_load(A + (size_t)(kk + PD) * 256 + v * 64 + l);
-            }
-#pragma unroll
-            for (int ct = 0; ct < CT; ++ct) {
-                v4i b[4];
-#pragma unroll
-                for (int v = 0; v < 4; ++v) b[v] = Bt[d % NS][ct][v][l];
-#pragma unroll
-                for (int u = 0; u < 4; ++u)
-#pragma unroll
-                    for (int v = 0; v < 4; ++v)
-                        acc[u + v][ct] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[u], b[v], acc[u + v][ct], 0, 0, 0);
-            }
-        }
-    }
-    const uint32_t p = mod_p(r);
-    const int K = mod_k(r);
-    uint32_t cw[7];
-    {
-        uint64_t x = 1;
-#pragma unroll
-        for (int w = 0; w < 7; ++w) {
-            cw[w] = (uint32_t)(x % p);
-            x = (x % p) * 256;
-        }
-    }
-#pragma unroll
-    for (int ct = 0; ct < CT; ++ct) {
-        const int q = ct * 32 + (l & 31);
-        if (q >= nq) continue;
-        uint32_t* o = outb + (size_t)q * q_stride + ((size_t)sh * 2 + r) * N * LANES;
-#pragma unroll
-        for (int e = 0; e < 16; ++e) {
-            int64_t t = 0;
-#pragma unroll
-            for (int w = 0; w < 7; ++w) t += (int64_t)acc[w][ct][e] * cw[w];
-            const int row = 8 * (e >> 2) + 4 * (l >> 5) + (e & 3);
-            const int a = g * 4 + (row >> 3), lane8 = row & 7;
-            o[(half * HALFN + a) * LANES + lane8] = from_i64(t, p, K);
-        }
-    }
-}
-
-// The shard operand through a global-address-space pointer: the loads are
+// The shard operand through global-address-space pointers: the loads are
 // global (not flat) loads, so waiting for them never waits for LDS traffic.
 typedef __attribute__((address_space(1))) const v4i gv4i;
+typedef __attribute__((address_space(1))) const uint32_t gu32;
 
-// k1b with the query tiles loaded PD steps ahead into a register ring, like
-// the shard tiles, and written to a 2-stage LDS buffer one step before use:
-// waiting for a query tile never waits for a younger shard prefetch (vector
-// memory counters retire in order). Loads are unconditional (clamped index).
-template <int CT, int PD>
-__global__ __launch_bounds__(256) void k1b_r(const uint32_t* const* __restrict__ slabs, const v4i* __restrict__ ql,
-                                             uint32_t* __restrict__ outb, int nst, int nq, size_t q_stride) {
-    __shared__ v4i Bt[2][CT][4][64];
-    const int S = 3 * nst;
-    const int half = blockIdx.y;
-    const int r = blockIdx.z & 1, sh = blockIdx.z >> 1;
-    const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
-    const int g = blockIdx.x * 4 + wave;
-    gv4i* A = (gv4i*)(slabs[sh]) + (((size_t)(r * 2 + half) * (HALFN / 4) + g) * S) * 32 * 256 + l;
-    gv4i* Q = (gv4i*)ql + ((size_t)(r * 2 + half) * S) * 32 * CT * 256 + tid;
-    const int nk = S * 32;
-    v16i acc[7][CT];
+constexpr int MB_BLOCK_V4 = MB_BLOCK_WORDS / 4;  // 16-byte units per block
+
+// one K-step of the shard operand for one lane: digit planes 0..2 and the
+// packed top digits
+struct ATile {
+    v4i d[3];
+    uint32_t top;
+};
+
+// A = lane's address in the first block of its (r, half, g) row of blocks
+__device__ __forceinline__ void load_a(ATile& t, const uint32_t* slab, size_t blk0, int l, int k) {
+    gv4i* A = (gv4i*)slab + (blk0 + k) * MB_BLOCK_V4 + l;
+    t.d[0] = __builtin_nontemporal_load(A);
+    t.d[1] = __builtin_nontemporal_load(A + 64);
+    t.d[2] = __builtin_nontemporal_load(A + 128);
+    t.top = __builtin_nontemporal_load((gu32*)slab + (blk0 + k) * MB_BLOCK_WORDS + 768 + l);
+}
+
+// the four int8 vectors of a lane's top digits
+__device__ __forceinline__ v4i top_digits(uint32_t w) {
+    v4i o;
+    o.x = (int)(w & 0x03030303u);
+    o.y = (int)((w >> 2) & 0x03030303u);
+    o.z = (int)((w >> 4) & 0x03030303u);
+    o.w = (int)((w >> 6) & 0x03030303u);
+    return o;
+}
+
+// 2^(8w) mod p for the 7 digit weights
+__device__ __forceinline__ void weight_consts(uint32_t p, uint32_t cw[7]) {
+    uint64_t x = 1;
 #pragma unroll
-    for (int w = 0; w < 7; ++w)
-#pragma unroll
-        for (int ct = 0; ct < CT; ++ct)
-#pragma unroll
-            for (int e = 0; e < 16; ++e) acc[w][ct][e] = 0;
-    v4i abuf[PD][4];
-    v4i bq[PD][CT];
-    // step 0's query tile straight to LDS; steps 1..PD in the register ring
-#pragma unroll
-    for (int i = 0; i < CT; ++i) (&Bt[0][0][0][0])[tid + 256 * i] = Q[256 * i];
-#pragma unroll
-    for (int d = 0; d < PD; ++d) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) abuf[d][v] = __builtin_nontemporal_load(A + (size_t)d * 256 + v * 64);
-        const int kb = min(d + 1, nk - 1);
-#pragma unroll
-        for (int i = 0; i < CT; ++i) bq[d][i] = Q[(size_t)kb * CT * 256 + 256 * i];
-    }
-    for (int kk0 = 0; kk0 < nk; kk0 += PD) {
-#pragma unroll
-        for (int d = 0; d < PD; ++d) {
-            const int kk = kk0 + d;
-            __syncthreads();  // tile kk visible; tile kk-1's buffer free
-            // tile kk+1 (loaded PD steps ago) -> LDS
-#pragma unroll
-            for (int i = 0; i < CT; ++i) (&Bt[(kk + 1) & 1][0][0][0])[tid + 256 * i] = bq[d][i];
-            v4i a[4];
-#pragma unroll
-            for (int v = 0; v < 4; ++v) a[v] = abuf[d][v];
-            v4i b[CT][4];
-#pragma unroll
-            for (int ct = 0; ct < CT; ++ct)
-#pragma unroll
-                for (int v = 0; v < 4; ++v) b[ct][v] = Bt[kk & 1][ct][v][l];
-            const int ka = min(kk + PD, nk - 1), kb = min(kk + PD + 1, nk - 1);
-#pragma unroll
-            for (int v = 0; v < 4; ++v) abuf[d][v] = __builtin_nontemporal_load(A + (size_t)ka * 256 + v * 64);
-#pragma unroll
-            for (int i = 0; i < CT; ++i) bq[d][i] = Q[(size_t)kb * CT * 256 + 256 * i];
-#pragma unroll
-            for (int ct = 0; ct < CT; ++ct)
-#pragma unroll
-                for (int u = 0; u < 4; ++u)
-#pragma unroll
-                    for (int v = 0; v < 4; ++v)
-                        acc[u + v][ct] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[u], b[ct][v], acc[u + v][ct], 0, 0, 0);
-        }
-    }
-    const uint32_t p = mod_p(r);
-    const int K = mod_k(r);
-    uint32_t cw[7];
-    {
-        uint64_t x = 1;
-#pragma unroll
-        for (int w = 0; w < 7; ++w) {
-            cw[w] = (uint32_t)(x % p);
-            x = (x % p) * 256;
-        }
-    }
-#pragma unroll
-    for (int ct = 0; ct < CT; ++ct) {
-        const int q = ct * 32 + (l & 31);
-        if (q >= nq) continue;
-        uint32_t* o = outb + (size_t)q * q_stride + ((size_t)sh * 2 + r) * N * LANES;
-#pragma unroll
-        for (int e = 0; e < 16; ++e) {
-            int64_t t = 0;
-#pragma unroll
-            for (int w = 0; w < 7; ++w) t += (int64_t)acc[w][ct][e] * cw[w];
-            const int row = 8 * (e >> 2) + 4 * (l >> 5) + (e & 3);
-            const int a = g * 4 + (row >> 3), lane8 = row & 7;
-            o[(half * HALFN + a) * LANES + lane8] = from_i64(t, p, K);
-        }
+    for (int w = 0; w < 7; ++w) {
+        cw[w] = (uint32_t)(x % p);
+        x = (x % p) * 256;
     }
 }
 
+// the per-query corrections behind the limb tiles of a ql buffer built for
+// CT column tiles and S = 3*nst arrays
+__device__ __forceinline__ const uint32_t* qcorr_ptr(const void* ql, int S, int CT) {
+    return reinterpret_cast<const uint32_t*>(ql) + (size_t)4 * S * 32 * CT * 1024;
+}
 
-// the same with U steps unrolled per loop iteration (U a multiple of PD)
+// k1b: one wave per a-group, CT column tiles per wave. The shard tiles are
+// prefetched PD K-steps ahead into a register ring; the query tiles are
+// loaded PD steps ahead into a register ring too and written to a 2-stage LDS
+// buffer one step before use, so waiting for a query tile never waits for a
+// younger shard prefetch (vector memory counters retire in order). Loads are
+// unconditional (clamped index). U steps are unrolled per loop iteration
+// (U a multiple of PD; a long body keeps the compiler's wait counts exact).
 template <int CT, int PD, int U>
-__global__ __launch_bounds__(256) void k1b_u(const uint32_t* const* __restrict__ slabs, const v4i* __restrict__ ql,
-                                             uint32_t* __restrict__ outb, int nst, int nq, size_t q_stride) {
+__global__ __launch_bounds__(256) void k1b_kernel(const uint32_t* const* __restrict__ slabs, const v4i* __restrict__ ql,
+                                                  uint32_t* __restrict__ outb, int nst, int nq, size_t q_stride) {
+    static_assert(U % PD == 0, "PD divides U");
     __shared__ v4i Bt[2][CT][4][64];
     const int S = 3 * nst;
     const int half = blockIdx.y;
     const int r = blockIdx.z & 1, sh = blockIdx.z >> 1;
     const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
     const int g = blockIdx.x * 4 + wave;
-    gv4i* A = (gv4i*)(slabs[sh]) + (((size_t)(r * 2 + half) * (HALFN / 4) + g) * S) * 32 * 256 + l;
+    const uint32_t* slab = slabs[sh];
+    const size_t blk0 = (((size_t)(r * 2 + half) * (HALFN / 4) + g) * S) * 32;
     gv4i* Q = (gv4i*)ql + ((size_t)(r * 2 + half) * S) * 32 * CT * 256 + tid;
     const int nk = S * 32;
     v16i acc[7][CT];
@@ -919,20 +835,18 @@ __global__ __launch_bounds__(256) void k1b_u(const uint32_t* const* __restrict__
         for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
             for (int e = 0; e < 16; ++e) acc[w][ct][e] = 0;
-    v4i abuf[PD][4];
+    ATile abuf[PD];
     v4i bq[PD][CT];
     // step 0's query tile straight to LDS; steps 1..PD in the register ring
 #pragma unroll
     for (int i = 0; i < CT; ++i) (&Bt[0][0][0][0])[tid + 256 * i] = Q[256 * i];
 #pragma unroll
     for (int d = 0; d < PD; ++d) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) abuf[d][v] = __builtin_nontemporal_load(A + (size_t)d * 256 + v * 64);
+        load_a(abuf[d], slab, blk0, l, d);
         const int kb = min(d + 1, nk - 1);
 #pragma unroll
         for (int i = 0; i < CT; ++i) bq[d][i] = Q[(size_t)kb * CT * 256 + 256 * i];
     }
-    static_assert(U % PD == 0, "PD divides U");
     for (int kk0 = 0; kk0 < nk; kk0 += U) {
 #pragma unroll
         for (int dd = 0; dd < U; ++dd) {
@@ -943,16 +857,17 @@ __global__ __launch_bounds__(256) void k1b_u(const uint32_t* const* __restrict__
 #pragma unroll
             for (int i = 0; i < CT; ++i) (&Bt[(kk + 1) & 1][0][0][0])[tid + 256 * i] = bq[d][i];
             v4i a[4];
-#pragma unroll
-            for (int v = 0; v < 4; ++v) a[v] = abuf[d][v];
+            a[0] = abuf[d].d[0];
+            a[1] = abuf[d].d[1];
+            a[2] = abuf[d].d[2];
+            a[3] = top_digits(abuf[d].top);
             v4i b[CT][4];
 #pragma unroll
             for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) b[ct][v] = Bt[kk & 1][ct][v][l];
             const int ka = min(kk + PD, nk - 1), kb = min(kk + PD + 1, nk - 1);
-#pragma unroll
-            for (int v = 0; v < 4; ++v) abuf[d][v] = __builtin_nontemporal_load(A + (size_t)ka * 256 + v * 64);
+            load_a(abuf[d], slab, blk0, l, ka);
 #pragma unroll
             for (int i = 0; i < CT; ++i) bq[d][i] = Q[(size_t)kb * CT * 256 + 256 * i];
 #pragma unroll
@@ -967,112 +882,23 @@ __global__ __launch_bounds__(256) void k1b_u(const uint32_t* const* __restrict__
     const uint32_t p = mod_p(r);
     const int K = mod_k(r);
     uint32_t cw[7];
-    {
-        uint64_t x = 1;
-#pragma unroll
-        for (int w = 0; w < 7; ++w) {
-            cw[w] = (uint32_t)(x % p);
-            x = (x % p) * 256;
-        }
-    }
+    weight_consts(p, cw);
+    const uint32_t* corr = qcorr_ptr(ql, S, CT) + (size_t)(r * 2 + half) * CT * 32;
 #pragma unroll
     for (int ct = 0; ct < CT; ++ct) {
         const int q = ct * 32 + (l & 31);
         if (q >= nq) continue;
         uint32_t* o = outb + (size_t)q * q_stride + ((size_t)sh * 2 + r) * N * LANES;
+        const int64_t cq = corr[q];
 #pragma unroll
         for (int e = 0; e < 16; ++e) {
-            int64_t t = 0;
+            int64_t t = cq;
 #pragma unroll
             for (int w = 0; w < 7; ++w) t += (int64_t)acc[w][ct][e] * cw[w];
             const int row = 8 * (e >> 2) + 4 * (l >> 5) + (e & 3);
             const int a = g * 4 + (row >> 3), lane8 = row & 7;
             o[(half * HALFN + a) * LANES + lane8] = from_i64(t, p, K);
         }
-    }
-}
-
-
-// Two column tiles on separate waves: a block of 8 waves covers 4 output
-// groups x 2 tiles of 32 queries, so each wave holds 7 accumulators (two
-// waves per SIMD fit) and the two waves of a group read the same shard
-// blocks at about the same time (the second read hits L1/L2; with NT the
-// loads are non-temporal). Query tiles as in k1b_r; U steps per iteration.
-template <int PD, int U, bool NT>
-__global__ __launch_bounds__(512) void k1b_pair(const uint32_t* const* __restrict__ slabs, const v4i* __restrict__ ql,
-                                                uint32_t* __restrict__ outb, int nst, int nq, size_t q_stride) {
-    constexpr int CT = 2;
-    static_assert(U % PD == 0, "PD divides U");
-    __shared__ v4i Bt[2][CT][4][64];
-    const int S = 3 * nst;
-    const int half = blockIdx.y;
-    const int r = blockIdx.z & 1, sh = blockIdx.z >> 1;
-    const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
-    const int ct = wave >> 2;
-    const int g = blockIdx.x * 4 + (wave & 3);
-    gv4i* A = (gv4i*)(slabs[sh]) + (((size_t)(r * 2 + half) * (HALFN / 4) + g) * S) * 32 * 256 + l;
-    gv4i* Q = (gv4i*)ql + ((size_t)(r * 2 + half) * S) * 32 * CT * 256 + tid;
-    const int nk = S * 32;
-    v16i acc[7];
-#pragma unroll
-    for (int w = 0; w < 7; ++w)
-#pragma unroll
-        for (int e = 0; e < 16; ++e) acc[w][e] = 0;
-    v4i abuf[PD][4];
-    v4i bq[PD];
-    (&Bt[0][0][0][0])[tid] = Q[0];
-#pragma unroll
-    for (int d = 0; d < PD; ++d) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) abuf[d][v] = NT ? __builtin_nontemporal_load(A + (size_t)d * 256 + v * 64) : A[(size_t)d * 256 + v * 64];
-        bq[d] = Q[(size_t)min(d + 1, nk - 1) * CT * 256];
-    }
-    for (int kk0 = 0; kk0 < nk; kk0 += U) {
-#pragma unroll
-        for (int dd = 0; dd < U; ++dd) {
-            const int d = dd % PD;
-            const int kk = kk0 + dd;
-            __syncthreads();  // tile kk visible; tile kk-1's buffer free
-            (&Bt[(kk + 1) & 1][0][0][0])[tid] = bq[d];
-            v4i a[4], b[4];
-#pragma unroll
-            for (int v = 0; v < 4; ++v) a[v] = abuf[d][v];
-#pragma unroll
-            for (int v = 0; v < 4; ++v) b[v] = Bt[kk & 1][ct][v][l];
-            const int ka = min(kk + PD, nk - 1), kb = min(kk + PD + 1, nk - 1);
-#pragma unroll
-            for (int v = 0; v < 4; ++v)
-                abuf[d][v] = NT ? __builtin_nontemporal_load(A + (size_t)ka * 256 + v * 64) : A[(size_t)ka * 256 + v * 64];
-            bq[d] = Q[(size_t)kb * CT * 256];
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-#pragma unroll
-                for (int v = 0; v < 4; ++v)
-                    acc[u + v] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[u], b[v], acc[u + v], 0, 0, 0);
-        }
-    }
-    const uint32_t p = mod_p(r);
-    const int K = mod_k(r);
-    uint32_t cw[7];
-    {
-        uint64_t x = 1;
-#pragma unroll
-        for (int w = 0; w < 7; ++w) {
-            cw[w] = (uint32_t)(x % p);
-            x = (x % p) * 256;
-        }
-    }
-    const int q = ct * 32 + (l & 31);
-    if (q >= nq) return;
-    uint32_t* o = outb + (size_t)q * q_stride + ((size_t)sh * 2 + r) * N * LANES;
-#pragma unroll
-    for (int e = 0; e < 16; ++e) {
-        int64_t t = 0;
-#pragma unroll
-        for (int w = 0; w < 7; ++w) t += (int64_t)acc[w][e] * cw[w];
-        const int row = 8 * (e >> 2) + 4 * (l >> 5) + (e & 3);
-        const int a = g * 4 + (row >> 3), lane8 = row & 7;
-        o[(half * HALFN + a) * LANES + lane8] = from_i64(t, p, K);
     }
 }
 
@@ -1086,17 +912,15 @@ struct K1bVariant {
     int threads = 256;
 };
 static const K1bVariant k_k1b_variants[] = {
-    {"lds ring, prefetch 4", 1, k1b_kernel<1, 4>}, {"register ring, prefetch 2", 1, k1b_r<1, 2>},
-    {"register ring, prefetch 3", 1, k1b_r<1, 3>}, {"register ring, prefetch 4", 1, k1b_r<1, 4>},
-    {"register ring, prefetch 4, unroll 32", 1, k1b_u<1, 4, 32>},
-    {"lds ring, prefetch 4", 2, k1b_kernel<2, 4>}, {"lds ring, prefetch 8", 2, k1b_kernel<2, 8>},
-    {"register ring, prefetch 2", 2, k1b_r<2, 2>}, {"register ring, prefetch 3", 2, k1b_r<2, 3>},
-    {"register ring, prefetch 4", 2, k1b_r<2, 4>}, {"register ring, prefetch 4, unroll 8", 2, k1b_u<2, 4, 8>},
-    {"register ring, prefetch 4, unroll 32", 2, k1b_u<2, 4, 32>},
-    {"tile per wave, prefetch 4", 2, k1b_pair<4, 4, false>, 512},
-    {"tile per wave, prefetch 3, unroll 6", 2, k1b_pair<3, 6, false>, 512},
-    {"tile per wave, prefetch 2, unroll 4", 2, k1b_pair<2, 4, false>, 512},
-    {"tile per wave, non-temporal, prefetch 4", 2, k1b_pair<4, 4, true>, 512},
+    {"prefetch 2", 1, k1b_kernel<1, 2, 2>},
+    {"prefetch 3", 1, k1b_kernel<1, 3, 3>},
+    {"prefetch 4", 1, k1b_kernel<1, 4, 4>},
+    {"prefetch 4, unroll 32", 1, k1b_kernel<1, 4, 32>},
+    {"prefetch 2", 2, k1b_kernel<2, 2, 2>},
+    {"prefetch 3", 2, k1b_kernel<2, 3, 3>},
+    {"prefetch 4", 2, k1b_kernel<2, 4, 4>},
+    {"prefetch 4, unroll 8", 2, k1b_kernel<2, 4, 8>},
+    {"prefetch 4, unroll 32", 2, k1b_kernel<2, 4, 32>},
 };
 constexpr int K1B_NVAR = sizeof(k_k1b_variants) / sizeof(k_k1b_variants[0]);
 // chosen variant per (column tiles, streams); -1 = default (register ring, prefetch 4)
@@ -1112,7 +936,7 @@ int k1b_get_choice(int ct, int nst) {
     int i = (ct >= 1 && ct <= 2 && nst >= 1 && nst <= 2) ? g_k1b_choice[ct][nst].load() : -1;
     if (i >= 0 && i < K1B_NVAR && k_k1b_variants[i].ct == ct) return i;
     for (int k = 0; k < K1B_NVAR; ++k)
-        if (k_k1b_variants[k].ct == ct && k_k1b_variants[k].fn == (ct == 1 ? (K1bFn)k1b_r<1, 4> : (K1bFn)k1b_r<2, 4>)) return k;
+        if (k_k1b_variants[k].ct == ct && k_k1b_variants[k].fn == (ct == 1 ? (K1bFn)k1b_kernel<1, 4, 4> : (K1bFn)k1b_kernel<2, 4, 4>)) return k;
     return 0;
 }
 
@@ -1417,29 +1241,27 @@ __global__ __launch_bounds__(256) void k1d_kernel(const uint32_t* const* __restr
     }
     __syncthreads();
     const int g = blockIdx.x * 4 + wave;
-    const u32x4* A = reinterpret_cast<const u32x4*>(slabs[sh]) + (((size_t)(r * 2 + half) * (HALFN / 4) + g) * S) * 32 * 256;
+    const uint32_t* slab = slabs[sh];
+    const size_t blk0 = (((size_t)(r * 2 + half) * (HALFN / 4) + g) * S) * 32;
     const int nk = S * 32;
     int acc[NQ][7];
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
 #pragma unroll
         for (int w = 0; w < 7; ++w) acc[q][w] = 0;
-    u32x4 abuf[PD][4];
+    ATile abuf[PD];
 #pragma unroll
-    for (int d = 0; d < PD; ++d)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) abuf[d][v] = __builtin_nontemporal_load(A + (size_t)d * 256 + v * 64 + l);
+    for (int d = 0; d < PD; ++d) load_a(abuf[d], slab, blk0, l, d);
     for (int kk0 = 0; kk0 < nk; kk0 += PD) {
 #pragma unroll
         for (int d = 0; d < PD; ++d) {
             const int kk = kk0 + d;
-            u32x4 a[4];
-#pragma unroll
-            for (int v = 0; v < 4; ++v) a[v] = abuf[d][v];
-            if (kk + PD < nk) {
-#pragma unroll
-                for (int v = 0; v < 4; ++v) abuf[d][v] = __builtin_nontemporal_load(A + (size_t)(kk + PD) * 256 + v * 64 + l);
-            }
+            v4i a[4];
+            a[0] = abuf[d].d[0];
+            a[1] = abuf[d].d[1];
+            a[2] = abuf[d].d[2];
+            a[3] = top_digits(abuf[d].top);
+            if (kk + PD < nk) load_a(abuf[d], slab, blk0, l, kk + PD);
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
                 u32x4 b[4];
@@ -1450,10 +1272,10 @@ __global__ __launch_bounds__(256) void k1d_kernel(const uint32_t* const* __restr
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
                         int t = acc[q][u + v];
-                        t = __builtin_amdgcn_sdot4((int)a[u].x, (int)b[v].x, t, false);
-                        t = __builtin_amdgcn_sdot4((int)a[u].y, (int)b[v].y, t, false);
-                        t = __builtin_amdgcn_sdot4((int)a[u].z, (int)b[v].z, t, false);
-                        t = __builtin_amdgcn_sdot4((int)a[u].w, (int)b[v].w, t, false);
+                        t = __builtin_amdgcn_sdot4(a[u].x, (int)b[v].x, t, false);
+                        t = __builtin_amdgcn_sdot4(a[u].y, (int)b[v].y, t, false);
+                        t = __builtin_amdgcn_sdot4(a[u].z, (int)b[v].z, t, false);
+                        t = __builtin_amdgcn_sdot4(a[u].w, (int)b[v].w, t, false);
                         acc[q][u + v] = t;
                     }
             }
@@ -1462,19 +1284,13 @@ __global__ __launch_bounds__(256) void k1d_kernel(const uint32_t* const* __restr
     const uint32_t p = mod_p(r);
     const int K = mod_k(r);
     uint32_t cw[7];
-    {
-        uint64_t x = 1;
-#pragma unroll
-        for (int w = 0; w < 7; ++w) {
-            cw[w] = (uint32_t)(x % p);
-            x = (x % p) * 256;
-        }
-    }
+    weight_consts(p, cw);
+    const uint32_t* corr = qcorr_ptr(ql, S, CT) + (size_t)(r * 2 + half) * CT * 32;
     const int row = l & 31;
     const int a = g * 4 + (row >> 3), lane8 = row & 7;
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-        int64_t t = 0;
+        int64_t t = corr[q];
 #pragma unroll
         for (int w = 0; w < 7; ++w) {
             const int tot = acc[q][w] + __shfl_xor(acc[q][w], 32);
@@ -1493,7 +1309,8 @@ void launch_relayout(const uint32_t* rows_tmp, uint32_t* slab, int L, int nst, h
 
 size_t qlimb_bytes(int nq, int nst) {
     const int CT = nq > 32 ? 2 : 1;
-    return (size_t)2 * 2 * 3 * nst * 32 * CT * 4 * 1024;
+    // limb tiles, then the per-query corrections [r*2+half][CT*32]
+    return (size_t)2 * 2 * 3 * nst * 32 * CT * 4 * 1024 + (size_t)4 * CT * 32 * 4;
 }
 
 void launch_k1b(const uint32_t* const* slabs, int nshards, const uint32_t* qp, int nq, uint32_t* ql, uint32_t* outb, int nst,
@@ -1501,6 +1318,7 @@ void launch_k1b(const uint32_t* const* slabs, int nshards, const uint32_t* qp, i
     const int CT = nq > 32 ? 2 : 1;
     const int S = 3 * nst;
     hipLaunchKernelGGL(qlimb_kernel, dim3(32, 4 * S), dim3(128), 0, s, qp, ql, nq, nst, CT);
+    hipLaunchKernelGGL(qcorr_kernel, dim3(4, CT * 32), dim3(256), 0, s, qp, ql + (size_t)4 * S * 32 * CT * 1024, nq, nst, CT * 32);
     if (main_start) (void)hipEventRecord(main_start, s);
     const size_t qstride = (size_t)nshards * 2 * N * LANES;
     if (nq == 1)

@@ -46,9 +46,12 @@ typedef struct {
     double primary_us;       /* device time of the primary phase */
     double secondary_us;     /* device time of the secondary phase */
     double total_us;         /* wall time of the call */
-    uint64_t bytes_streamed; /* bytes read by the primary inner product */
+    uint64_t bytes_streamed; /* bytes read from the shard slabs by the primary inner product */
     double k1_us;            /* device time of the primary inner-product launches */
     int k1_launches;         /* number of those launches */
+    uint64_t ref_bytes;      /* the same pass in the reference's accounting, 6*N*l*32 B per
+                              * shard (MEM_LOAD_DATA, base_pir.rs:1242); the slabs here keep
+                              * 26 bits per residue, so bytes_streamed is 13/16 of it */
 } dnspir_timing;
 
 /* Initialise the library on HIP device `device` (builds the NTT / Galois
@@ -61,8 +64,10 @@ const char* dnspir_last_error(void);
 /* ---- one shard: PIRDatabase (base_pir.rs:225-1247) ------------------------ */
 
 /* Bytes of caller-owned device memory a shard of index-group order
- * `group_order` needs: PIRDatabase::required_memory_general * 32 B
- * (base_pir.rs:369-371) plus 192 KiB for the cached a-part. */
+ * `group_order` needs (replaces PIRDatabase::required_memory_general * 32 B,
+ * base_pir.rs:369-371): the slab, which keeps 26 bits per residue for
+ * group orders N/2 and N (13/16 of the reference's size) and 32 bits below,
+ * plus 192 KiB for the cached a-part. */
 size_t dnspir_required_bytes(int group_order);
 /* PIRDatabase::create (base_pir.rs:280-349). used_seeds: 0 = FirstSet,
  * 1 = SecondSet. dev_mem must be 256-byte aligned device memory of at least

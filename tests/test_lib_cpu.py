@@ -37,8 +37,10 @@ def test_shape_rule_matches_oracle(n):
 
 def test_reply_bytes_and_required_bytes():
     assert pir.reply_bytes() == 22272
-    assert _lib.lib().dnspir_required_bytes(2048) == 805306368 + 196608
-    assert _lib.lib().dnspir_required_bytes(1024) == 402653184 + 196608
+    # 26 bits per residue: 13/16 of the reference's 6*N*l*32 B slab
+    assert _lib.lib().dnspir_required_bytes(2048) == 805306368 * 13 // 16 + 196608
+    assert _lib.lib().dnspir_required_bytes(1024) == 402653184 * 13 // 16 + 196608
+    assert _lib.lib().dnspir_required_bytes(512) == 6 * 2048 * 512 * 32 + 196608
     assert _lib.lib().dnspir_required_bytes(1536) == 0  # not a divisor of N
     assert _lib.lib().dnspir_required_bytes(2048 * 3 // 4) == 0
 

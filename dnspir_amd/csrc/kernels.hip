@@ -71,33 +71,6 @@ __device__ void ntt_inv(uint32_t* a, const DevTables* __restrict__ T, int r, int
     __syncthreads();
 }
 
-// `nd` interleaved cyclic DFTs of length n = 2^logn (element j of DFT d at
-// a[j*nd + d]) with root omega = psi_1^(-2N/n), modulus p1, decimation in
-// frequency: natural order in, bit-reversed order out.
-__device__ void dft_dif_p1(uint32_t* a, int logn, int nd, const DevTables* __restrict__ T, int tid, int nt) {
-    const uint32_t p = P1;
-    const int n = 1 << logn;
-#pragma unroll 1
-    for (int lh = logn - 1; lh >= 0; --lh) {
-        const int half = 1 << lh;
-        const int tw_step = M2N >> (lh + 1);  // 2N / len
-        const int work = nd * (n / 2);
-        for (int w = tid; w < work; w += nt) {
-            const int d = w % nd;
-            const int b = w / nd;
-            const int blk = b >> lh;
-            const int j = b & (half - 1);
-            const int i0 = (blk << (lh + 1)) + j;
-            const uint32_t u = a[i0 * nd + d];
-            const uint32_t v = a[(i0 + half) * nd + d];
-            const int e = (M2N - ((tw_step * j) & (M2N - 1))) & (M2N - 1);
-            a[i0 * nd + d] = add_mod(u, v, p);
-            a[(i0 + half) * nd + d] = mul_shoup(sub_mod(u, v, p), T->psipow[0][e], T->psipowq[0][e], p);
-        }
-        __syncthreads();
-    }
-}
-
 __device__ __forceinline__ int brv_bits(int v, int bits) {
     return bits == 0 ? 0 : (int)(__brev((unsigned)v) >> (32 - bits));
 }
@@ -234,7 +207,7 @@ __device__ __forceinline__ void inv_pass4(uint32_t* buf, const uint32_t* tw, con
 // inputs in [0, 2p), outputs in [0, p) (forward) / [0, 2p) (inverse, which
 // leaves the N factor in)
 template <bool PAD = false>
-__device__ void ntt256_fwd(uint32_t* buf, const uint32_t* tw, const uint32_t* twq, uint32_t p, int g) {
+__device__ __forceinline__ void ntt256_fwd(uint32_t* buf, const uint32_t* tw, const uint32_t* twq, uint32_t p, int g) {
 #pragma unroll
     for (int P = 0; P < 3; ++P) {
         fwd_pass8<PAD>(buf, tw, twq, p, P, g);
@@ -250,7 +223,7 @@ __device__ void ntt256_fwd(uint32_t* buf, const uint32_t* tw, const uint32_t* tw
     __syncthreads();
 }
 template <bool PAD = false>
-__device__ void ntt256_inv_unscaled(uint32_t* buf, const uint32_t* tw, const uint32_t* twq, uint32_t p, int g) {
+__device__ __forceinline__ void ntt256_inv_unscaled(uint32_t* buf, const uint32_t* tw, const uint32_t* twq, uint32_t p, int g) {
     inv_pass4<PAD>(buf, tw, twq, p, g);
     __syncthreads();
 #pragma unroll
@@ -290,9 +263,10 @@ __global__ __launch_bounds__(256) void k5_col_ntt_kernel(const uint16_t* __restr
 // half), z = e_s * k mod 2N.
 // Stages top, top-1, ..., top-R+1 of the DIF transform over the l rows of a
 // tile [l][TS] (TS interleaved columns), in registers per group of 2^R
-// elements; lazy values in [0, 2p), Shoup products by psi^-(2N/2^(lh+1) j).
+// elements; lazy values in [0, 2p), Shoup products by psi^-(2N/2^(lh+1) j),
+// read from the LDS table dtw[2^lh + j] (companions in dtwq).
 template <int TS, int R>
-__device__ __forceinline__ void k5_dif_pass(uint32_t* tile, int logl, int top, const DevTables* __restrict__ T, int tid) {
+__device__ __forceinline__ void k5_dif_pass(uint32_t* tile, int logl, int top, const uint32_t* dtw, const uint32_t* dtwq, int tid) {
     constexpr int G = 1 << R;
     const int sh = top - R + 1;  // group stride 2^sh in rows
     const int ngroups = (1 << logl) / G * TS;
@@ -306,13 +280,11 @@ __device__ __forceinline__ void k5_dif_pass(uint32_t* tile, int logl, int top, c
 #pragma unroll
         for (int t = 0; t < R; ++t) {
             const int lh = top - t, hk = G >> (t + 1);
-            const int tws = M2N >> (lh + 1);
 #pragma unroll
             for (int k = 0; k < G; ++k) {
                 if (k & hk) continue;
-                const int j = (base + (k << sh)) & ((1 << lh) - 1);
-                const int e = (M2N - ((tws * j) & (M2N - 1))) & (M2N - 1);
-                bf_gs(v[k], v[k + hk], T->psipow[0][e], T->psipowq[0][e], P1);
+                const int ti = (1 << lh) + ((base + (k << sh)) & ((1 << lh) - 1));
+                bf_gs(v[k], v[k + hk], dtw[ti], dtwq[ti], P1);
             }
         }
 #pragma unroll
@@ -323,11 +295,18 @@ __device__ __forceinline__ void k5_dif_pass(uint32_t* tile, int logl, int top, c
 template <int TS>
 __global__ __launch_bounds__(256) void k5_rows_kernel(uint32_t* __restrict__ M, int l, int logl, int kq, int conj,
                                                       const DevTables* __restrict__ T) {
-    extern __shared__ uint32_t tile[];  // [l][TS]
+    extern __shared__ uint32_t tile[];  // [l][TS], l * TS <= 16384
+    __shared__ uint32_t dtw[16384 / TS], dtwq[16384 / TS];  // DIF twiddles, l entries
     const int s0 = blockIdx.x * TS, li = blockIdx.y;
     uint32_t* base = M + (size_t)li * l * N;
     const int tid = threadIdx.x;
     const int lstep = LOGN + 1 - logl;  // 2N / l = 2^lstep
+    for (int t = tid + 1; t < l; t += 256) {  // stage lh, position j: t = 2^lh + j
+        const int lh = 31 - __clz(t), j = t - (1 << lh);
+        const int e = (M2N - (((M2N >> (lh + 1)) * j) & (M2N - 1))) & (M2N - 1);
+        dtw[t] = T->psipow[0][e];
+        dtwq[t] = T->psipowq[0][e];
+    }
     for (int idx = tid; idx < l * TS; idx += 256) {
         const int j = idx / TS, d = idx % TS;
         const int s = s0 + d;
@@ -339,12 +318,12 @@ __global__ __launch_bounds__(256) void k5_rows_kernel(uint32_t* __restrict__ M, 
     __syncthreads();
     int top = logl - 1;
     while (top >= 2) {
-        k5_dif_pass<TS, 3>(tile, logl, top, T, tid);
+        k5_dif_pass<TS, 3>(tile, logl, top, dtw, dtwq, tid);
         __syncthreads();
         top -= 3;
     }
-    if (top == 1) k5_dif_pass<TS, 2>(tile, logl, top, T, tid);
-    else if (top == 0) k5_dif_pass<TS, 1>(tile, logl, top, T, tid);
+    if (top == 1) k5_dif_pass<TS, 2>(tile, logl, top, dtw, dtwq, tid);
+    else if (top == 0) k5_dif_pass<TS, 1>(tile, logl, top, dtw, dtwq, tid);
     if (top >= 0) __syncthreads();
     for (int idx = tid; idx < l * TS; idx += 256) {
         const int i = idx / TS, d = idx % TS;
@@ -386,7 +365,7 @@ __global__ __launch_bounds__(256) void k5_cols_final_kernel(const uint32_t* __re
         if (x > (int32_t)(P1 / 2)) x -= (int32_t)P1;
         int32_t xt = x % (int32_t)T_PLAIN;
         if (xt < 0) xt += T_PLAIN;
-        int32_t y = (int32_t)(((int64_t)xt * scale) % T_PLAIN);
+        int32_t y = (int32_t)(((uint32_t)xt * (uint32_t)scale) % T_PLAIN);  // < 1025^2
         if (y > (int32_t)(T_PLAIN / 2)) y -= (int32_t)T_PLAIN;
         a[0][xs] = y < 0 ? (uint32_t)(y + (int32_t)P1) : (uint32_t)y;
         a[1][xs] = y < 0 ? (uint32_t)(y + (int32_t)P2) : (uint32_t)y;

@@ -162,6 +162,8 @@ int dne::ctx_init(dnspir_ctx* c, hipStream_t s) {
         c->own = true;
     }
     for (auto& e : c->ev) HIP_OK(hipEventCreate(&e));
+    HIP_OK(hipHostMalloc((void**)&c->h_err, sizeof(unsigned), hipHostMallocDefault));
+    *c->h_err = 0;
     return 0;
 }
 
@@ -268,8 +270,21 @@ int run_primary(dnspir_ctx* c, const uint32_t* const* slabs, int len, int l_p, c
 
 // Secondary phase for `nlanes` lanes whose entries are described by the lane
 // lists; the reply (a, b per lane) lands in c->acoef/bcoef/a17o/b12o.
+// after a sync: the exchange flag of the last two-CU chain launch
+int check_pair(dnspir_ctx* c) {
+    if (!c->pair_pending) return 0;
+    c->pair_pending = false;
+    if (*c->h_err) {
+        snprintf(g_last_error, sizeof(g_last_error), "secondary chain: partner exchange timed out");
+        return DNSPIR_E_HIP;
+    }
+    return 0;
+}
+
+// pair: run the chains on two CUs each (chain_pair_kernel, lower latency for a
+// few chains); the caller checks the exchange flag after its sync (check_pair)
 int run_secondary(dnspir_ctx* c, const uint16_t* entries, long long col_stride, const long long* d_off, const int* d_ncols,
-                  const int* d_slot, int nlanes, int l_s) {
+                  const int* d_slot, int nlanes, int l_s, bool pair = false) {
     const int L_s = loop_len(l_s), nst_s = nstreams(l_s);
     const hipStream_t s = c->stream;
     int rc;
@@ -282,8 +297,18 @@ int run_secondary(dnspir_ctx* c, const uint16_t* entries, long long col_stride, 
     if ((rc = c->a17o.ensure((size_t)nlanes * N * 4))) return rc;
     if ((rc = c->b12o.ensure((size_t)nlanes * N * 4))) return rc;
     launch_k5(entries, d_off, col_stride, c->M.as<uint32_t>(), c->te.as<uint32_t>(), l_s, d_ncols, nlanes, g.tables, s);
-    launch_chain_sec(c->te.as<uint32_t>(), c->aout.as<uint32_t>(), c->cout.as<uint32_t>(), c->qs.as<uint32_t>(), d_slot, L_s,
-                     nst_s, nlanes, g.tables, s);
+    if (pair) {
+        const size_t words = chain_pair_xbuf_words(nst_s * nlanes);
+        if ((rc = c->xch.ensure((words + 64) * 4))) return rc;
+        unsigned* err = reinterpret_cast<unsigned*>(c->xch.as<uint32_t>() + words);
+        launch_chain_pair(c->te.as<uint32_t>(), c->aout.as<uint32_t>(), c->cout.as<uint32_t>(), c->qs.as<uint32_t>(), d_slot,
+                          L_s, nst_s, nlanes, c->xch.as<uint32_t>(), err, g.tables, s);
+        HIP_OK(hipMemcpyAsync(c->h_err, err, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+        c->pair_pending = true;
+    } else {
+        launch_chain_sec(c->te.as<uint32_t>(), c->aout.as<uint32_t>(), c->cout.as<uint32_t>(), c->qs.as<uint32_t>(), d_slot,
+                         L_s, nst_s, nlanes, g.tables, s);
+    }
     launch_chain_out(c->aout.as<uint32_t>(), c->cout.as<uint32_t>(), nst_s, nlanes, c->acoef.as<uint64_t>(),
                      c->bcoef.as<uint64_t>(), c->a17o.as<uint32_t>(), c->b12o.as<uint32_t>(), g.tables, s);
     return 0;
@@ -346,7 +371,8 @@ int run_double(dnspir_ctx* c, const dnspir_shard* const* fleets[2], const size_t
         ref_streamed += (uint64_t)len * ref_slab_bytes(l_p);
     }
     HIP_OK(hipEventRecord(c->ev[1], s));
-    if ((rc = run_secondary(c, c->se.as<uint16_t>(), (long long)LANES * N, d_off, d_ncols, d_slot, (int)off.size(), l_s)))
+    if ((rc = run_secondary(c, c->se.as<uint16_t>(), (long long)LANES * N, d_off, d_ncols, d_slot, (int)off.size(), l_s,
+                            true)))
         return rc;
     HIP_OK(hipEventRecord(c->ev[2], s));
     if (tm) {
@@ -679,6 +705,7 @@ int dnspir_double_pir(dnspir_ctx* c, const dnspir_shard* const* f0, size_t len0,
     HIP_OK(hipMemcpyAsync(a.data(), c->acoef.p, a.size() * 8, hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipMemcpyAsync(b.data(), c->bcoef.p, b.size() * 8, hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
+    if ((rc = check_pair(c))) return rc;
     uint64_t* outs[2] = {out0, out1};
     int li = 0;
     for (int sl = 0; sl < 2; ++sl) {
@@ -737,6 +764,7 @@ static int process_device_impl(dnspir_ctx* c, const dnspir_shard* const* f0, siz
     }
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(c->stream));
+    if ((rc = check_pair(c))) return rc;
     fill_timing(c, tm, now_us() - t0);
     return 0;
 }

@@ -126,8 +126,11 @@ struct dnspir_ctx {
     bool own = false;
     // size-dependent scratch (arena-capable)
     dne::DevBuf se, M, te, aout, cout, qcoef, qs, qp, acc, outb, ql, acoef, bcoef, a17o, b12o, qbytes, rbytes;
-    // small persistent buffers (pointer lists, lane lists)
-    dne::DevBuf ptrs, idx, keyp;
+    // small persistent buffers (pointer lists, lane lists, the two-CU chain's
+    // exchange words and timeout flag)
+    dne::DevBuf ptrs, idx, keyp, xch;
+    unsigned* h_err = nullptr;  // pinned copy of the chain timeout flag
+    bool pair_pending = false;  // h_err is to be checked after the next sync
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // start, primary end, end, k1 start, k1 end
     int k1_launches = 0;
     std::vector<const dnspir_shard*> cached[2];
@@ -146,7 +149,10 @@ struct dnspir_ctx {
             if (g) (void)hipGraphExecDestroy(g), g = nullptr;
         graph_key.clear();
     }
-    ~dnspir_ctx() { drop_graphs(); }
+    ~dnspir_ctx() {
+        drop_graphs();
+        if (h_err) (void)hipHostFree(h_err);
+    }
 
     dne::DevBuf* scratch_bufs[17] = {&se, &M, &te, &aout, &cout, &qcoef, &qs, &qp, &acc,
                                      &outb, &ql, &acoef, &bcoef, &a17o, &b12o, &qbytes, &rbytes};

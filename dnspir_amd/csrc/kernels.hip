@@ -1198,6 +1198,193 @@ void launch_chain_sec(const uint32_t* te_tmp, uint32_t* a_out, uint32_t* c_out, 
                        qs, lane_slot, L, nlanes, 2, T);
 }
 
+// The secondary chain on two CUs per chain (latency form for a few queries):
+// block (chain, r) keeps a and c mod p_r. Each step it inverse-transforms
+// sigma_5(a) mod p_r, swaps the 2048 coefficients with its partner block (the
+// other modulus) through xbuf, computes the CRT digits of every coefficient,
+// forward-transforms the two digit vectors mod p_r and updates a and c mod
+// p_r: half the transforms of chain_kernel<true> per block, plus one
+// exchange. Partners are blocks b and b ^ 8 (one XCD under round-robin
+// dispatch). Each exchanged word carries a step tag in bits 27-31 (values
+// are < 2p < 2^27), so a block waits for exactly its partner's words of this
+// step and needs no fence; xbuf starts zeroed (tag 0 is never used). Every
+// wait is bounded by spin_limit polls; a timeout sets *err and the block
+// runs on, so the kernel always completes.
+__global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __restrict__ te_tmp, uint32_t* __restrict__ a_out,
+                                                          uint32_t* __restrict__ c_out, const uint32_t* __restrict__ qs,
+                                                          const int* __restrict__ lane_slot, int L, int nlanes, int nchains,
+                                                          uint32_t* xbuf, unsigned* err, int spin_limit,
+                                                          const DevTables* __restrict__ T) {
+    __shared__ uint32_t tw[2][N], twq[2][N];  // [fwd/inv][k], modulus r
+    __shared__ uint32_t sa[N];
+    __shared__ uint32_t sc[2][N];      // [buffer][slot]
+    __shared__ uint32_t xa[NPAD];      // inverse transform of sigma_5(a)
+    __shared__ uint32_t xd[2][NPAD];   // digit transforms
+    const int b = blockIdx.x;
+    const int chain = (b >> 4) * 8 + (b & 7), r = (b >> 3) & 1;
+    if (chain >= nchains) return;
+    const int st = chain / nlanes, li = chain % nlanes;
+    const int tid = threadIdx.x;
+    const uint32_t p = mod_p(r);
+    const int K = mod_k(r);
+    for (int k = tid; k < N; k += 1024) {
+        tw[0][k] = T->psi[r][k]; twq[0][k] = T->psiq[r][k];
+        tw[1][k] = T->ipsi[r][k]; twq[1][k] = T->ipsiq[r][k];
+        sa[k] = 0;
+        sc[0][k] = 0;
+    }
+    // per-thread constants for the update phase: slots tid, tid + 1024
+    uint32_t cg0[2], cg1[2], cqa[2], cb0[2], cb1[2], cqb[2];
+    int sig_upd[2];
+    const uint32_t* qa = T->qa[2 + st][0];
+    const uint32_t* qv = qs + (size_t)lane_slot[li] * 6 * 2 * N;  // [v][r][slot]
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int s = tid + 1024 * k;
+        cg0[k] = T->gka[0][r][s];
+        cg1[k] = T->gka[1][r][s];
+        cqa[k] = qa[r * N + s];
+        cb0[k] = qv[(0 * 2 + r) * N + s];
+        cb1[k] = qv[(1 * 2 + r) * N + s];
+        cqb[k] = qv[((4 + st) * 2 + r) * N + s];
+        sig_upd[k] = T->sigma5[s];
+    }
+    // phase A: threads 0..255, radix-4 groups g, g + 256 first
+    const int gA = tid & 255;
+    int sigA[8];
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) sigA[4 * k + c] = T->sigma5[4 * (gA + 256 * k) + c];
+    __syncthreads();
+
+    uint32_t* xo = xbuf + (size_t)chain * 4 * N;  // [parity][r][N]
+    const uint32_t* te_base = te_tmp + ((size_t)(st * nlanes + li) * L) * 2 * N;
+    int cur = 0;
+#pragma unroll 1
+    for (int i = 0; i < L; ++i) {
+        // ---- A: inverse NTT of sigma_5(a) mod p_r ----
+        if (tid < 256) {
+            const uint32_t* itw = tw[1];
+            const uint32_t* itwq = twq[1];
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const int ii = gA + 256 * k;
+                uint32_t v0 = sa[sigA[4 * k]], v1 = sa[sigA[4 * k + 1]];
+                uint32_t v2 = sa[sigA[4 * k + 2]], v3 = sa[sigA[4 * k + 3]];
+                bf_gs(v0, v1, itw[1024 + 2 * ii], itwq[1024 + 2 * ii], p);
+                bf_gs(v2, v3, itw[1025 + 2 * ii], itwq[1025 + 2 * ii], p);
+                const uint32_t w = itw[512 + ii], wq = itwq[512 + ii];
+                bf_gs(v0, v2, w, wq, p);
+                bf_gs(v1, v3, w, wq, p);
+                const int b4 = lx<true>(4 * ii);
+                xa[b4] = v0; xa[b4 + 1] = v1; xa[b4 + 2] = v2; xa[b4 + 3] = v3;
+            }
+        }
+        __syncthreads();
+#pragma unroll 1
+        for (int P = 2; P >= 0; --P) {
+            if (tid < 256) inv_pass8<true>(xa, tw[1], twq[1], p, P, gA);
+            __syncthreads();
+        }
+        // ---- exchange with the partner (other modulus) ----
+        const uint32_t tag = (uint32_t)(i % 31 + 1) << 27;
+        uint32_t* mine = xo + ((size_t)(i & 1) * 2 + r) * N;
+        uint32_t* theirs = xo + ((size_t)(i & 1) * 2 + (r ^ 1)) * N;
+        uint32_t own[2], oth[2];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int j = tid + 1024 * k;
+            own[k] = xa[lx<true>(j)];  // [0, 2p)
+            __hip_atomic_store(mine + j, tag | own[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int j = tid + 1024 * k;
+            uint32_t u = __hip_atomic_load(theirs + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int n = 0;
+            while ((u & 0xF8000000u) != tag) {
+                // give up at the limit, or soon after any wait has (later steps then pass quickly)
+                if ((++n & 255) == 0 &&
+                    (n >= spin_limit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
+                    atomicOr(err, 1u);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                u = __hip_atomic_load(theirs + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            oth[k] = u & 0x07FFFFFFu;
+        }
+        // ---- B: CRT, centered lift, digits (base_pir.rs:544-570), mod p_r ----
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int j = tid + 1024 * k;
+            const int xj = lx<true>(j);
+            uint32_t x1 = r == 0 ? own[k] : oth[k], x2 = r == 0 ? oth[k] : own[k];
+            x1 = x1 >= P1 ? x1 - P1 : x1;
+            x2 = x2 >= P2 ? x2 - P2 : x2;
+            const uint32_t e1 = mul_mod(x1, NINV_Q2INV_P1, P1, 26);
+            const uint32_t e2 = mul_mod(x2, NINV_Q1INV_P2, P2, 25);
+            uint64_t v = (uint64_t)e1 * P2 + (uint64_t)e2 * P1;
+            v = v >= Q ? v - Q : v;
+            const int64_t x = center_q(v);
+            const int64_t hi = (x + (1ll << (LOG2_B - 1))) >> LOG2_B;
+            const int32_t lo = (int32_t)(x - (hi << LOG2_B));               // [-2^25, 2^25)
+            const int32_t hi2 = (int32_t)((hi + (1ll << (LOG2_B - 1))) >> LOG2_B);
+            const int32_t lo2 = (int32_t)(hi - ((int64_t)hi2 << LOG2_B));   // |lo2| <= 2^24
+            uint32_t t = (uint32_t)(lo + (int32_t)(2 * p));
+            t = t >= 2 * p ? t - 2 * p : t;
+            xd[0][xj] = t >= p ? t - p : t;
+            t = (uint32_t)(lo2 + (int32_t)p);
+            xd[1][xj] = t >= p ? t - p : t;
+        }
+        __syncthreads();
+        uint32_t te[2];
+        const uint32_t* te_row = te_base + (size_t)i * 2 * N + (size_t)r * N;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) te[k] = te_row[tid + 1024 * k];
+        // ---- C: forward NTTs of the two digit vectors mod p_r (threads 0..511) ----
+#pragma unroll
+        for (int P = 0; P < 3; ++P) {
+            if (tid < 512) fwd_pass8<true>(xd[tid >> 8], tw[0], twq[0], p, P, tid & 255);
+            __syncthreads();
+        }
+        if (tid < 512) fwd_pass4<true>(xd[tid >> 8], tw[0], twq[0], p, tid & 255);
+        __syncthreads();
+        // ---- D: update a and c mod p_r ----
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int s = tid + 1024 * k;
+            const uint32_t d0 = xd[0][lx<true>(s)], d1 = xd[1][lx<true>(s)];  // [0, 4p)
+            sa[s] = red64((uint64_t)d0 * cg0[k] + (uint64_t)d1 * cg1[k] + (uint64_t)te[k] * cqa[k], p, K);
+            const uint64_t cv = (uint64_t)sc[cur][sig_upd[k]] + (uint64_t)d0 * cb0[k] + (uint64_t)d1 * cb1[k] +
+                                (uint64_t)te[k] * cqb[k];
+            sc[cur ^ 1][s] = red64(cv, p, K);
+        }
+        cur ^= 1;
+        __syncthreads();
+    }
+    uint32_t* ao = a_out + ((size_t)chain * 2 + r) * N;
+    uint32_t* co = c_out + ((size_t)chain * 2 + r) * N;
+    for (int s = tid; s < N; s += 1024) {
+        ao[s] = sa[s];
+        co[s] = sc[cur][s];
+    }
+}
+
+// exchange words per chain of chain_pair_kernel
+size_t chain_pair_xbuf_words(int nchains) { return (size_t)nchains * 4 * N; }
+
+void launch_chain_pair(const uint32_t* te_tmp, uint32_t* a_out, uint32_t* c_out, const uint32_t* qs, const int* lane_slot,
+                       int L, int nst, int nlanes, uint32_t* xbuf, unsigned* err, const DevTables* T, hipStream_t s) {
+    const int nchains = nst * nlanes;
+    launch_fill_u32(xbuf, chain_pair_xbuf_words(nchains), 0, s);
+    launch_fill_u32(err, 1, 0, s);
+    const int spin_limit = 1 << 21;
+    hipLaunchKernelGGL(chain_pair_kernel, dim3(16 * ((nchains + 7) / 8)), dim3(1024), 0, s, te_tmp, a_out, c_out, qs,
+                       lane_slot, L, nlanes, nchains, xbuf, err, spin_limit, T);
+}
+
 // K1 for one or two queries on the limb-block layout: the same exact limb
 // arithmetic as k1b_kernel, on the vector ALU (v_dot4_i32_i8 over 4 digit
 // pairs at a time) so that many waves per CU keep enough loads in flight to

@@ -203,6 +203,59 @@ __device__ __forceinline__ void inv_pass4(uint32_t* buf, const uint32_t* tw, con
         buf[b4] = v0; buf[b4 + 1] = v1; buf[b4 + 2] = v2; buf[b4 + 3] = v3;
     }
 }
+// Radix-4 passes for 512 threads per transform (g < 512): stages t, t+1
+// (forward) or t+1, t (inverse) with 2^t groups at stage t; element stride
+// q = N / 2^(t+2) inside a group of 4.
+template <bool PAD = false>
+__device__ __forceinline__ void fwd_pass4g(uint32_t* buf, const uint32_t* tw, const uint32_t* twq, uint32_t p, int t, int g) {
+    const int m = 1 << t, lq = LOGN - t - 2, q = 1 << lq;
+    const int i = g >> lq, j0 = (i << (lq + 2)) + (g & (q - 1));
+    uint32_t v0 = buf[lx<PAD>(j0)], v1 = buf[lx<PAD>(j0 + q)], v2 = buf[lx<PAD>(j0 + 2 * q)], v3 = buf[lx<PAD>(j0 + 3 * q)];
+    const uint32_t w = tw[m + i], wq = twq[m + i];
+    bf_ct(v0, v2, w, wq, p);
+    bf_ct(v1, v3, w, wq, p);
+    bf_ct(v0, v1, tw[2 * m + 2 * i], twq[2 * m + 2 * i], p);
+    bf_ct(v2, v3, tw[2 * m + 2 * i + 1], twq[2 * m + 2 * i + 1], p);
+    buf[lx<PAD>(j0)] = v0; buf[lx<PAD>(j0 + q)] = v1; buf[lx<PAD>(j0 + 2 * q)] = v2; buf[lx<PAD>(j0 + 3 * q)] = v3;
+}
+// last forward stage (10) by 512 threads: pairs (2i, 2i + 1), i = g, g + 512
+template <bool PAD = false>
+__device__ __forceinline__ void fwd_pass2_last(uint32_t* buf, const uint32_t* tw, const uint32_t* twq, uint32_t p, int g) {
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int i = g + 512 * k;
+        uint32_t x = buf[lx<PAD>(2 * i)], y = buf[lx<PAD>(2 * i + 1)];
+        bf_ct(x, y, tw[1024 + i], twq[1024 + i], p);
+        buf[lx<PAD>(2 * i)] = x;
+        buf[lx<PAD>(2 * i + 1)] = y;
+    }
+}
+template <bool PAD = false>
+__device__ __forceinline__ void inv_pass4g(uint32_t* buf, const uint32_t* tw, const uint32_t* twq, uint32_t p, int t, int g) {
+    const int m = 1 << t, lq = LOGN - t - 2, q = 1 << lq;
+    const int i = g >> lq, j0 = (i << (lq + 2)) + (g & (q - 1));
+    uint32_t v0 = buf[lx<PAD>(j0)], v1 = buf[lx<PAD>(j0 + q)], v2 = buf[lx<PAD>(j0 + 2 * q)], v3 = buf[lx<PAD>(j0 + 3 * q)];
+    bf_gs(v0, v1, tw[2 * m + 2 * i], twq[2 * m + 2 * i], p);
+    bf_gs(v2, v3, tw[2 * m + 2 * i + 1], twq[2 * m + 2 * i + 1], p);
+    const uint32_t w = tw[m + i], wq = twq[m + i];
+    bf_gs(v0, v2, w, wq, p);
+    bf_gs(v1, v3, w, wq, p);
+    buf[lx<PAD>(j0)] = v0; buf[lx<PAD>(j0 + q)] = v1; buf[lx<PAD>(j0 + 2 * q)] = v2; buf[lx<PAD>(j0 + 3 * q)] = v3;
+}
+// last inverse stage (0) by 512 threads: pairs (j, j + N/2), j = g, g + 512
+template <bool PAD = false>
+__device__ __forceinline__ void inv_pass2_last(uint32_t* buf, const uint32_t* tw, const uint32_t* twq, uint32_t p, int g) {
+    const uint32_t w = tw[1], wq = twq[1];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int j = g + 512 * k;
+        uint32_t x = buf[lx<PAD>(j)], y = buf[lx<PAD>(j + N / 2)];
+        bf_gs(x, y, w, wq, p);
+        buf[lx<PAD>(j)] = x;
+        buf[lx<PAD>(j + N / 2)] = y;
+    }
+}
+
 // whole-block transforms by 256 threads (g = thread index) on an LDS array;
 // inputs in [0, 2p), outputs in [0, p) (forward) / [0, 2p) (inverse, which
 // leaves the N factor in)
@@ -1249,13 +1302,11 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
         cqb[k] = qv[((4 + st) * 2 + r) * N + s];
         sig_upd[k] = T->sigma5[s];
     }
-    // phase A: threads 0..255, radix-4 groups g, g + 256 first
-    const int gA = tid & 255;
-    int sigA[8];
+    // phase A: threads 0..511, radix-4 group g = tid first (stages 10, 9)
+    const int gA = tid & 511;
+    int sigA[4];
 #pragma unroll
-    for (int k = 0; k < 2; ++k)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) sigA[4 * k + c] = T->sigma5[4 * (gA + 256 * k) + c];
+    for (int c = 0; c < 4; ++c) sigA[c] = T->sigma5[4 * gA + c];
     __syncthreads();
 
     uint32_t* xo = xbuf + (size_t)chain * 4 * N;  // [parity][r][N]
@@ -1263,30 +1314,28 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
     int cur = 0;
 #pragma unroll 1
     for (int i = 0; i < L; ++i) {
-        // ---- A: inverse NTT of sigma_5(a) mod p_r ----
-        if (tid < 256) {
+        // ---- A: inverse NTT of sigma_5(a) mod p_r (threads 0..511, radix 4) ----
+        if (tid < 512) {
             const uint32_t* itw = tw[1];
             const uint32_t* itwq = twq[1];
-#pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                const int ii = gA + 256 * k;
-                uint32_t v0 = sa[sigA[4 * k]], v1 = sa[sigA[4 * k + 1]];
-                uint32_t v2 = sa[sigA[4 * k + 2]], v3 = sa[sigA[4 * k + 3]];
-                bf_gs(v0, v1, itw[1024 + 2 * ii], itwq[1024 + 2 * ii], p);
-                bf_gs(v2, v3, itw[1025 + 2 * ii], itwq[1025 + 2 * ii], p);
-                const uint32_t w = itw[512 + ii], wq = itwq[512 + ii];
-                bf_gs(v0, v2, w, wq, p);
-                bf_gs(v1, v3, w, wq, p);
-                const int b4 = lx<true>(4 * ii);
-                xa[b4] = v0; xa[b4 + 1] = v1; xa[b4 + 2] = v2; xa[b4 + 3] = v3;
-            }
+            const int ii = gA;
+            uint32_t v0 = sa[sigA[0]], v1 = sa[sigA[1]], v2 = sa[sigA[2]], v3 = sa[sigA[3]];
+            bf_gs(v0, v1, itw[1024 + 2 * ii], itwq[1024 + 2 * ii], p);
+            bf_gs(v2, v3, itw[1025 + 2 * ii], itwq[1025 + 2 * ii], p);
+            const uint32_t w = itw[512 + ii], wq = itwq[512 + ii];
+            bf_gs(v0, v2, w, wq, p);
+            bf_gs(v1, v3, w, wq, p);
+            const int b4 = lx<true>(4 * ii);
+            xa[b4] = v0; xa[b4 + 1] = v1; xa[b4 + 2] = v2; xa[b4 + 3] = v3;
         }
         __syncthreads();
-#pragma unroll 1
-        for (int P = 2; P >= 0; --P) {
-            if (tid < 256) inv_pass8<true>(xa, tw[1], twq[1], p, P, gA);
+#pragma unroll
+        for (int t = 7; t >= 1; t -= 2) {
+            if (tid < 512) inv_pass4g<true>(xa, tw[1], twq[1], p, t, gA);
             __syncthreads();
         }
+        if (tid < 512) inv_pass2_last<true>(xa, tw[1], twq[1], p, gA);
+        __syncthreads();
         // ---- exchange with the partner (other modulus) ----
         const uint32_t tag = (uint32_t)(i % 31 + 1) << 27;
         uint32_t* mine = xo + ((size_t)(i & 1) * 2 + r) * N;
@@ -1343,13 +1392,13 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
         const uint32_t* te_row = te_base + (size_t)i * 2 * N + (size_t)r * N;
 #pragma unroll
         for (int k = 0; k < 2; ++k) te[k] = te_row[tid + 1024 * k];
-        // ---- C: forward NTTs of the two digit vectors mod p_r (threads 0..511) ----
+        // ---- C: forward NTTs of the two digit vectors mod p_r (512 threads each, radix 4) ----
 #pragma unroll
-        for (int P = 0; P < 3; ++P) {
-            if (tid < 512) fwd_pass8<true>(xd[tid >> 8], tw[0], twq[0], p, P, tid & 255);
+        for (int t = 0; t <= 8; t += 2) {
+            fwd_pass4g<true>(xd[tid >> 9], tw[0], twq[0], p, t, tid & 511);
             __syncthreads();
         }
-        if (tid < 512) fwd_pass4<true>(xd[tid >> 8], tw[0], twq[0], p, tid & 255);
+        fwd_pass2_last<true>(xd[tid >> 9], tw[0], twq[0], p, tid & 511);
         __syncthreads();
         // ---- D: update a and c mod p_r ----
 #pragma unroll

@@ -170,6 +170,7 @@ int dne::ctx_init(dnspir_ctx* c, hipStream_t s) {
 int dne::ctx_presize(dnspir_ctx* c, int max_nq) {
     int rc;
     if ((rc = c->ptrs.ensure(PTR_SLOTS * sizeof(void*)))) return rc;
+    if ((rc = c->xch.ensure((chain_pair_xbuf_words(PAIR_MAX_CHAINS) + 64) * 4))) return rc;
     if ((rc = c->idx.ensure((size_t)3 * max_nq * (sizeof(long long) + 2 * sizeof(int)) + 64))) return rc;
     return c->keyp.ensure((size_t)2 * max_nq * sizeof(void*));
 }
@@ -270,8 +271,18 @@ int run_primary(dnspir_ctx* c, const uint32_t* const* slabs, int len, int l_p, c
 
 // Secondary phase for `nlanes` lanes whose entries are described by the lane
 // lists; the reply (a, b per lane) lands in c->acoef/bcoef/a17o/b12o.
+}  // namespace
+
+// queues the copy of the two-CU chain's timeout flag to pinned host memory
+// (outside any graph capture: a copy to host memory cannot be captured)
+static int copy_pair_flag(dnspir_ctx* c) {
+    HIP_OK(hipMemcpyAsync(c->h_err, c->xch.p, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+    c->pair_pending = true;
+    return 0;
+}
+
 // after a sync: the exchange flag of the last two-CU chain launch
-int check_pair(dnspir_ctx* c) {
+int dne::check_pair(dnspir_ctx* c) {
     if (!c->pair_pending) return 0;
     c->pair_pending = false;
     if (*c->h_err) {
@@ -281,8 +292,11 @@ int check_pair(dnspir_ctx* c) {
     return 0;
 }
 
+namespace {
+
 // pair: run the chains on two CUs each (chain_pair_kernel, lower latency for a
-// few chains); the caller checks the exchange flag after its sync (check_pair)
+// few chains); the caller sets c->pair_pending and checks the exchange flag
+// after its sync (check_pair)
 int run_secondary(dnspir_ctx* c, const uint16_t* entries, long long col_stride, const long long* d_off, const int* d_ncols,
                   const int* d_slot, int nlanes, int l_s, bool pair = false) {
     const int L_s = loop_len(l_s), nst_s = nstreams(l_s);
@@ -298,13 +312,11 @@ int run_secondary(dnspir_ctx* c, const uint16_t* entries, long long col_stride, 
     if ((rc = c->b12o.ensure((size_t)nlanes * N * 4))) return rc;
     launch_k5(entries, d_off, col_stride, c->M.as<uint32_t>(), c->te.as<uint32_t>(), l_s, d_ncols, nlanes, g.tables, s);
     if (pair) {
+        // xch: the timeout flag in word 0, the exchange words from word 64
         const size_t words = chain_pair_xbuf_words(nst_s * nlanes);
         if ((rc = c->xch.ensure((words + 64) * 4))) return rc;
-        unsigned* err = reinterpret_cast<unsigned*>(c->xch.as<uint32_t>() + words);
         launch_chain_pair(c->te.as<uint32_t>(), c->aout.as<uint32_t>(), c->cout.as<uint32_t>(), c->qs.as<uint32_t>(), d_slot,
-                          L_s, nst_s, nlanes, c->xch.as<uint32_t>(), err, g.tables, s);
-        HIP_OK(hipMemcpyAsync(c->h_err, err, sizeof(unsigned), hipMemcpyDeviceToHost, s));
-        c->pair_pending = true;
+                          L_s, nst_s, nlanes, c->xch.as<uint32_t>() + 64, c->xch.as<unsigned>(), g.tables, s);
     } else {
         launch_chain_sec(c->te.as<uint32_t>(), c->aout.as<uint32_t>(), c->cout.as<uint32_t>(), c->qs.as<uint32_t>(), d_slot,
                          L_s, nst_s, nlanes, g.tables, s);
@@ -374,6 +386,7 @@ int run_double(dnspir_ctx* c, const dnspir_shard* const* fleets[2], const size_t
     if ((rc = run_secondary(c, c->se.as<uint16_t>(), (long long)LANES * N, d_off, d_ncols, d_slot, (int)off.size(), l_s,
                             true)))
         return rc;
+    if ((rc = copy_pair_flag(c))) return rc;
     HIP_OK(hipEventRecord(c->ev[2], s));
     if (tm) {
         tm->bytes_streamed = streamed;
@@ -878,6 +891,10 @@ int dne::batch_impl(dnspir_ctx* c, const dnspir_shard* const* fleet, size_t len,
     if (qlen < query_layout(l_p, l_s, offs, bits)) return DNSPIR_E_SHORT_QUERY;
     const hipStream_t s = c->stream;
     if ((rc = batch_reserve(c, l_p, len, nq, 0))) return rc;
+    // few chains run on two CUs each; their exchange buffer is sized once for
+    // the largest such batch (no allocation inside a graph capture)
+    const bool pair = 3 * nq * nstreams(l_s) <= PAIR_MAX_CHAINS;
+    if (pair && (rc = c->xch.ensure((chain_pair_xbuf_words(PAIR_MAX_CHAINS) + 64) * 4))) return rc;
     const uint32_t* const* slabs;
     const uint32_t* const* a17s;
     if ((rc = upload_fleet(c, 0, fleet, len, &slabs, &a17s))) return rc;
@@ -934,7 +951,7 @@ int dne::batch_impl(dnspir_ctx* c, const dnspir_shard* const* fleet, size_t len,
     };
     auto enqueue_secondary = [&]() -> int {
         int r2;
-        if ((r2 = run_secondary(c, c->se.as<uint16_t>(), 3LL * N, d_off, d_ncols, d_slot, 3 * nq, l_s))) return r2;
+        if ((r2 = run_secondary(c, c->se.as<uint16_t>(), 3LL * N, d_off, d_ncols, d_slot, 3 * nq, l_s, pair))) return r2;
         launch_encode_reply(c->a17o.as<uint32_t>(), c->b12o.as<uint32_t>(), 0, nq, rdev, s);
         return 0;
     };
@@ -944,7 +961,8 @@ int dne::batch_impl(dnspir_ctx* c, const dnspir_shard* const* fleet, size_t len,
         // everything the captured launches bake in
         std::vector<uintptr_t> key = {(uintptr_t)qdev, qlen, (uintptr_t)nq, (uintptr_t)rdev, len, (uintptr_t)l_p,
                                       (uintptr_t)use_keys, (uintptr_t)c->keyp.p, (uintptr_t)c->ptrs.p, (uintptr_t)c->idx.p,
-                                      (uintptr_t)k1b_get_choice(1, nstreams(l_p)), (uintptr_t)k1b_get_choice(2, nstreams(l_p))};
+                                      (uintptr_t)k1b_get_choice(1, nstreams(l_p)), (uintptr_t)k1b_get_choice(2, nstreams(l_p)),
+                                      (uintptr_t)c->xch.p, (uintptr_t)c->h_err};
         for (auto* b : c->scratch_bufs) key.push_back((uintptr_t)b->p);
         if (!c->graph_exec[0] || key != c->graph_key) {
             c->drop_graphs();
@@ -982,9 +1000,12 @@ int dne::batch_impl(dnspir_ctx* c, const dnspir_shard* const* fleet, size_t len,
         HIP_OK(hipEventRecord(c->ev[2], s));
     }
     c->k1_timed = !graphed;
+    c->pair_pending = false;
+    if (pair && (rc = copy_pair_flag(c))) return rc;
     HIP_OK(hipGetLastError());
     if (!sync) return 0;
     HIP_OK(hipStreamSynchronize(s));
+    if ((rc = check_pair(c))) return rc;
     fill_timing(c, tm, now_us() - t0);
     if (tm) {
         tm->bytes_streamed = (uint64_t)((nq + 63) / 64) * len * slab_bytes(l_p);

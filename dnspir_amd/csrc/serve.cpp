@@ -333,6 +333,7 @@ struct dnspir_server {
                         hipSuccess ||
                     hipStreamSynchronize(c->stream) != hipSuccess))
             rc = DNSPIR_E_HIP;
+        if (!rc) rc = check_pair(c);
         if (rc == DNSPIR_E_HIP) fprintf(stderr, "dnspir server: %s\n", dnspir_last_error());
         if (!rc)
             for (int i = 0; i < nb; ++i)

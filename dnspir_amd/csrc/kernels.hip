@@ -1251,6 +1251,21 @@ void launch_chain_sec(const uint32_t* te_tmp, uint32_t* a_out, uint32_t* c_out, 
                        qs, lane_slot, L, nlanes, 2, T);
 }
 
+// c <- sigma_5(c) + D0 cb0 + D1 cb1 + TE cqb mod p for slots tid, tid + 1024
+// (the b-recurrence of the fused secondary inner product)
+__device__ __forceinline__ void c_update(const uint32_t* __restrict__ c_in, uint32_t* __restrict__ c_out,
+                                         const uint32_t (*xd)[NPAD], const uint32_t te[2], const uint32_t cb0[2],
+                                         const uint32_t cb1[2], const uint32_t cqb[2], const int sig[2], uint32_t p, int K,
+                                         int tid) {
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int s = tid + 1024 * k;
+        const uint32_t d0 = xd[0][lx<true>(s)], d1 = xd[1][lx<true>(s)];  // [0, 4p)
+        const uint64_t cv = (uint64_t)c_in[sig[k]] + (uint64_t)d0 * cb0[k] + (uint64_t)d1 * cb1[k] + (uint64_t)te[k] * cqb[k];
+        c_out[s] = red64(cv, p, K);
+    }
+}
+
 // The secondary chain on two CUs per chain (latency form for a few queries):
 // block (chain, r) keeps a and c mod p_r. Each step it inverse-transforms
 // sigma_5(a) mod p_r, swaps the 2048 coefficients with its partner block (the
@@ -1312,6 +1327,7 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
     uint32_t* xo = xbuf + (size_t)chain * 4 * N;  // [parity][r][N]
     const uint32_t* te_base = te_tmp + ((size_t)(st * nlanes + li) * L) * 2 * N;
     int cur = 0;
+    uint32_t te[2] = {0, 0};  // TE row of the current (then previous) step
 #pragma unroll 1
     for (int i = 0; i < L; ++i) {
         // ---- A: inverse NTT of sigma_5(a) mod p_r (threads 0..511, radix 4) ----
@@ -1346,6 +1362,12 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
             const int j = tid + 1024 * k;
             own[k] = xa[lx<true>(j)];  // [0, 2p)
             __hip_atomic_store(mine + j, tag | own[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        // while the partner's words travel: the c-update of the previous step
+        // (its digit transforms are still in xd, its TE row in te)
+        if (i > 0) {
+            c_update(sc[cur], sc[cur ^ 1], xd, te, cb0, cb1, cqb, sig_upd, p, K, tid);
+            cur ^= 1;
         }
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
@@ -1388,7 +1410,6 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
             xd[1][xj] = t >= p ? t - p : t;
         }
         __syncthreads();
-        uint32_t te[2];
         const uint32_t* te_row = te_base + (size_t)i * 2 * N + (size_t)r * N;
 #pragma unroll
         for (int k = 0; k < 2; ++k) te[k] = te_row[tid + 1024 * k];
@@ -1400,19 +1421,18 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
         }
         fwd_pass2_last<true>(xd[tid >> 9], tw[0], twq[0], p, tid & 511);
         __syncthreads();
-        // ---- D: update a and c mod p_r ----
+        // ---- D: update a mod p_r (c follows in the next step's exchange) ----
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
             const int s = tid + 1024 * k;
             const uint32_t d0 = xd[0][lx<true>(s)], d1 = xd[1][lx<true>(s)];  // [0, 4p)
             sa[s] = red64((uint64_t)d0 * cg0[k] + (uint64_t)d1 * cg1[k] + (uint64_t)te[k] * cqa[k], p, K);
-            const uint64_t cv = (uint64_t)sc[cur][sig_upd[k]] + (uint64_t)d0 * cb0[k] + (uint64_t)d1 * cb1[k] +
-                                (uint64_t)te[k] * cqb[k];
-            sc[cur ^ 1][s] = red64(cv, p, K);
         }
-        cur ^= 1;
         __syncthreads();
     }
+    c_update(sc[cur], sc[cur ^ 1], xd, te, cb0, cb1, cqb, sig_upd, p, K, tid);
+    cur ^= 1;
+    __syncthreads();
     uint32_t* ao = a_out + ((size_t)chain * 2 + r) * N;
     uint32_t* co = c_out + ((size_t)chain * 2 + r) * N;
     for (int s = tid; s < N; s += 1024) {

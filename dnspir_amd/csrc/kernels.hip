@@ -1251,17 +1251,18 @@ void launch_chain_sec(const uint32_t* te_tmp, uint32_t* a_out, uint32_t* c_out, 
                        qs, lane_slot, L, nlanes, 2, T);
 }
 
-// c <- sigma_5(c) + D0 cb0 + D1 cb1 + TE cqb mod p for slots tid, tid + 1024
-// (the b-recurrence of the fused secondary inner product)
+// c <- sigma_5(c) + sigma_5(a) cb0 + NTT(D1) cb1' + TE cqb mod p for slots
+// tid, tid + 1024 (the b-recurrence of the fused secondary inner product,
+// with NTT(D0) = sigma_5(a) - 2^26 NTT(D1) folded into cb1' = cb1 - 2^26 cb0)
 __device__ __forceinline__ void c_update(const uint32_t* __restrict__ c_in, uint32_t* __restrict__ c_out,
-                                         const uint32_t (*xd)[NPAD], const uint32_t te[2], const uint32_t cb0[2],
-                                         const uint32_t cb1[2], const uint32_t cqb[2], const int sig[2], uint32_t p, int K,
-                                         int tid) {
+                                         const uint32_t* d1t, const uint32_t sga[2], const uint32_t te[2],
+                                         const uint32_t cb0[2], const uint32_t cb1[2], const uint32_t cqb[2], const int sig[2],
+                                         uint32_t p, int K, int tid) {
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
         const int s = tid + 1024 * k;
-        const uint32_t d0 = xd[0][lx<true>(s)], d1 = xd[1][lx<true>(s)];  // [0, 4p)
-        const uint64_t cv = (uint64_t)c_in[sig[k]] + (uint64_t)d0 * cb0[k] + (uint64_t)d1 * cb1[k] + (uint64_t)te[k] * cqb[k];
+        const uint32_t d1 = d1t[lx<true>(s)];  // [0, 4p)
+        const uint64_t cv = (uint64_t)c_in[sig[k]] + (uint64_t)sga[k] * cb0[k] + (uint64_t)d1 * cb1[k] + (uint64_t)te[k] * cqb[k];
         c_out[s] = red64(cv, p, K);
     }
 }
@@ -1287,7 +1288,7 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
     __shared__ uint32_t sa[N];
     __shared__ uint32_t sc[2][N];      // [buffer][slot]
     __shared__ uint32_t xa[NPAD];      // inverse transform of sigma_5(a)
-    __shared__ uint32_t xd[2][NPAD];   // digit transforms
+    __shared__ uint32_t xd[2][NPAD];   // NTT(D1) in xd[0]; xd[1] only keeps LDS > 80 KB (one block per CU)
     const int b = blockIdx.x;
     const int chain = (b >> 4) * 8 + (b & 7), r = (b >> 3) & 1;
     if (chain >= nchains) return;
@@ -1316,6 +1317,11 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
         cb1[k] = qv[(1 * 2 + r) * N + s];
         cqb[k] = qv[((4 + st) * 2 + r) * N + s];
         sig_upd[k] = T->sigma5[s];
+        // NTT(D0) = sigma_5(a) - 2^26 NTT(D1) (x = D0 + 2^26 D1 and NTT(x mod p) =
+        // sigma_5(a)): fold -2^26 into the D1 constants, keep the D0 ones for sigma_5(a)
+        const uint64_t t26 = (1ull << LOG2_B) % p;
+        cg1[k] = (uint32_t)(((uint64_t)cg1[k] + p - t26 * cg0[k] % p) % p);
+        cb1[k] = (uint32_t)(((uint64_t)cb1[k] + p - t26 * cb0[k] % p) % p);
     }
     // phase A: threads 0..511, radix-4 group g = tid first (stages 10, 9)
     const int gA = tid & 511;
@@ -1327,7 +1333,8 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
     uint32_t* xo = xbuf + (size_t)chain * 4 * N;  // [parity][r][N]
     const uint32_t* te_base = te_tmp + ((size_t)(st * nlanes + li) * L) * 2 * N;
     int cur = 0;
-    uint32_t te[2] = {0, 0};  // TE row of the current (then previous) step
+    uint32_t te[2] = {0, 0};   // TE row of the current (then previous) step
+    uint32_t sga[2] = {0, 0};  // sigma_5(a) at slots tid, tid + 1024, same steps
 #pragma unroll 1
     for (int i = 0; i < L; ++i) {
         // ---- A: inverse NTT of sigma_5(a) mod p_r (threads 0..511, radix 4) ----
@@ -1366,7 +1373,7 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
         // while the partner's words travel: the c-update of the previous step
         // (its digit transforms are still in xd, its TE row in te)
         if (i > 0) {
-            c_update(sc[cur], sc[cur ^ 1], xd, te, cb0, cb1, cqb, sig_upd, p, K, tid);
+            c_update(sc[cur], sc[cur ^ 1], xd[0], sga, te, cb0, cb1, cqb, sig_upd, p, K, tid);
             cur ^= 1;
         }
 #pragma unroll
@@ -1386,7 +1393,10 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
             }
             oth[k] = u & 0x07FFFFFFu;
         }
-        // ---- B: CRT, centered lift, digits (base_pir.rs:544-570), mod p_r ----
+        // sigma_5(a) of this step (sa is rewritten in D)
+#pragma unroll
+        for (int k = 0; k < 2; ++k) sga[k] = sa[sig_upd[k]];
+        // ---- B: CRT, centered lift, digits (base_pir.rs:544-570): D1 mod p_r ----
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
             const int j = tid + 1024 * k;
@@ -1399,38 +1409,33 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
             uint64_t v = (uint64_t)e1 * P2 + (uint64_t)e2 * P1;
             v = v >= Q ? v - Q : v;
             const int64_t x = center_q(v);
-            const int64_t hi = (x + (1ll << (LOG2_B - 1))) >> LOG2_B;
-            const int32_t lo = (int32_t)(x - (hi << LOG2_B));               // [-2^25, 2^25)
-            const int32_t hi2 = (int32_t)((hi + (1ll << (LOG2_B - 1))) >> LOG2_B);
-            const int32_t lo2 = (int32_t)(hi - ((int64_t)hi2 << LOG2_B));   // |lo2| <= 2^24
-            uint32_t t = (uint32_t)(lo + (int32_t)(2 * p));
-            t = t >= 2 * p ? t - 2 * p : t;
+            // |x| < q/2 < 2^50: the second balanced digit is hi itself (the third is 0)
+            const int32_t d1 = (int32_t)((x + (1ll << (LOG2_B - 1))) >> LOG2_B);  // |d1| <= 2^24
+            const uint32_t t = (uint32_t)(d1 + (int32_t)p);
             xd[0][xj] = t >= p ? t - p : t;
-            t = (uint32_t)(lo2 + (int32_t)p);
-            xd[1][xj] = t >= p ? t - p : t;
         }
         __syncthreads();
         const uint32_t* te_row = te_base + (size_t)i * 2 * N + (size_t)r * N;
 #pragma unroll
         for (int k = 0; k < 2; ++k) te[k] = te_row[tid + 1024 * k];
-        // ---- C: forward NTTs of the two digit vectors mod p_r (512 threads each, radix 4) ----
+        // ---- C: forward NTT of D1 mod p_r (threads 0..511, radix 4) ----
 #pragma unroll
         for (int t = 0; t <= 8; t += 2) {
-            fwd_pass4g<true>(xd[tid >> 9], tw[0], twq[0], p, t, tid & 511);
+            if (tid < 512) fwd_pass4g<true>(xd[0], tw[0], twq[0], p, t, gA);
             __syncthreads();
         }
-        fwd_pass2_last<true>(xd[tid >> 9], tw[0], twq[0], p, tid & 511);
+        if (tid < 512) fwd_pass2_last<true>(xd[0], tw[0], twq[0], p, gA);
         __syncthreads();
         // ---- D: update a mod p_r (c follows in the next step's exchange) ----
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
             const int s = tid + 1024 * k;
-            const uint32_t d0 = xd[0][lx<true>(s)], d1 = xd[1][lx<true>(s)];  // [0, 4p)
-            sa[s] = red64((uint64_t)d0 * cg0[k] + (uint64_t)d1 * cg1[k] + (uint64_t)te[k] * cqa[k], p, K);
+            const uint32_t d1 = xd[0][lx<true>(s)];  // NTT(D1), [0, 4p)
+            sa[s] = red64((uint64_t)sga[k] * cg0[k] + (uint64_t)d1 * cg1[k] + (uint64_t)te[k] * cqa[k], p, K);
         }
         __syncthreads();
     }
-    c_update(sc[cur], sc[cur ^ 1], xd, te, cb0, cb1, cqb, sig_upd, p, K, tid);
+    c_update(sc[cur], sc[cur ^ 1], xd[0], sga, te, cb0, cb1, cqb, sig_upd, p, K, tid);
     cur ^= 1;
     __syncthreads();
     uint32_t* ao = a_out + ((size_t)chain * 2 + r) * N;

@@ -461,7 +461,7 @@ __global__ __launch_bounds__(1024) void chain_kernel(const uint32_t* __restrict_
     __shared__ uint32_t tw[2][2][N], twq[2][2][N];  // [fwd/inv][r][k]
     __shared__ uint32_t sa[2][N];
     __shared__ uint32_t sc[2][2][N];                // [buffer][r][slot]
-    __shared__ uint32_t xb[4][NPAD];  // transform buffers, padded index lx<true>
+    __shared__ uint32_t xb[2][NPAD];  // transform buffers [r], padded index lx<true>
     const int li = blockIdx.x, st = blockIdx.y;
     const int tid = threadIdx.x;
     for (int k = tid; k < N; k += 1024) {
@@ -489,6 +489,12 @@ __global__ __launch_bounds__(1024) void chain_kernel(const uint32_t* __restrict_
             cb1[k] = qv[(1 * 2 + r) * N + s];
             cqb[k] = qv[((4 + st) * 2 + r) * N + s];
         }
+        // NTT(D0) = sigma_5(a) - 2^26 NTT(D1) (x = D0 + 2^26 D1 and NTT(x mod p) =
+        // sigma_5(a)): fold -2^26 into the D1 constants, keep the D0 ones for sigma_5(a)
+        const uint32_t p = mod_p(r);
+        const uint64_t t26 = (1ull << LOG2_B) % p;
+        cg1[k] = (uint32_t)(((uint64_t)cg1[k] + p - t26 * cg0[k] % p) % p);
+        if (SEC) cb1[k] = (uint32_t)(((uint64_t)cb1[k] + p - t26 * cb0[k] % p) % p);
     }
     sig_upd[0] = T->sigma5[tid];
     sig_upd[1] = T->sigma5[tid + 1024];
@@ -533,7 +539,7 @@ __global__ __launch_bounds__(1024) void chain_kernel(const uint32_t* __restrict_
             if (tid < 512) inv_pass8<true>(xb[rA], tw[1][rA], twq[1][rA], mod_p(rA), P, gA);
             __syncthreads();
         }
-        // ---- B: CRT, centered lift, digits (base_pir.rs:544-570) ----
+        // ---- B: CRT, centered lift, digit D1 (base_pir.rs:544-570) ----
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
             const int j = tid + 1024 * k;
@@ -546,31 +552,34 @@ __global__ __launch_bounds__(1024) void chain_kernel(const uint32_t* __restrict_
             uint64_t v = (uint64_t)e1 * P2 + (uint64_t)e2 * P1;
             v = v >= Q ? v - Q : v;
             const int64_t x = center_q(v);
-            const int64_t hi = (x + (1ll << (LOG2_B - 1))) >> LOG2_B;
-            const int32_t lo = (int32_t)(x - (hi << LOG2_B));               // [-2^25, 2^25)
-            const int32_t hi2 = (int32_t)((hi + (1ll << (LOG2_B - 1))) >> LOG2_B);
-            const int32_t lo2 = (int32_t)(hi - ((int64_t)hi2 << LOG2_B));   // |lo2| <= 2^24
-            uint32_t t;
-            t = (uint32_t)(lo + (int32_t)(2 * P1)); t = t >= 2 * P1 ? t - 2 * P1 : t; xb[0][xj] = t >= P1 ? t - P1 : t;
-            t = (uint32_t)(lo + (int32_t)(2 * P2)); t = t >= 2 * P2 ? t - 2 * P2 : t; xb[1][xj] = t >= P2 ? t - P2 : t;
-            t = (uint32_t)(lo2 + (int32_t)P1); xb[2][xj] = t >= P1 ? t - P1 : t;
-            t = (uint32_t)(lo2 + (int32_t)P2); xb[3][xj] = t >= P2 ? t - P2 : t;
+            // |x| < q/2 < 2^50: the second balanced digit is hi itself (the third is 0)
+            const int32_t d1 = (int32_t)((x + (1ll << (LOG2_B - 1))) >> LOG2_B);  // |d1| <= 2^24
+            uint32_t t = (uint32_t)(d1 + (int32_t)P1);
+            xb[0][xj] = t >= P1 ? t - P1 : t;
+            t = (uint32_t)(d1 + (int32_t)P2);
+            xb[1][xj] = t >= P2 ? t - P2 : t;
         }
         __syncthreads();
+        // sigma_5(a) of this step (sa is rewritten in D)
+        uint32_t sga[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) sga[k] = sa[k & 1][sig_upd[k >> 1]];
         // TE row of this step (used in phase D; loaded here so phase C hides it)
         uint32_t te[4];
         const uint32_t* te_row = te_base + (size_t)i * 2 * N;
 #pragma unroll
         for (int k = 0; k < 4; ++k) te[k] = te_row[(k & 1) * N + tid + 1024 * (k >> 1)];
-        // ---- C: forward NTTs of the four digit vectors ----
+        // ---- C: forward NTTs of D1 mod p1 and mod p2 (threads 0..511) ----
         {
             const uint32_t p = mod_p(rC);
-#pragma unroll  // constant pass index (the inverse loop above stays rolled: 128 VGPRs)
+            int gCx = gC;
+            asm volatile("" : "+v"(gCx));  // pass addresses recomputed per step (VGPRs)
+#pragma unroll  // constant pass index
             for (int P = 0; P < 3; ++P) {
-                fwd_pass8<true>(xb[q4], tw[0][rC], twq[0][rC], p, P, gC);
+                if (tid < 512) fwd_pass8<true>(xb[rC], tw[0][rC], twq[0][rC], p, P, gCx);
                 __syncthreads();
             }
-            fwd_pass4<true>(xb[q4], tw[0][rC], twq[0][rC], p, gC);
+            if (tid < 512) fwd_pass4<true>(xb[rC], tw[0][rC], twq[0][rC], p, gCx);
         }
         __syncthreads();
         // ---- D: update a (and c) per slot ----
@@ -579,20 +588,23 @@ __global__ __launch_bounds__(1024) void chain_kernel(const uint32_t* __restrict_
             const int s = tid + 1024 * (k >> 1), r = k & 1;
             const uint32_t p = mod_p(r);
             const int K = mod_k(r);
-            const uint32_t d0 = xb[r][lx<true>(s)], d1 = xb[2 + r][lx<true>(s)];  // [0, 4p)
-            sa[r][s] = red64((uint64_t)d0 * cg0[k] + (uint64_t)d1 * cg1[k] + (uint64_t)te[k] * cqa[k], p, K);
+            const uint32_t d1 = xb[r][lx<true>(s)];  // NTT(D1), [0, 4p)
+            sa[r][s] = red64((uint64_t)sga[k] * cg0[k] + (uint64_t)d1 * cg1[k] + (uint64_t)te[k] * cqa[k], p, K);
             if (SEC) {
-                const uint64_t cv = (uint64_t)sc[cur][r][sig_upd[k >> 1]] + (uint64_t)d0 * cb0[k] + (uint64_t)d1 * cb1[k] +
-                                    (uint64_t)te[k] * cqb[k];
+                const uint64_t cv = (uint64_t)sc[cur][r][sig_upd[k >> 1]] + (uint64_t)sga[k] * cb0[k] +
+                                    (uint64_t)d1 * cb1[k] + (uint64_t)te[k] * cqb[k];
                 sc[cur ^ 1][r][s] = red64(cv, p, K);
             } else {
-                // IPGK rows and the TE row, Galois order: [arr][r][gpos]
+                // IPGK rows and the TE row, Galois order: [arr][r][gpos];
+                // NTT(D0) = sigma_5(a) - 2^26 NTT(D1)
                 uint32_t* dst = ip_tmp + (((size_t)(st * nlanes + li) * L + i) * 6) * N;
                 const int gp = gpos_upd[k >> 1];
                 dst[(0 * 2 + r) * N + gp] = te[k];
-                const uint32_t e0 = red2p(d0, 2 * p), e1 = red2p(d1, 2 * p);
-                dst[(1 * 2 + r) * N + gp] = e0 >= p ? e0 - p : e0;
-                dst[(2 * 2 + r) * N + gp] = e1 >= p ? e1 - p : e1;
+                uint32_t e1 = red2p(d1, 2 * p);
+                e1 = e1 >= p ? e1 - p : e1;
+                const uint32_t m = mul_mod(e1, (uint32_t)((1ull << LOG2_B) % p), p, K);
+                dst[(1 * 2 + r) * N + gp] = sga[k] >= m ? sga[k] - m : sga[k] + p - m;
+                dst[(2 * 2 + r) * N + gp] = e1;
             }
         }
         cur ^= 1;

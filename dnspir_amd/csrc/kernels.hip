@@ -102,6 +102,15 @@ __device__ __forceinline__ void bf_gs(uint32_t& x, uint32_t& y, uint32_t w, uint
     y = shoup_lazy(u - v + p2, w, wq, p);
 }
 
+// Barrier that orders LDS traffic only: global loads stay in flight across it
+// (__syncthreads also waits for every outstanding global access). For loops
+// whose threads share data through LDS alone.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // LDS index of element j in a transform buffer. Padded buffers (N + N/8
 // words) insert 4 words after every 32: the radix-8 pass at stride 4 then
 // hits 32 distinct banks per half-wave instead of 4, and 4-aligned groups
@@ -507,7 +516,7 @@ __global__ __launch_bounds__(1024) void chain_kernel(const uint32_t* __restrict_
 #pragma unroll
         for (int c = 0; c < 4; ++c) sigA[4 * k + c] = T->sigma5[4 * (gA + 256 * k) + c];
     const int q4 = tid >> 8, rC = q4 & 1, gC = tid & 255;
-    __syncthreads();
+    lds_barrier();
 
     const uint32_t* te_base = te_tmp + ((size_t)(st * nlanes + li) * L) * 2 * N;
     int cur = 0;
@@ -533,11 +542,11 @@ __global__ __launch_bounds__(1024) void chain_kernel(const uint32_t* __restrict_
                 b[b4] = v0; b[b4 + 1] = v1; b[b4 + 2] = v2; b[b4 + 3] = v3;
             }
         }
-        __syncthreads();
+        lds_barrier();
 #pragma unroll 1
         for (int P = 2; P >= 0; --P) {
             if (tid < 512) inv_pass8<true>(xb[rA], tw[1][rA], twq[1][rA], mod_p(rA), P, gA);
-            __syncthreads();
+            lds_barrier();
         }
         // ---- B: CRT, centered lift, digit D1 (base_pir.rs:544-570) ----
 #pragma unroll
@@ -559,7 +568,7 @@ __global__ __launch_bounds__(1024) void chain_kernel(const uint32_t* __restrict_
             t = (uint32_t)(d1 + (int32_t)P2);
             xb[1][xj] = t >= P2 ? t - P2 : t;
         }
-        __syncthreads();
+        lds_barrier();
         // sigma_5(a) of this step (sa is rewritten in D)
         uint32_t sga[4];
 #pragma unroll
@@ -577,11 +586,11 @@ __global__ __launch_bounds__(1024) void chain_kernel(const uint32_t* __restrict_
 #pragma unroll  // constant pass index
             for (int P = 0; P < 3; ++P) {
                 if (tid < 512) fwd_pass8<true>(xb[rC], tw[0][rC], twq[0][rC], p, P, gCx);
-                __syncthreads();
+                lds_barrier();
             }
             if (tid < 512) fwd_pass4<true>(xb[rC], tw[0][rC], twq[0][rC], p, gCx);
         }
-        __syncthreads();
+        lds_barrier();
         // ---- D: update a (and c) per slot ----
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
@@ -608,7 +617,7 @@ __global__ __launch_bounds__(1024) void chain_kernel(const uint32_t* __restrict_
             }
         }
         cur ^= 1;
-        __syncthreads();
+        lds_barrier();
     }
     uint32_t* ao = a_out + ((size_t)(st * nlanes + li) * 2) * N;
     for (int s = tid; s < N; s += 1024) {
@@ -1340,7 +1349,7 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
     int sigA[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) sigA[c] = T->sigma5[4 * gA + c];
-    __syncthreads();
+    lds_barrier();
 
     uint32_t* xo = xbuf + (size_t)chain * 4 * N;  // [parity][r][N]
     const uint32_t* te_base = te_tmp + ((size_t)(st * nlanes + li) * L) * 2 * N;
@@ -1363,14 +1372,14 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
             const int b4 = lx<true>(4 * ii);
             xa[b4] = v0; xa[b4 + 1] = v1; xa[b4 + 2] = v2; xa[b4 + 3] = v3;
         }
-        __syncthreads();
+        lds_barrier();
 #pragma unroll
         for (int t = 7; t >= 1; t -= 2) {
             if (tid < 512) inv_pass4g<true>(xa, tw[1], twq[1], p, t, gA);
-            __syncthreads();
+            lds_barrier();
         }
         if (tid < 512) inv_pass2_last<true>(xa, tw[1], twq[1], p, gA);
-        __syncthreads();
+        lds_barrier();
         // ---- exchange with the partner (other modulus) ----
         const uint32_t tag = (uint32_t)(i % 31 + 1) << 27;
         uint32_t* mine = xo + ((size_t)(i & 1) * 2 + r) * N;
@@ -1405,6 +1414,11 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
             }
             oth[k] = u & 0x07FFFFFFu;
         }
+        // this step's TE row (used in D): in flight through B and C, whose
+        // barriers order LDS only
+        const uint32_t* te_row = te_base + (size_t)i * 2 * N + (size_t)r * N;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) te[k] = te_row[tid + 1024 * k];
         // sigma_5(a) of this step (sa is rewritten in D)
 #pragma unroll
         for (int k = 0; k < 2; ++k) sga[k] = sa[sig_upd[k]];
@@ -1426,18 +1440,15 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
             const uint32_t t = (uint32_t)(d1 + (int32_t)p);
             xd[0][xj] = t >= p ? t - p : t;
         }
-        __syncthreads();
-        const uint32_t* te_row = te_base + (size_t)i * 2 * N + (size_t)r * N;
-#pragma unroll
-        for (int k = 0; k < 2; ++k) te[k] = te_row[tid + 1024 * k];
+        lds_barrier();
         // ---- C: forward NTT of D1 mod p_r (threads 0..511, radix 4) ----
 #pragma unroll
         for (int t = 0; t <= 8; t += 2) {
             if (tid < 512) fwd_pass4g<true>(xd[0], tw[0], twq[0], p, t, gA);
-            __syncthreads();
+            lds_barrier();
         }
         if (tid < 512) fwd_pass2_last<true>(xd[0], tw[0], twq[0], p, gA);
-        __syncthreads();
+        lds_barrier();
         // ---- D: update a mod p_r (c follows in the next step's exchange) ----
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
@@ -1445,11 +1456,11 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
             const uint32_t d1 = xd[0][lx<true>(s)];  // NTT(D1), [0, 4p)
             sa[s] = red64((uint64_t)sga[k] * cg0[k] + (uint64_t)d1 * cg1[k] + (uint64_t)te[k] * cqa[k], p, K);
         }
-        __syncthreads();
+        lds_barrier();
     }
     c_update(sc[cur], sc[cur ^ 1], xd[0], sga, te, cb0, cb1, cqb, sig_upd, p, K, tid);
     cur ^= 1;
-    __syncthreads();
+    lds_barrier();
     uint32_t* ao = a_out + ((size_t)chain * 2 + r) * N;
     uint32_t* co = c_out + ((size_t)chain * 2 + r) * N;
     for (int s = tid; s < N; s += 1024) {

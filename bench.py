@@ -644,7 +644,8 @@ def main():
             "traffic": traffic,
             "launch_ms": round(k1_ms, 3),
             "bytes_per_launch": bytes_streamed,
-            "bytes_note": ("the shard slabs read by one launch: 256 x 654,311,424 B (26 bits per residue); "
+            "bytes_note": ("the shard slabs read by one launch: 256 x 641,728,512 B (26 bits per residue mod p1, "
+                           "25 mod p2); "
                            "the reference's accounting of the same pass (6*N*l*32 B per shard) is "
                            "ref_bytes_per_launch"),
             "ref_bytes_per_launch": ref_bytes,

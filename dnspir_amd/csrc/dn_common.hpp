@@ -22,8 +22,15 @@ constexpr uint64_t Q = (uint64_t)P1 * P2;
 constexpr uint32_t T_PLAIN = 1025;   // plaintext modulus t
 constexpr int LOG2_B = 26;           // gadget base
 // one block of the shard slab's limb-block layout: 1024 residues as three
-// int8 digit planes (3 x 1024 B) and 64 words of 2-bit top digits
-constexpr int MB_BLOCK_WORDS = 832;
+// int8 digit planes (3 x 1024 B), then the top digits: 2 bits each mod p1
+// (64 words), 1 bit each mod p2 (64 half-words)
+constexpr int MB_BLOCK_BYTES_P1 = 3328;
+constexpr int MB_BLOCK_BYTES_P2 = 3200;
+// byte offset of block rb (counted inside modulus r) in a slab of 2 * N/8 * S
+// * 32 blocks per modulus: the p1 blocks first, then the p2 blocks
+__host__ __device__ constexpr size_t mb_offset(int r, size_t rb, int S) {
+    return r == 0 ? rb * MB_BLOCK_BYTES_P1 : (size_t)2 * (N / 8) * S * 32 * MB_BLOCK_BYTES_P1 + rb * MB_BLOCK_BYTES_P2;
+}
 
 __host__ __device__ constexpr uint32_t mod_p(int r) { return r == 0 ? P1 : P2; }
 __host__ __device__ constexpr int mod_k(int r) { return r == 0 ? 26 : 25; }

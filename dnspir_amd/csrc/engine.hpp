@@ -39,12 +39,13 @@ inline int nstreams(int l) { return l == N ? 2 : 1; }
 // the reference's slab size, (GK_DIGITS+1)*2*N*l elements of 32 B: the bytes
 // one query's inner product reads there (MEM_LOAD_DATA, base_pir.rs:1242)
 inline size_t ref_slab_bytes(int l) { return (size_t)6 * N * l * 32; }
-// bytes of this engine's slab: 26 bits per residue in the limb-block layout
-// (loop length N/2: 4 (modulus, half) x N/8 a-groups x 3*nst arrays x 32
-// j-chunks of MB_BLOCK_WORDS words), 32-bit residues in the rotated layout
+// bytes of this engine's slab: 26 bits per residue mod p1 and 25 mod p2 in
+// the limb-block layout (loop length N/2: 2 halves x N/8 a-groups x 3*nst
+// arrays x 32 j-chunks of blocks per modulus), 32-bit residues in the
+// rotated layout
 inline size_t slab_bytes(int l) {
     if (loop_len(l) != N / 2) return ref_slab_bytes(l);
-    return (size_t)4 * (N / 8) * 3 * nstreams(l) * 32 * dn::MB_BLOCK_WORDS * 4;
+    return (size_t)2 * (N / 8) * 3 * nstreams(l) * 32 * (dn::MB_BLOCK_BYTES_P1 + dn::MB_BLOCK_BYTES_P2);
 }
 
 // A bump region of device memory that a context's scratch buffers are carved

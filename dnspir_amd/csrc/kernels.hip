@@ -662,20 +662,23 @@ __global__ __launch_bounds__(256) void relayout_rot_kernel(const uint32_t* __res
 // Galois position (half, j): the closed form of main_pir_loop as a plain
 // matrix, out[a] = sum_(s,j) M2[a][s][j] Q_s[j] (s = stream*3 + array).
 //
-// Each residue x in [0, p) is stored as 26 bits: the representative
-// x' = x, or x - p when x + 0x808080 >= 2^25, written as balanced base-256
-// digits x' = d0 + 2^8 d1 + 2^16 d2 + 2^24 t with d0..d2 in [-128, 127] and
-// t in {-2, -1, 0, 1} (p < 2^26), and t kept biased as u = t + 2 in 2 bits.
-// So x = d0 + 2^8 d1 + 2^16 d2 + 2^24 u - 2^25 (mod p), and the constant
-// -2^25 sum_k Q[k] of every output is added per query at recombination.
+// Each residue x in [0, p) is stored as 26 (p1) or 25 (p2) bits: the
+// representative x' = x, or x - p when x + 0x808080 >= 2^25, written as
+// balanced base-256 digits x' = d0 + 2^8 d1 + 2^16 d2 + 2^24 t with d0..d2 in
+// [-128, 127] and t in {-2, -1, 0, 1} mod p1 (p1 < 2^26), t in {0, 1} mod p2.
+// Mod p1, t is kept biased as u = t + 2 in 2 bits, so x = d0 + 2^8 d1 +
+// 2^16 d2 + 2^24 u - 2^25 (mod p1) and the constant -2^25 sum_k Q[k] of every
+// output is added per query at recombination; mod p2, u = t in 1 bit.
 //
-// Stored as MB_BLOCK_BYTES blocks, one per (r, half, a-group g of 4, s,
-// j-chunk c of 32): byte v*1024 + l*16 + kk = digit d_v (v < 3) of the value
-// at lane l = h*32 + (a%4)*8 + lane8, j = 32c + 16h + kk (exactly the A
-// operand a 64-lane wave feeds to v_mfma_i32_32x32x32_i8), then at byte
-// 3072 + 4l one word holding that lane's 16 top digits u, value kk at bit
+// Stored as blocks of MB_BLOCK_BYTES_P1 / _P2 bytes (all p1 blocks, then all
+// p2 blocks; mb_offset), one per (r, half, a-group g of 4, s, j-chunk c of
+// 32): byte v*1024 + l*16 + kk = digit d_v (v < 3) of the value at lane
+// l = h*32 + (a%4)*8 + lane8, j = 32c + 16h + kk (exactly the A operand a
+// 64-lane wave feeds to v_mfma_i32_32x32x32_i8), then the lane's 16 top
+// digits u: mod p1 one word at byte 3072 + 4l, value kk at bit
 // 8*(kk%4) + 2*(kk/4), so (word >> 2m) & 0x03030303 is the int8 vector of
-// kk = 4m..4m+3.
+// kk = 4m..4m+3; mod p2 one half-word at byte 3072 + 2l, value kk = 4m + b
+// at bit m + {0, 8, 4, 12}[b] (top_digits1 unpacks it).
 __device__ __forceinline__ uint32_t limbs8(uint32_t x) { return (x + 0x00808080u) ^ 0x00808080u; }
 
 // the representative whose top balanced digit lies in [-2, 1], as its 4
@@ -717,10 +720,12 @@ __global__ __launch_bounds__(256) void relayout_mb_kernel(const uint32_t* __rest
             const int jl = 16 * h + kk;
             // row i = a - j + L-1  ->  tile row (a - a0) - jl + 31
             w[kk] = limbs8_top2(tile[(a - a0) - jl + 31][lane8][jl], p);
-            top |= (((w[kk] >> 24) + 2u) & 3u) << (8 * (kk & 3) + 2 * (kk >> 2));
+            if (r == 0) top |= (((w[kk] >> 24) + 2u) & 3u) << (8 * (kk & 3) + 2 * (kk >> 2));
+            else top |= ((w[kk] >> 24) & 1u) << ((kk >> 2) + (0xC480u >> (4 * (kk & 3)) & 15u));
         }
         const int g = a0 / 4 + gi;
-        uint32_t* blk = slab + ((((size_t)(r * 2 + half) * (HALFN / 4) + g) * S + s) * 32 + c) * MB_BLOCK_WORDS;
+        uint32_t* blk = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(slab) +
+                                                    mb_offset(r, (((size_t)half * (HALFN / 4) + g) * S + s) * 32 + c, S));
 #pragma unroll
         for (int v = 0; v < 3; ++v) {
             u32x4 o;
@@ -730,7 +735,8 @@ __global__ __launch_bounds__(256) void relayout_mb_kernel(const uint32_t* __rest
             o.w = ((w[12] >> (8 * v)) & 255) | (((w[13] >> (8 * v)) & 255) << 8) | (((w[14] >> (8 * v)) & 255) << 16) | (((w[15] >> (8 * v)) & 255) << 24);
             *reinterpret_cast<u32x4*>(blk + v * 256 + l * 4) = o;
         }
-        blk[768 + l] = top;
+        if (r == 0) blk[768 + l] = top;
+        else reinterpret_cast<uint16_t*>(blk + 768)[l] = (uint16_t)top;
     }
 }
 
@@ -791,9 +797,9 @@ __global__ __launch_bounds__(256) void qcorr_kernel(const uint32_t* __restrict__
         if (tid < w) part[tid] += part[tid + w];
         __syncthreads();
     }
-    if (tid == 0) {
+    if (tid == 0) {  // the p2 top digits are unbiased
         const uint64_t p = mod_p(r);
-        const uint64_t m = ((1ull << 25) % p) * (part[0] % p) % p;
+        const uint64_t m = r == 0 ? ((1ull << 25) % p) * (part[0] % p) % p : 0;
         corr[(size_t)rh * ncols + b] = (uint32_t)((p - m) % p);
     }
 }
@@ -816,7 +822,7 @@ typedef int v16i __attribute__((ext_vector_type(16)));
 typedef __attribute__((address_space(1))) const v4i gv4i;
 typedef __attribute__((address_space(1))) const uint32_t gu32;
 
-constexpr int MB_BLOCK_V4 = MB_BLOCK_WORDS / 4;  // 16-byte units per block
+typedef __attribute__((address_space(1))) const uint16_t gu16;
 
 // one K-step of the shard operand for one lane: digit planes 0..2 and the
 // packed top digits
@@ -825,22 +831,47 @@ struct ATile {
     uint32_t top;
 };
 
-// A = lane's address in the first block of its (r, half, g) row of blocks
-__device__ __forceinline__ void load_a(ATile& t, const uint32_t* slab, size_t blk0, int l, int k) {
-    gv4i* A = (gv4i*)slab + (blk0 + k) * MB_BLOCK_V4 + l;
+// the lane's blocks of one (r, half, a-group) row: first block's address,
+// block size in 16-byte units
+struct ARow {
+    const char* base;
+    int bsz16;
+};
+__device__ __forceinline__ ARow a_row(const uint32_t* slab, int r, int half, int g, int S) {
+    ARow a;
+    a.base = reinterpret_cast<const char*>(slab) + mb_offset(r, (((size_t)half * (HALFN / 4) + g) * S) * 32, S);
+    a.bsz16 = (r == 0 ? MB_BLOCK_BYTES_P1 : MB_BLOCK_BYTES_P2) / 16;
+    return a;
+}
+__device__ __forceinline__ void load_a(ATile& t, const ARow& a, int r, int l, int k) {
+    gv4i* A = (gv4i*)a.base + (size_t)k * a.bsz16 + l;
     t.d[0] = __builtin_nontemporal_load(A);
     t.d[1] = __builtin_nontemporal_load(A + 64);
     t.d[2] = __builtin_nontemporal_load(A + 128);
-    t.top = __builtin_nontemporal_load((gu32*)slab + (blk0 + k) * MB_BLOCK_WORDS + 768 + l);
+    if (r == 0) t.top = __builtin_nontemporal_load((gu32*)(A - l + 192) + l);
+    else t.top = __builtin_nontemporal_load((gu16*)(A - l + 192) + l);
 }
 
-// the four int8 vectors of a lane's top digits
+// the four int8 vectors of a lane's top digits (2 bits each, mod p1)
 __device__ __forceinline__ v4i top_digits(uint32_t w) {
     v4i o;
     o.x = (int)(w & 0x03030303u);
     o.y = (int)((w >> 2) & 0x03030303u);
     o.z = (int)((w >> 4) & 0x03030303u);
     o.w = (int)((w >> 6) & 0x03030303u);
+    return o;
+}
+// the same for 1-bit top digits (mod p2): value 4m + b at bit m + {0, 8, 4, 12}[b]
+__device__ __forceinline__ int top1_vec(uint32_t w, int m) {
+    const uint32_t t = (w >> m) & 0x1111u;
+    return (int)((t | (t << 12)) & 0x01010101u);
+}
+__device__ __forceinline__ v4i top_digits1(uint32_t w) {
+    v4i o;
+    o.x = top1_vec(w, 0);
+    o.y = top1_vec(w, 1);
+    o.z = top1_vec(w, 2);
+    o.w = top1_vec(w, 3);
     return o;
 }
 
@@ -877,8 +908,7 @@ __global__ __launch_bounds__(256) void k1b_kernel(const uint32_t* const* __restr
     const int r = blockIdx.z & 1, sh = blockIdx.z >> 1;
     const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
     const int g = blockIdx.x * 4 + wave;
-    const uint32_t* slab = slabs[sh];
-    const size_t blk0 = (((size_t)(r * 2 + half) * (HALFN / 4) + g) * S) * 32;
+    const ARow arow = a_row(slabs[sh], r, half, g, S);
     gv4i* Q = (gv4i*)ql + ((size_t)(r * 2 + half) * S) * 32 * CT * 256 + tid;
     const int nk = S * 32;
     v16i acc[7][CT];
@@ -895,7 +925,7 @@ __global__ __launch_bounds__(256) void k1b_kernel(const uint32_t* const* __restr
     for (int i = 0; i < CT; ++i) (&Bt[0][0][0][0])[tid + 256 * i] = Q[256 * i];
 #pragma unroll
     for (int d = 0; d < PD; ++d) {
-        load_a(abuf[d], slab, blk0, l, d);
+        load_a(abuf[d], arow, r, l, d);
         const int kb = min(d + 1, nk - 1);
 #pragma unroll
         for (int i = 0; i < CT; ++i) bq[d][i] = Q[(size_t)kb * CT * 256 + 256 * i];
@@ -913,14 +943,14 @@ __global__ __launch_bounds__(256) void k1b_kernel(const uint32_t* const* __restr
             a[0] = abuf[d].d[0];
             a[1] = abuf[d].d[1];
             a[2] = abuf[d].d[2];
-            a[3] = top_digits(abuf[d].top);
+            a[3] = r == 0 ? top_digits(abuf[d].top) : top_digits1(abuf[d].top);
             v4i b[CT][4];
 #pragma unroll
             for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) b[ct][v] = Bt[kk & 1][ct][v][l];
             const int ka = min(kk + PD, nk - 1), kb = min(kk + PD + 1, nk - 1);
-            load_a(abuf[d], slab, blk0, l, ka);
+            load_a(abuf[d], arow, r, l, ka);
 #pragma unroll
             for (int i = 0; i < CT; ++i) bq[d][i] = Q[(size_t)kb * CT * 256 + 256 * i];
 #pragma unroll
@@ -1504,8 +1534,7 @@ __global__ __launch_bounds__(256) void k1d_kernel(const uint32_t* const* __restr
     }
     __syncthreads();
     const int g = blockIdx.x * 4 + wave;
-    const uint32_t* slab = slabs[sh];
-    const size_t blk0 = (((size_t)(r * 2 + half) * (HALFN / 4) + g) * S) * 32;
+    const ARow arow = a_row(slabs[sh], r, half, g, S);
     const int nk = S * 32;
     int acc[NQ][7];
 #pragma unroll
@@ -1514,7 +1543,7 @@ __global__ __launch_bounds__(256) void k1d_kernel(const uint32_t* const* __restr
         for (int w = 0; w < 7; ++w) acc[q][w] = 0;
     ATile abuf[PD];
 #pragma unroll
-    for (int d = 0; d < PD; ++d) load_a(abuf[d], slab, blk0, l, d);
+    for (int d = 0; d < PD; ++d) load_a(abuf[d], arow, r, l, d);
     for (int kk0 = 0; kk0 < nk; kk0 += PD) {
 #pragma unroll
         for (int d = 0; d < PD; ++d) {
@@ -1523,8 +1552,8 @@ __global__ __launch_bounds__(256) void k1d_kernel(const uint32_t* const* __restr
             a[0] = abuf[d].d[0];
             a[1] = abuf[d].d[1];
             a[2] = abuf[d].d[2];
-            a[3] = top_digits(abuf[d].top);
-            if (kk + PD < nk) load_a(abuf[d], slab, blk0, l, kk + PD);
+            a[3] = r == 0 ? top_digits(abuf[d].top) : top_digits1(abuf[d].top);
+            if (kk + PD < nk) load_a(abuf[d], arow, r, l, kk + PD);
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
                 u32x4 b[4];

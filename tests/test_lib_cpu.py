@@ -37,9 +37,10 @@ def test_shape_rule_matches_oracle(n):
 
 def test_reply_bytes_and_required_bytes():
     assert pir.reply_bytes() == 22272
-    # 26 bits per residue: 13/16 of the reference's 6*N*l*32 B slab
-    assert _lib.lib().dnspir_required_bytes(2048) == 805306368 * 13 // 16 + 196608
-    assert _lib.lib().dnspir_required_bytes(1024) == 402653184 * 13 // 16 + 196608
+    # 26 bits per residue mod p1 and 25 mod p2: 51/64 of the reference's
+    # 6*N*l*32 B slab (3328- and 3200-byte blocks of 1024 residues)
+    assert _lib.lib().dnspir_required_bytes(2048) == 805306368 * 51 // 64 + 196608
+    assert _lib.lib().dnspir_required_bytes(1024) == 402653184 * 51 // 64 + 196608
     assert _lib.lib().dnspir_required_bytes(512) == 6 * 2048 * 512 * 32 + 196608
     assert _lib.lib().dnspir_required_bytes(1536) == 0  # not a divisor of N
     assert _lib.lib().dnspir_required_bytes(2048 * 3 // 4) == 0

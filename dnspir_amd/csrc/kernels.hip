@@ -808,8 +808,8 @@ __global__ __launch_bounds__(256) void qcorr_kernel(const uint32_t* __restrict__
 // K1 (single, split and batched queries) on the limb-block layout: for each
 // (shard, r, half) the product OUT[(a, lane)][b] = sum_(s,j) M2[(a,lane)][(s,j)] Q_b[(s,j)]
 // on int8 MFMA with exact limb decomposition: the shard operand is split into
-// 3 signed base-256 digits and the biased top digit u in [0, 3], the query
-// into 4 signed digits; the 16 digit products are summed in int32 per weight
+// 3 signed base-256 digits and the top digit u (biased, in [0, 3], mod p1; 0
+// or 1 mod p2), the query into 4 signed digits; the 16 digit products are summed in int32 per weight
 // w = u + v (|sum| <= 4 * 6144 * 2^14 < 2^29), and
 // OUT = sum_w 2^(8w) ACC_w + corr_b mod p. One wave per a-group (32 output
 // rows), CT column tiles of 32 queries. Writes outb[q][shard][r][gpos][lane].

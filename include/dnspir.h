@@ -51,7 +51,8 @@ typedef struct {
     int k1_launches;         /* number of those launches */
     uint64_t ref_bytes;      /* the same pass in the reference's accounting, 6*N*l*32 B per
                               * shard (MEM_LOAD_DATA, base_pir.rs:1242); the slabs here keep
-                              * 26 bits per residue, so bytes_streamed is 13/16 of it */
+                              * 26 bits per residue mod p1 and 25 mod p2, so
+                              * bytes_streamed is 51/64 of it */
 } dnspir_timing;
 
 /* Initialise the library on HIP device `device` (builds the NTT / Galois
@@ -65,8 +66,9 @@ const char* dnspir_last_error(void);
 
 /* Bytes of caller-owned device memory a shard of index-group order
  * `group_order` needs (replaces PIRDatabase::required_memory_general * 32 B,
- * base_pir.rs:369-371): the slab, which keeps 26 bits per residue for
- * group orders N/2 and N (13/16 of the reference's size) and 32 bits below,
+ * base_pir.rs:369-371): the slab, which keeps 26 bits per residue mod p1 and
+ * 25 mod p2 for group orders N/2 and N (51/64 of the reference's size) and
+ * 32 bits below,
  * plus 192 KiB for the cached a-part. */
 size_t dnspir_required_bytes(int group_order);
 /* PIRDatabase::create (base_pir.rs:280-349). used_seeds: 0 = FirstSet,

@@ -369,13 +369,13 @@ __global__ __launch_bounds__(256) void k5_rows_kernel(uint32_t* __restrict__ M, 
         dtw[t] = T->psipow[0][e];
         dtwq[t] = T->psipowq[0][e];
     }
-    for (int idx = tid; idx < l * TS; idx += 256) {
-        const int j = idx / TS, d = idx % TS;
-        const int s = s0 + d;
-        const int z = (T->slot_exp[s] * kq) & (M2N - 1);
-        const int coset = z & ((1 << lstep) - 1);
+    // each thread keeps one slot of the tile (256 is a multiple of TS)
+    const int d = tid % TS, s = s0 + d;
+    const int z = (T->slot_exp[s] * kq) & (M2N - 1);
+    const int coset = z & ((1 << lstep) - 1);
+    for (int j = tid / TS; j < l; j += 256 / TS) {
         const int e = (M2N - ((coset * j) & (M2N - 1))) & (M2N - 1);
-        tile[idx] = mul_shoup(base[(size_t)j * N + s], T->psipow[0][e], T->psipowq[0][e], P1);
+        tile[j * TS + d] = mul_shoup(base[(size_t)j * N + s], T->psipow[0][e], T->psipowq[0][e], P1);
     }
     __syncthreads();
     int top = logl - 1;
@@ -387,11 +387,7 @@ __global__ __launch_bounds__(256) void k5_rows_kernel(uint32_t* __restrict__ M, 
     if (top == 1) k5_dif_pass<TS, 2>(tile, logl, top, dtw, dtwq, tid);
     else if (top == 0) k5_dif_pass<TS, 1>(tile, logl, top, dtw, dtwq, tid);
     if (top >= 0) __syncthreads();
-    for (int idx = tid; idx < l * TS; idx += 256) {
-        const int i = idx / TS, d = idx % TS;
-        const int s = s0 + d;
-        const int z = (T->slot_exp[s] * kq) & (M2N - 1);
-        const int coset = z & ((1 << lstep) - 1);
+    for (int i = tid / TS; i < l; i += 256 / TS) {
         int y;
         if (conj && i >= N / 2) y = (z * T->pow5[i - N / 2]) & (M2N - 1);
         else y = ((M2N - z) * T->pow5[i]) & (M2N - 1);
@@ -508,13 +504,11 @@ __global__ __launch_bounds__(1024) void chain_kernel(const uint32_t* __restrict_
     sig_upd[0] = T->sigma5[tid];
     sig_upd[1] = T->sigma5[tid + 1024];
     int gpos_upd[2] = {T->slot_gpos[tid], T->slot_gpos[tid + 1024]};
-    // phase A input gather indices: radix-4 groups g, g + 256 of modulus rA
-    const int rA = (tid >> 8) & 1, gA = tid & 255;
-    int sigA[8];
+    // phase A: 512 threads per modulus rA, radix-4 group gA first (stages 10, 9)
+    const int rA = tid >> 9, gA = tid & 511;
+    int sigA[4];
 #pragma unroll
-    for (int k = 0; k < 2; ++k)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) sigA[4 * k + c] = T->sigma5[4 * (gA + 256 * k) + c];
+    for (int c = 0; c < 4; ++c) sigA[c] = T->sigma5[4 * gA + c];
     const int q4 = tid >> 8, rC = q4 & 1, gC = tid & 255;
     lds_barrier();
 
@@ -522,30 +516,32 @@ __global__ __launch_bounds__(1024) void chain_kernel(const uint32_t* __restrict_
     int cur = 0;
 #pragma unroll 1
     for (int i = 0; i < L; ++i) {
-        // ---- A: inverse NTTs of sigma_5(a) ----
-        if (tid < 512) {
+        // ---- A: inverse NTTs of sigma_5(a) (512 threads per modulus, radix 4) ----
+        {
             const uint32_t p = mod_p(rA);
             const uint32_t* itw = tw[1][rA];
             const uint32_t* itwq = twq[1][rA];
-#pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                const int ii = gA + 256 * k;
-                uint32_t v0 = sa[rA][sigA[4 * k]], v1 = sa[rA][sigA[4 * k + 1]];
-                uint32_t v2 = sa[rA][sigA[4 * k + 2]], v3 = sa[rA][sigA[4 * k + 3]];
-                bf_gs(v0, v1, itw[1024 + 2 * ii], itwq[1024 + 2 * ii], p);
-                bf_gs(v2, v3, itw[1025 + 2 * ii], itwq[1025 + 2 * ii], p);
-                const uint32_t w = itw[512 + ii], wq = itwq[512 + ii];
-                bf_gs(v0, v2, w, wq, p);
-                bf_gs(v1, v3, w, wq, p);
-                uint32_t* b = xb[rA];
-                const int b4 = lx<true>(4 * ii);
-                b[b4] = v0; b[b4 + 1] = v1; b[b4 + 2] = v2; b[b4 + 3] = v3;
-            }
+            const int ii = gA;
+            uint32_t v0 = sa[rA][sigA[0]], v1 = sa[rA][sigA[1]], v2 = sa[rA][sigA[2]], v3 = sa[rA][sigA[3]];
+            bf_gs(v0, v1, itw[1024 + 2 * ii], itwq[1024 + 2 * ii], p);
+            bf_gs(v2, v3, itw[1025 + 2 * ii], itwq[1025 + 2 * ii], p);
+            const uint32_t w = itw[512 + ii], wq = itwq[512 + ii];
+            bf_gs(v0, v2, w, wq, p);
+            bf_gs(v1, v3, w, wq, p);
+            uint32_t* b = xb[rA];
+            const int b4 = lx<true>(4 * ii);
+            b[b4] = v0; b[b4 + 1] = v1; b[b4 + 2] = v2; b[b4 + 3] = v3;
         }
         lds_barrier();
-#pragma unroll 1
-        for (int P = 2; P >= 0; --P) {
-            if (tid < 512) inv_pass8<true>(xb[rA], tw[1][rA], twq[1][rA], mod_p(rA), P, gA);
+        {
+            int gAx = gA;
+            asm volatile("" : "+v"(gAx));  // pass addresses recomputed per step (VGPRs)
+#pragma unroll
+            for (int t = 7; t >= 1; t -= 2) {
+                inv_pass4g<true>(xb[rA], tw[1][rA], twq[1][rA], mod_p(rA), t, gAx);
+                lds_barrier();
+            }
+            inv_pass2_last<true>(xb[rA], tw[1][rA], twq[1][rA], mod_p(rA), gAx);
             lds_barrier();
         }
         // ---- B: CRT, centered lift, digit D1 (base_pir.rs:544-570) ----

@@ -172,8 +172,9 @@ struct dnspir_ctx {
 namespace dne {
 
 int ctx_init(dnspir_ctx* c, hipStream_t s);
-// chains at most this many run on two CUs each (chain_pair_kernel)
-constexpr int PAIR_MAX_CHAINS = 32;
+// chains at most this many run on two CUs each (chain_pair_kernel): up to one
+// block per CU of the 256
+constexpr int PAIR_MAX_CHAINS = 128;
 // after the stream sync that follows a call: DNSPIR_E_HIP if that call's
 // two-CU chain timed out waiting for a partner block
 int check_pair(dnspir_ctx* c);

@@ -74,11 +74,19 @@ def cpu_baseline(lp, nshards, ls, query_block, threads):
     outs, pu, su = orc.double_pir([sh, sh], None, query_block, None, ls)
     per_shard_ms = pu / 1e3 / 2
     ms = nshards * per_shard_ms + su / 1e3
+    # the same on one thread (one fleet entry)
+    orc.set_threads(1)
+    _, pu1, su1 = orc.double_pir([sh], None, query_block, None, ls)
+    orc.set_threads(threads)
+    ms1 = nshards * pu1 / 1e3 + su1 / 1e3
     return {
         "value": round(ms, 1), "unit": "ms/query", "cores": threads, "kind": "port",
         "sample": (f"oracle (C, OpenMP) on this host: primary phase over 2 full shards "
                    f"({per_shard_ms:.1f} ms/shard, x{nshards}) + one secondary phase at l_s={ls} "
                    f"({su / 1e3:.1f} ms); shard preprocessing {t_pre:.1f} s not counted"),
+        "one_thread": {"value": round(ms1, 1), "unit": "ms/query", "cores": 1,
+                       "sample": (f"primary phase over 1 full shard ({pu1 / 1e3:.1f} ms, x{nshards}) + one "
+                                  f"secondary phase at l_s={ls} ({su1 / 1e3:.1f} ms)")},
     }
 
 

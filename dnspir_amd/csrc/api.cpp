@@ -316,7 +316,8 @@ int run_secondary(dnspir_ctx* c, const uint16_t* entries, long long col_stride, 
         const size_t words = chain_pair_xbuf_words(nst_s * nlanes);
         if ((rc = c->xch.ensure((words + 64) * 4))) return rc;
         launch_chain_pair(c->te.as<uint32_t>(), c->aout.as<uint32_t>(), c->cout.as<uint32_t>(), c->qs.as<uint32_t>(), d_slot,
-                          L_s, nst_s, nlanes, c->xch.as<uint32_t>() + 64, c->xch.as<unsigned>(), g.tables, s);
+                          L_s, nst_s, nlanes, c->xch.as<uint32_t>() + 64, c->xch.as<unsigned>(), c->exchange_polls,
+                          g.tables, s);
     } else {
         launch_chain_sec(c->te.as<uint32_t>(), c->aout.as<uint32_t>(), c->cout.as<uint32_t>(), c->qs.as<uint32_t>(), d_slot,
                          L_s, nst_s, nlanes, g.tables, s);
@@ -659,6 +660,14 @@ int dnspir_ctx_set_graphs(dnspir_ctx* c, int on) {
     std::lock_guard<std::mutex> lk(c->mu);
     c->graphs = on != 0;
     return c->graphs_broken ? 1 : 0;
+}
+
+int dnspir_ctx_set_exchange_polls(dnspir_ctx* c, int max_polls) {
+    if (!c) return DNSPIR_E_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->drop_graphs();  // the bound is baked into captured launches
+    c->exchange_polls = max_polls;
+    return 0;
 }
 
 int dnspir_choose_index_group_secondary(size_t num_shards) {

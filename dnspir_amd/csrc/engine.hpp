@@ -131,6 +131,7 @@ struct dnspir_ctx {
     // exchange words and timeout flag)
     dne::DevBuf ptrs, idx, keyp, xch;
     unsigned* h_err = nullptr;  // pinned copy of the chain timeout flag
+    int exchange_polls = 1 << 21;  // bound of each exchange wait of the two-CU chain
     bool pair_pending = false;  // h_err is to be checked after the next sync
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // start, primary end, end, k1 start, k1 end
     int k1_launches = 0;

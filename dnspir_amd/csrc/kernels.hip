@@ -1427,6 +1427,10 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
         for (int k = 0; k < 2; ++k) {
             const int j = tid + 1024 * k;
             uint32_t u = __hip_atomic_load(theirs + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (spin_limit <= 0) {  // no waiting at all (exercises the timeout path)
+                atomicOr(err, 1u);
+                u = tag;
+            }
             int n = 0;
             while ((u & 0xF8000000u) != tag) {
                 // give up at the limit, or soon after any wait has (later steps then pass quickly)
@@ -1499,11 +1503,11 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
 size_t chain_pair_xbuf_words(int nchains) { return (size_t)nchains * 4 * N; }
 
 void launch_chain_pair(const uint32_t* te_tmp, uint32_t* a_out, uint32_t* c_out, const uint32_t* qs, const int* lane_slot,
-                       int L, int nst, int nlanes, uint32_t* xbuf, unsigned* err, const DevTables* T, hipStream_t s) {
+                       int L, int nst, int nlanes, uint32_t* xbuf, unsigned* err, int spin_limit, const DevTables* T,
+                       hipStream_t s) {
     const int nchains = nst * nlanes;
     launch_fill_u32(xbuf, chain_pair_xbuf_words(nchains), 0, s);
     launch_fill_u32(err, 1, 0, s);
-    const int spin_limit = 1 << 21;
     hipLaunchKernelGGL(chain_pair_kernel, dim3(16 * ((nchains + 7) / 8)), dim3(1024), 0, s, te_tmp, a_out, c_out, qs,
                        lane_slot, L, nlanes, nchains, xbuf, err, spin_limit, T);
 }

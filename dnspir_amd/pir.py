@@ -63,7 +63,10 @@ class PIRDatabase:
     def __del__(self):
         h = getattr(self, "h", None)
         if h is not None and h.value:
-            lib().dnspir_shard_destroy(h)
+            try:
+                lib().dnspir_shard_destroy(h)
+            except Exception:  # interpreter shutdown: the library is already gone
+                pass
             self.h = None
 
     @property
@@ -130,10 +133,18 @@ class Context:
             raise DnspirError(rc, "ctx_set_graphs")
         return rc == 1
 
+    def set_exchange_polls(self, max_polls):
+        """Bound of each partner wait of the two-CU secondary chain (0: give
+        up at once, which makes the call fail with DNSPIR_E_HIP)."""
+        check(lib().dnspir_ctx_set_exchange_polls(self.h, int(max_polls)), "ctx_set_exchange_polls")
+
     def __del__(self):
         h = getattr(self, "h", None)
         if h is not None and h.value:
-            lib().dnspir_ctx_destroy(h)
+            try:
+                lib().dnspir_ctx_destroy(h)
+            except Exception:  # interpreter shutdown: the library is already gone
+                pass
             self.h = None
 
 
@@ -272,7 +283,10 @@ class Rng:
     def __del__(self):
         h = getattr(self, "h", None)
         if h:
-            lib().dnspir_rng_free(h)
+            try:
+                lib().dnspir_rng_free(h)
+            except Exception:  # interpreter shutdown: the library is already gone
+                pass
             self.h = None
 
 
@@ -388,7 +402,10 @@ class Server:
             self.h = None
 
     def __del__(self):
-        self.close()
+        try:
+            self.close()
+        except Exception:  # interpreter shutdown: the library is already gone
+            pass
 
 
 def process_reply(reply, sk_seed):

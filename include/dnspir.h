@@ -108,6 +108,12 @@ void* dnspir_ctx_stream(dnspir_ctx* ctx);
  * Returns 1 if capture failed once on this context (it then launches
  * eagerly), else 0. */
 int dnspir_ctx_set_graphs(dnspir_ctx* ctx, int on);
+/* Few-query calls run each secondary chain on two CUs that exchange data
+ * every step; each wait for the partner CU ends after `max_polls` polls
+ * (default 2^21, about a second) and the call then fails with DNSPIR_E_HIP.
+ * 0 makes every wait give up at once (to exercise that error path). No
+ * counterpart in the reference. */
+int dnspir_ctx_set_exchange_polls(dnspir_ctx* ctx, int max_polls);
 
 /* ---- two-level composition: perform_batched_double_pir (double_pir.rs:126-228)
  * fleets: random-access shard lists (shards may repeat). q0/q1: query blocks

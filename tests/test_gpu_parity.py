@@ -183,3 +183,26 @@ def test_error_paths(pir):
     with pytest.raises(DnspirError) as ei:
         pir.process_queries([g], None, q[:-1], None)
     assert ei.value.code == -1
+
+
+def test_chain_exchange_timeout_is_an_error(pir):
+    """The two-CU secondary chain bounds every wait for its partner block: with
+    the bound forced to zero from the host the call fails with DNSPIR_E_HIP
+    (every kernel still completes), and the same context answers exactly
+    afterwards."""
+    from dnspir_amd import DnspirError, lib
+    g, o, e = _shard(pir, 1024, 1, 0)
+    rng = pir.Rng(bytes([9] * 32))
+    q, sk = pir.prepare_query(rng, 123, 0, 1024, 8)
+    ctx = pir.Context()
+    ctx.set_exchange_polls(0)
+    _say("[parity] chain exchange bound 0: expect DNSPIR_E_HIP")
+    with pytest.raises(DnspirError) as ei:
+        pir.process_queries([g], None, q, None, ctx=ctx)
+    assert ei.value.code == -8
+    assert b"timed out" in lib().dnspir_last_error()
+    ctx.set_exchange_polls(1 << 21)
+    reps, _ = pir.process_queries([g], None, q, None, ctx=ctx)
+    oreps, _, _ = orc.process_queries([o], None, q, None)
+    assert reps[0] == oreps[0]
+    assert np.array_equal(pir.process_reply(reps[0], sk), e[123])

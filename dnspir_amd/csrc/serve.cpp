@@ -65,6 +65,8 @@ struct Database {
     std::vector<size_t> need;  // need[nb]: scratch bytes of a batch of nb
     bool streaming = false;    // a batch is in its primary phase (reading the shards)
     double pass_us = 0;        // running mean of the primary phase's duration (0 = none yet)
+    double last_arrival = 0;   // submit time of the latest request
+    double gap_us = 0;         // running mean of the time between its requests (0 = none yet)
 };
 
 struct KeyEntry {
@@ -235,9 +237,12 @@ struct dnspir_server {
             // batching window: while other batches run, a database whose pass
             // streams for long lets its oldest request wait up to window x
             // pass time for more requests (each pass costs the same HBM bytes
-            // whatever its batch size); an idle GPU starts at once
-            if (inflight > 0 && cfg.batch_window > 0 && d.pass_us > 0 && (int)pick.size() < d.max_batch) {
-                const double due = r->t_submit + cfg.batch_window * d.pass_us;
+            // whatever its batch size) if at least one more request is
+            // expected in that time at its recent arrival rate; an idle GPU
+            // starts at once
+            const double hold = cfg.batch_window * d.pass_us;
+            if (inflight > 0 && hold > 0 && d.gap_us > 0 && hold >= d.gap_us && (int)pick.size() < d.max_batch) {
+                const double due = r->t_submit + hold;
                 if (now_us() < due) {
                     wake_at = wake_at == 0 ? due : std::min(wake_at, due);
                     ++it;
@@ -424,6 +429,11 @@ struct dnspir_server {
         }
         r->ticket = next_ticket++;
         r->t_submit = now_us();
+        if (d.last_arrival > 0) {  // arrival rate of this database (batching window)
+            const double gap = r->t_submit - d.last_arrival;
+            d.gap_us = d.gap_us == 0 ? gap : 0.9 * d.gap_us + 0.1 * gap;
+        }
+        d.last_arrival = r->t_submit;
         Request* raw = r.release();
         reqs[raw->ticket] = raw;
         pending.push_back(raw);

@@ -474,7 +474,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serve-rates", default="50,200,500,1000")
-    ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--serve-seconds", type=float, default=6.0)
     ap.add_argument("--serve-clients", type=int, default=1000)
     ap.add_argument("--serve-small-entries", type=int, default=32768)
     ap.add_argument("--serve-slots", type=int, default=4)

@@ -457,6 +457,9 @@ __global__ __launch_bounds__(256) void k5_cols_final_kernel(const uint32_t* __re
 // ---------------------------------------------------------------------------
 constexpr uint32_t NINV_Q2INV_P1 = 67055624u;  // N^-1 p2^-1 mod p1
 constexpr uint32_t NINV_Q1INV_P2 = 8189u;      // N^-1 p1^-1 mod p2
+// their Shoup companions floor(w 2^32 / p)
+constexpr uint32_t NINV_Q2INV_P1_SH = (uint32_t)(((uint64_t)NINV_Q2INV_P1 << 32) / P1);
+constexpr uint32_t NINV_Q1INV_P2_SH = (uint32_t)(((uint64_t)NINV_Q1INV_P2 << 32) / P2);
 
 template <bool SEC>
 __global__ __launch_bounds__(1024) void chain_kernel(const uint32_t* __restrict__ te_tmp, uint32_t* __restrict__ ip_tmp,
@@ -549,11 +552,9 @@ __global__ __launch_bounds__(1024) void chain_kernel(const uint32_t* __restrict_
         for (int k = 0; k < 2; ++k) {
             const int j = tid + 1024 * k;
             const int xj = lx<true>(j);
-            uint32_t x1 = xb[0][xj], x2 = xb[1][xj];
-            x1 = x1 >= P1 ? x1 - P1 : x1;
-            x2 = x2 >= P2 ? x2 - P2 : x2;
-            const uint32_t e1 = mul_mod(x1, NINV_Q2INV_P1, P1, 26);
-            const uint32_t e2 = mul_mod(x2, NINV_Q1INV_P2, P2, 25);
+            const uint32_t x1 = xb[0][xj], x2 = xb[1][xj];  // [0, 2p)
+            const uint32_t e1 = mul_shoup(x1, NINV_Q2INV_P1, NINV_Q2INV_P1_SH, P1);
+            const uint32_t e2 = mul_shoup(x2, NINV_Q1INV_P2, NINV_Q1INV_P2_SH, P2);
             uint64_t v = (uint64_t)e1 * P2 + (uint64_t)e2 * P1;
             v = v >= Q ? v - Q : v;
             const int64_t x = center_q(v);
@@ -1457,11 +1458,9 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
         for (int k = 0; k < 2; ++k) {
             const int j = tid + 1024 * k;
             const int xj = lx<true>(j);
-            uint32_t x1 = r == 0 ? own[k] : oth[k], x2 = r == 0 ? oth[k] : own[k];
-            x1 = x1 >= P1 ? x1 - P1 : x1;
-            x2 = x2 >= P2 ? x2 - P2 : x2;
-            const uint32_t e1 = mul_mod(x1, NINV_Q2INV_P1, P1, 26);
-            const uint32_t e2 = mul_mod(x2, NINV_Q1INV_P2, P2, 25);
+            const uint32_t x1 = r == 0 ? own[k] : oth[k], x2 = r == 0 ? oth[k] : own[k];  // [0, 2p)
+            const uint32_t e1 = mul_shoup(x1, NINV_Q2INV_P1, NINV_Q2INV_P1_SH, P1);
+            const uint32_t e2 = mul_shoup(x2, NINV_Q1INV_P2, NINV_Q1INV_P2_SH, P2);
             uint64_t v = (uint64_t)e1 * P2 + (uint64_t)e2 * P1;
             v = v >= Q ? v - Q : v;
             const int64_t x = center_q(v);

@@ -888,6 +888,35 @@ __device__ __forceinline__ const uint32_t* qcorr_ptr(const void* ql, int S, int 
     return reinterpret_cast<const uint32_t*>(ql) + (size_t)4 * S * 32 * CT * 1024;
 }
 
+// k1b epilogue: the seven weight accumulators of each of the lane's 16 rows
+// per column tile combined with the digit weights and the query's top-digit
+// correction, reduced mod p and stored as the 32-bit per-shard result.
+template <int CT>
+__device__ __forceinline__ void k1b_store(const v16i (&acc)[7][CT], const void* ql, uint32_t* __restrict__ outb, int S, int r,
+                                          int half, int sh, int g, int l, int nq, size_t q_stride) {
+    const uint32_t p = mod_p(r);
+    const int K = mod_k(r);
+    uint32_t cw[7];
+    weight_consts(p, cw);
+    const uint32_t* corr = qcorr_ptr(ql, S, CT) + (size_t)(r * 2 + half) * CT * 32;
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct) {
+        const int q = ct * 32 + (l & 31);
+        if (q >= nq) continue;
+        uint32_t* o = outb + (size_t)q * q_stride + ((size_t)sh * 2 + r) * N * LANES;
+        const int64_t cq = corr[q];
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            int64_t t = cq;
+#pragma unroll
+            for (int w = 0; w < 7; ++w) t += (int64_t)acc[w][ct][e] * cw[w];
+            const int row = 8 * (e >> 2) + 4 * (l >> 5) + (e & 3);
+            const int a = g * 4 + (row >> 3), lane8 = row & 7;
+            o[(half * HALFN + a) * LANES + lane8] = from_i64(t, p, K);
+        }
+    }
+}
+
 // k1b: one wave per a-group, CT column tiles per wave. The shard tiles are
 // prefetched PD K-steps ahead into a register ring; the query tiles are
 // loaded PD steps ahead into a register ring too and written to a 2-stage LDS
@@ -959,27 +988,151 @@ __global__ __launch_bounds__(256) void k1b_kernel(const uint32_t* const* __restr
                         acc[u + v][ct] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[u], b[ct][v], acc[u + v][ct], 0, 0, 0);
         }
     }
-    const uint32_t p = mod_p(r);
-    const int K = mod_k(r);
-    uint32_t cw[7];
-    weight_consts(p, cw);
-    const uint32_t* corr = qcorr_ptr(ql, S, CT) + (size_t)(r * 2 + half) * CT * 32;
+    k1b_store<CT>(acc, ql, outb, S, r, half, sh, g, l, nq, q_stride);
+}
+
+// 3-slot LDS ring: the query tile of step k+1 is read from LDS into a second
+// register set before step k's MFMAs issue, so the matrix pipe never waits on
+// an LDS read after the step's barrier. At step k: slot (k+1)%3 was written
+// before the previous barrier; slot (k+2)%3, which receives tile k+2 from the
+// register ring, held tile k-1, read at step k-2, and the barrier after that
+// step (each barrier waits for the wave's LDS reads) separates the read from
+// this write. U is a multiple of 2, 3 and PD, so every slot index and register
+// set is a compile-time constant (the K extent 96 * nst is a multiple of U).
+template <int CT, int PD, int U>
+__global__ __launch_bounds__(256) void k1b_ring_kernel(const uint32_t* const* __restrict__ slabs, const v4i* __restrict__ ql,
+                                                       uint32_t* __restrict__ outb, int nst, int nq, size_t q_stride) {
+    static_assert(U % PD == 0 && U % 6 == 0, "U a multiple of PD, 2 and 3");
+    __shared__ v4i Bt[3][CT][4][64];
+    const int S = 3 * nst;
+    const int half = blockIdx.y;
+    const int r = blockIdx.z & 1, sh = blockIdx.z >> 1;
+    const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
+    const int g = blockIdx.x * 4 + wave;
+    const ARow arow = a_row(slabs[sh], r, half, g, S);
+    gv4i* Q = (gv4i*)ql + ((size_t)(r * 2 + half) * S) * 32 * CT * 256 + tid;
+    const int nk = S * 32;
+    v16i acc[7][CT];
 #pragma unroll
-    for (int ct = 0; ct < CT; ++ct) {
-        const int q = ct * 32 + (l & 31);
-        if (q >= nq) continue;
-        uint32_t* o = outb + (size_t)q * q_stride + ((size_t)sh * 2 + r) * N * LANES;
-        const int64_t cq = corr[q];
+    for (int w = 0; w < 7; ++w)
 #pragma unroll
-        for (int e = 0; e < 16; ++e) {
-            int64_t t = cq;
+        for (int ct = 0; ct < CT; ++ct)
 #pragma unroll
-            for (int w = 0; w < 7; ++w) t += (int64_t)acc[w][ct][e] * cw[w];
-            const int row = 8 * (e >> 2) + 4 * (l >> 5) + (e & 3);
-            const int a = g * 4 + (row >> 3), lane8 = row & 7;
-            o[(half * HALFN + a) * LANES + lane8] = from_i64(t, p, K);
+            for (int e = 0; e < 16; ++e) acc[w][ct][e] = 0;
+    ATile abuf[PD];
+    v4i bq[PD][CT];
+    // tiles 0 and 1 straight to LDS; tiles 2..PD+1 in the register ring
+#pragma unroll
+    for (int i = 0; i < CT; ++i) {
+        (&Bt[0][0][0][0])[tid + 256 * i] = Q[256 * i];
+        (&Bt[1][0][0][0])[tid + 256 * i] = Q[(size_t)CT * 256 + 256 * i];
+    }
+#pragma unroll
+    for (int d = 0; d < PD; ++d) {
+        load_a(abuf[d], arow, r, l, d);
+        const int kb = min(d + 2, nk - 1);
+#pragma unroll
+        for (int i = 0; i < CT; ++i) bq[d][i] = Q[(size_t)kb * CT * 256 + 256 * i];
+    }
+    __syncthreads();
+    v4i b[2][CT][4];
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) b[0][ct][v] = Bt[0][ct][v][l];
+    for (int kk0 = 0; kk0 < nk; kk0 += U) {
+#pragma unroll
+        for (int dd = 0; dd < U; ++dd) {
+            const int d = dd % PD, cur = dd & 1, nxt = cur ^ 1;
+            const int kk = kk0 + dd;
+            // tile kk+1 -> registers (slot (dd+1)%3), tile kk+2 -> slot (dd+2)%3
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) b[nxt][ct][v] = Bt[(dd + 1) % 3][ct][v][l];
+#pragma unroll
+            for (int i = 0; i < CT; ++i) (&Bt[(dd + 2) % 3][0][0][0])[tid + 256 * i] = bq[d][i];
+            v4i a[4];
+            a[0] = abuf[d].d[0];
+            a[1] = abuf[d].d[1];
+            a[2] = abuf[d].d[2];
+            a[3] = r == 0 ? top_digits(abuf[d].top) : top_digits1(abuf[d].top);
+            const int ka = min(kk + PD, nk - 1), kb = min(kk + PD + 2, nk - 1);
+            load_a(abuf[d], arow, r, l, ka);
+#pragma unroll
+            for (int i = 0; i < CT; ++i) bq[d][i] = Q[(size_t)kb * CT * 256 + 256 * i];
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v)
+                        acc[u + v][ct] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[u], b[cur][ct][v], acc[u + v][ct], 0, 0, 0);
+            __syncthreads();
         }
     }
+    k1b_store<CT>(acc, ql, outb, S, r, half, sh, g, l, nq, q_stride);
+}
+
+// No LDS and no barriers: every wave loads the query tiles itself, PB steps
+// ahead in a register ring (the four waves of a block read the same tile, so
+// three of the four reads hit the CU's vector cache). The waves of a block
+// run free of each other.
+template <int CT, int PD, int PB, int U>
+__global__ __launch_bounds__(256) void k1b_direct_kernel(const uint32_t* const* __restrict__ slabs, const v4i* __restrict__ ql,
+                                                         uint32_t* __restrict__ outb, int nst, int nq, size_t q_stride) {
+    static_assert(U % PD == 0 && U % PB == 0, "U a multiple of PD and PB");
+    const int S = 3 * nst;
+    const int half = blockIdx.y;
+    const int r = blockIdx.z & 1, sh = blockIdx.z >> 1;
+    const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63;
+    const int g = blockIdx.x * 4 + wave;
+    const ARow arow = a_row(slabs[sh], r, half, g, S);
+    gv4i* Q = (gv4i*)ql + ((size_t)(r * 2 + half) * S) * 32 * CT * 256 + l;
+    const int nk = S * 32;
+    v16i acc[7][CT];
+#pragma unroll
+    for (int w = 0; w < 7; ++w)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+            for (int e = 0; e < 16; ++e) acc[w][ct][e] = 0;
+    ATile abuf[PD];
+    v4i bq[PB][CT][4];
+#pragma unroll
+    for (int d = 0; d < PD; ++d) load_a(abuf[d], arow, r, l, d);
+#pragma unroll
+    for (int d = 0; d < PB; ++d)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) bq[d][ct][v] = Q[(size_t)d * CT * 256 + (ct * 4 + v) * 64];
+    for (int kk0 = 0; kk0 < nk; kk0 += U) {
+#pragma unroll
+        for (int dd = 0; dd < U; ++dd) {
+            const int da = dd % PD, db = dd % PB;
+            const int kk = kk0 + dd;
+            v4i a[4];
+            a[0] = abuf[da].d[0];
+            a[1] = abuf[da].d[1];
+            a[2] = abuf[da].d[2];
+            a[3] = r == 0 ? top_digits(abuf[da].top) : top_digits1(abuf[da].top);
+            load_a(abuf[da], arow, r, l, min(kk + PD, nk - 1));
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v)
+                        acc[u + v][ct] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[u], bq[db][ct][v], acc[u + v][ct], 0, 0, 0);
+            const int kb = min(kk + PB, nk - 1);
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) bq[db][ct][v] = Q[(size_t)kb * CT * 256 + (ct * 4 + v) * 64];
+        }
+    }
+    k1b_store<CT>(acc, ql, outb, S, r, half, sh, g, l, nq, q_stride);
 }
 
 // Candidates for the batched inner product, per column-tile count. All give
@@ -1001,6 +1154,14 @@ static const K1bVariant k_k1b_variants[] = {
     {"prefetch 4", 2, k1b_kernel<2, 4, 4>},
     {"prefetch 4, unroll 8", 2, k1b_kernel<2, 4, 8>},
     {"prefetch 4, unroll 32", 2, k1b_kernel<2, 4, 32>},
+    {"LDS ring 3, prefetch 2", 1, k1b_ring_kernel<1, 2, 6>},
+    {"LDS ring 3, prefetch 3", 1, k1b_ring_kernel<1, 3, 6>},
+    {"LDS ring 3, prefetch 2", 2, k1b_ring_kernel<2, 2, 6>},
+    {"LDS ring 3, prefetch 3", 2, k1b_ring_kernel<2, 3, 6>},
+    {"no LDS, prefetch 3/2", 1, k1b_direct_kernel<1, 3, 2, 6>},
+    {"no LDS, prefetch 3/3", 1, k1b_direct_kernel<1, 3, 3, 6>},
+    {"no LDS, prefetch 2/2", 2, k1b_direct_kernel<2, 2, 2, 2>},
+    {"no LDS, prefetch 3/2", 2, k1b_direct_kernel<2, 3, 2, 6>},
 };
 constexpr int K1B_NVAR = sizeof(k_k1b_variants) / sizeof(k_k1b_variants[0]);
 // chosen variant per (column tiles, streams); -1 = default (register ring, prefetch 4)

@@ -1,11 +1,14 @@
 #!/usr/bin/env python3
 """Benchmark of the PIR server response path on one MI355X.
 
-Workload (the reference's `bench --db-entries 4194304 --preprocessed-db-count 64`
-shape, crates/pir/src/pir_wrapper.rs:493-615): 2^22 entries of 2560 B = 256
-full-size shards (16384 entries each); 64 distinct seeded shards are
-preprocessed and kept resident in HBM and the fleet reuses them cyclically
-(pir_wrapper.rs:578); 8 seeded query indices. A step is one complete query:
+Workload (the reference's `bench --db-entries 4194304` shape,
+crates/pir/src/pir_wrapper.rs:493-615): 2^22 entries of 2560 B = 256
+full-size shards (16384 entries each), all 256 distinct, seeded, preprocessed
+and resident in HBM (164 GB); 8 seeded query indices. (`--resident 64` runs
+the reference bench's `--preprocessed-db-count 64` fleet, which reuses 64
+shards cyclically, pir_wrapper.rs:578; there the inner product, whose blocks
+of one output range cover all shards at once, reads a repeated shard from
+cache, so that figure is not an HBM measurement.) A step is one complete query:
 query bytes (already in HBM) -> unpack -> primary inner product over all 256
 shards -> mod switch -> per-query secondary database (encode + Galois /
 key-switch chain) -> reply bytes (in HBM). Every reply is decrypted and
@@ -204,7 +207,7 @@ def make_shards(pir, lp, count, seed0):
         s = pir.PIRDatabase(lp)
         s.set_db_device(coeffs)
         shards.append(s)
-        entries.append(coeffs)
+        entries.append(coeffs.cpu())  # kept for the decryption checks only
     torch.cuda.synchronize()
     return shards, entries
 
@@ -464,7 +467,9 @@ def main():
     ap.add_argument("--steps", type=int, default=16)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--entries", type=int, default=1 << 22)
-    ap.add_argument("--resident", type=int, default=64)
+    ap.add_argument("--resident", type=int, default=256)
+    ap.add_argument("--k1-mode", type=int, default=0, choices=[0, 2, 3],
+                    help="single-query inner product: 0 auto, 2 two streams, 3 three streams")
     ap.add_argument("--queries", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
@@ -531,6 +536,7 @@ def main():
     rb = pir.reply_bytes()
     replies = [torch.zeros(rb, dtype=torch.uint8, device="cuda") for _ in qs]
     ctx = pir.Context()
+    ctx.set_k1_mode(args.k1_mode)
 
     for w in range(args.warmup):
         pi, si, sk, qd = qs[w % len(qs)]
@@ -630,7 +636,8 @@ def main():
         "data": "synthetic: seeded uniform 2560-byte entries packed into 2048 ten-bit coefficients",
         "config": {
             "workload": (f"{args.entries} entries = {nshards} shards x {8 * lp} entries (l={lp}), "
-                         f"{resident} resident reused cyclically, secondary l={ls}, "
+                         + (f"all {resident} distinct and resident" if resident == nshards
+                            else f"{resident} resident reused cyclically") + f", secondary l={ls}, "
                          f"{args.queries} seeded queries"),
             "entries": args.entries, "shards": nshards, "resident_shards": resident,
             "group_order_primary": lp, "group_order_secondary": ls,
@@ -644,7 +651,8 @@ def main():
         "preprocess_s": round(t_pre, 2),
         "roofline": {
             "bound": "hbm",
-            "kernel": "k1d_kernel (one query: 256 shards in one launch)",
+            "kernel": ("k1d_kernel (one query: 256 shards in one launch, all three streams)" if args.k1_mode == 3 else
+                       "k1w_kernel (one query: 256 shards in one launch, TE and IPGK1 streams)"),
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -652,8 +660,11 @@ def main():
             "traffic": traffic,
             "launch_ms": round(k1_ms, 3),
             "bytes_per_launch": bytes_streamed,
-            "bytes_note": ("the shard slabs read by one launch: 256 x 641,728,512 B (26 bits per residue mod p1, "
-                           "25 mod p2); "
+            "bytes_note": (("the shard arrays read by one launch: 256 x 641,728,512 B (26 bits per residue mod p1, "
+                            "25 mod p2); " if args.k1_mode == 3 else
+                            "the shard arrays read by one launch: 256 x 2/3 x 641,728,512 B (the TE and IPGK1 "
+                            "arrays at 26 bits per residue mod p1, 25 mod p2; IPGK0 is folded into per-query "
+                            "weights that all shards share, 64 MiB per query, read from L2); ") +
                            "the reference's accounting of the same pass (6*N*l*32 B per shard) is "
                            "ref_bytes_per_launch"),
             "ref_bytes_per_launch": ref_bytes,

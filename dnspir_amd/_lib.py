@@ -98,6 +98,7 @@ SIGNATURES = {
     "dnspir_ctx_stream": (vp, [vp]),
     "dnspir_ctx_set_graphs": (ci, [vp, ci]),
     "dnspir_ctx_set_exchange_polls": (ci, [vp, ci]),
+    "dnspir_ctx_set_k1_mode": (ci, [vp, ci]),
     "dnspir_double_pir": (ci, [vp, vp, sz, vp, sz, u64p, u64p, u64p, u64p, ctypes.POINTER(Timing)]),
     "dnspir_process_queries": (ci, [vp, vp, sz, vp, sz, u8p, sz, u8p, sz, u8p, u8p, ctypes.POINTER(Timing)]),
     "dnspir_process_queries_device": (ci, [vp, vp, sz, vp, sz, vp, sz, vp, sz, vp, vp, ctypes.POINTER(Timing)]),

@@ -248,14 +248,23 @@ int upload_lanes(dnspir_ctx* c, const std::vector<long long>& off, const std::ve
 // per-(query, shard) results for the epilogue in c->outb (limb-block shards)
 // or c->acc (rotated shards, nq = 1).
 int run_primary(dnspir_ctx* c, const uint32_t* const* slabs, int len, int l_p, const uint32_t* qp, int nq, bool* used_outb,
-                hipEvent_t k1_start) {
+                hipEvent_t k1_start, uint64_t* bytes = nullptr) {
     const int L_p = loop_len(l_p), nst_p = nstreams(l_p);
     const hipStream_t s = c->stream;
     int rc;
+    if (bytes) *bytes = (uint64_t)len * slab_bytes(l_p);
     if (L_p == N / 2) {
-        if ((rc = c->ql.ensure(qlimb_bytes(nq, nst_p)))) return rc;
         if ((rc = c->outb.ensure((size_t)nq * len * 2 * N * LANES * 4))) return rc;
-        launch_k1b(slabs, len, qp, nq, c->ql.as<uint32_t>(), c->outb.as<uint32_t>(), nst_p, s, k1_start);
+        const bool two = nq == 1 && (c->k1_mode == 2 || (c->k1_mode == 0 && len >= K1W_MIN_SHARDS));
+        if (two) {
+            if ((rc = c->wscan.ensure(k1w_scan_bytes())) || (rc = c->wtile.ensure(k1w_tile_bytes(nst_p)))) return rc;
+            launch_k1w(slabs, len, qp, c->wscan.as<uint32_t>(), c->wtile.as<uint32_t>(), c->outb.as<uint32_t>(), nst_p,
+                       g.tables, s, k1_start);
+            if (bytes) *bytes = *bytes / 3 * 2;  // the TE and IPGK1 streams
+        } else {
+            if ((rc = c->ql.ensure(qlimb_bytes(nq, nst_p)))) return rc;
+            launch_k1b(slabs, len, qp, nq, c->ql.as<uint32_t>(), c->outb.as<uint32_t>(), nst_p, s, k1_start);
+        }
         *used_outb = true;
     } else {
         if (nq != 1) return DNSPIR_E_SHAPE;
@@ -365,8 +374,9 @@ int run_double(dnspir_ctx* c, const dnspir_shard* const* fleets[2], const size_t
         if (!present[sl]) continue;
         const int len = (int)lens[sl];
         bool ob = false;
+        uint64_t bytes = 0;
         if ((rc = run_primary(c, slabs[sl], len, l_p, c->qp.as<uint32_t>() + (size_t)sl * 4 * 2 * N, 1, &ob,
-                              c->k1_launches == 0 ? c->ev[3] : nullptr)))
+                              c->k1_launches == 0 ? c->ev[3] : nullptr, &bytes)))
             return rc;
         if (c->k1_launches == 0) HIP_OK(hipEventRecord(c->ev[4], s));
         ++c->k1_launches;
@@ -380,7 +390,7 @@ int run_double(dnspir_ctx* c, const dnspir_shard* const* fleets[2], const size_t
         e.lpr = LANES;
         e.lane0_mult = 0;
         launch_epi(e, g.tables, s);
-        streamed += (uint64_t)len * slab_bytes(l_p);
+        streamed += bytes;
         ref_streamed += (uint64_t)len * ref_slab_bytes(l_p);
     }
     HIP_OK(hipEventRecord(c->ev[1], s));
@@ -670,6 +680,13 @@ int dnspir_ctx_set_exchange_polls(dnspir_ctx* c, int max_polls) {
     return 0;
 }
 
+int dnspir_ctx_set_k1_mode(dnspir_ctx* c, int mode) {
+    if (!c || (mode != 0 && mode != 2 && mode != 3)) return DNSPIR_E_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->k1_mode = mode;
+    return 0;
+}
+
 int dnspir_choose_index_group_secondary(size_t num_shards) {
     int log2l = 0;
     while (((size_t)1 << log2l) < num_shards) ++log2l;
@@ -824,16 +841,11 @@ int dnspir_process_queries(dnspir_ctx* c, const dnspir_shard* const* f0, size_t 
     }
     uint8_t* rd[2] = {r0 ? c->rbytes.as<uint8_t>() : nullptr, r1 ? c->rbytes.as<uint8_t>() + REPLY_BYTES : nullptr};
     rc = process_device_impl(c, f0, len0, f1, len1, qd[0], std::min(ql0, qcap), qd[1], std::min(ql1, qcap), rd[0], rd[1],
-                             nullptr, t0);
+                             tm, t0);
     if (rc) return rc;
     if (r0 && q0) HIP_OK(hipMemcpy(r0, rd[0], REPLY_BYTES, hipMemcpyDeviceToHost));
     if (r1 && q1) HIP_OK(hipMemcpy(r1, rd[1], REPLY_BYTES, hipMemcpyDeviceToHost));
-    fill_timing(c, tm, now_us() - t0);
-    if (tm) {
-        int lp = f0[0]->l;
-        tm->bytes_streamed = (uint64_t)((q0 ? len0 : 0) + (q1 ? len1 : 0)) * slab_bytes(lp);
-        tm->ref_bytes = (uint64_t)((q0 ? len0 : 0) + (q1 ? len1 : 0)) * ref_slab_bytes(lp);
-    }
+    fill_timing(c, tm, now_us() - t0);  // bytes_streamed / ref_bytes were set by run_double
     return 0;
 }
 

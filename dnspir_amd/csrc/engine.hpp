@@ -130,6 +130,9 @@ struct dnspir_ctx {
     // small persistent buffers (pointer lists, lane lists, the two-CU chain's
     // exchange words and timeout flag)
     dne::DevBuf ptrs, idx, keyp, xch;
+    // two-stream inner product of single queries: the C table and the weight tiles
+    dne::DevBuf wscan, wtile;
+    int k1_mode = 0;  // 0: two streams from K1W_MIN_SHARDS shards on, 2: always, 3: never
     unsigned* h_err = nullptr;  // pinned copy of the chain timeout flag
     int exchange_polls = 1 << 21;  // bound of each exchange wait of the two-CU chain
     bool pair_pending = false;  // h_err is to be checked after the next sync
@@ -176,6 +179,10 @@ int ctx_init(dnspir_ctx* c, hipStream_t s);
 // chains at most this many run on two CUs each (chain_pair_kernel): up to one
 // block per CU of the 256
 constexpr int PAIR_MAX_CHAINS = 128;
+// fleets of at least this many limb-block shards take the two-stream inner
+// product for single queries (below, preparing the weights costs about what
+// the third stream would)
+constexpr int K1W_MIN_SHARDS = 8;
 // after the stream sync that follows a call: DNSPIR_E_HIP if that call's
 // two-CU chain timed out waiting for a partner block
 int check_pair(dnspir_ctx* c);

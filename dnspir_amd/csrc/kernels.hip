@@ -1752,6 +1752,237 @@ __global__ __launch_bounds__(256) void k1d_kernel(const uint32_t* const* __restr
     }
 }
 
+// ---------------------------------------------------------------------------
+// Two-stream K1 for one or two queries (limb-block shards). The key-switch
+// digits satisfy NTT(D0_i) = sigma(a_i) - 2^26 NTT(D1_i) mod p_r
+// (base_pir.rs:544-579: D0 + 2^26 D1 is the exact lift of sigma(a_i)), and the
+// chain a_(i+1) = sigma(a_i) gka0 + D1_i (gka1 - 2^26 gka0) + qry_a TE_i
+// (base_pir.rs:838-861, a_0 = 0) is linear in the stored rows. Unrolled, the
+// IPGK0 term of main_pir_loop (base_pir.rs:1153-1208) becomes a row-dependent
+// weight on the IPGK1 and TE rows, so K1 reads two of the three streams:
+//   out[a] = sum_(st,j) IPGK1[a][j] G[a][j] + TE_st[a][j] Q_st[a][j]
+//   G[a][j]    = gkb1'[j] + gka1'[j] C[(j - a) mod L][j]
+//   Q_st[a][j] = qb_st[j] + qa_st[j] C[(j - a) mod L][j]
+//   C[M][j]    = sum_(n=1..M) gkb0[j-n] prod_(m=1..n-1) gka0[j-m]
+// in Galois positions (j, j-n within one half, mod L = N/2), x' = x - 2^26
+// x0. The weights depend on the query and the seeds only, so all shards of a
+// fleet share them.
+// ---------------------------------------------------------------------------
+
+// C[q][r][h][M][j] (u32): one thread per (q, r, h, j), the recurrence over M
+__global__ __launch_bounds__(256) void k1w_scan_kernel(const uint32_t* __restrict__ qp, uint32_t* __restrict__ C,
+                                                       const DevTables* __restrict__ T) {
+    __shared__ uint32_t g0s[HALFN], gas[HALFN];
+    const int h = blockIdx.y, r = blockIdx.z & 1, q = blockIdx.z >> 1;
+    const uint32_t p = mod_p(r);
+    const int K = mod_k(r);
+    const uint32_t* g0 = qp + (((size_t)q * 4 + 0) * 2 + r) * N + h * HALFN;
+    for (int y = threadIdx.x; y < HALFN; y += blockDim.x) {
+        g0s[y] = g0[y];
+        gas[y] = T->gka[0][r][T->gpos_slot[h * HALFN + y]];
+    }
+    __syncthreads();
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t* out = C + ((size_t)(q * 2 + r) * 2 + h) * HALFN * HALFN + j;
+    uint32_t c = 0, P = 1;
+    out[0] = 0;
+    for (int M = 1; M < HALFN; ++M) {
+        const int y = (j - M) & (HALFN - 1);
+        c = add_mod(c, mul_mod(g0s[y], P, p, K), p);
+        P = mul_mod(P, gas[y], p, K);
+        out[(size_t)M * HALFN] = c;
+    }
+}
+
+// bytes of the weight tiles of one query: [r][h][g][s'][c] tiles of 512 B
+// (s' = 2 st + {0: TE, 1: IPGK1}), then the corrections [r][h][a] (u32)
+
+// Weight tiles in the layout k1w_kernel reads: tile (r, h, g, s', c) holds at
+// byte v*128 + (ai*2 + hh)*16 + kk digit v of the weight of a = 4g + ai,
+// j = 32c + 16hh + kk (balanced digits, as the query limbs of the batched
+// kernels). corr[a] = -2^25 sum_(s',j) W[a][j] mod p1 for the biased top
+// digits (zero mod p2). One block per (16 a's, h, r, q).
+__global__ __launch_bounds__(256) void k1w_tiles_kernel(const uint32_t* __restrict__ qp, const uint32_t* __restrict__ C,
+                                                        uint32_t* __restrict__ W, int nst, const DevTables* __restrict__ T) {
+    __shared__ uint32_t E[16][HALFN + 1];  // E[d][j] = C[(j - a0 - d) mod L][j]; +1 spreads the rows over banks
+    __shared__ uint32_t base[4][HALFN], mult[4][HALFN];
+    __shared__ unsigned long long csum[16];
+    const int a0 = blockIdx.x * 16, h = blockIdx.y, r = blockIdx.z & 1, q = blockIdx.z >> 1;
+    const int tid = threadIdx.x;
+    const uint32_t p = mod_p(r);
+    const int K = mod_k(r);
+    const uint32_t two26 = (uint32_t)((1ull << 26) % p);
+    const uint32_t* Cb = C + ((size_t)(q * 2 + r) * 2 + h) * HALFN * HALFN;
+    for (int e = tid; e < 16 * HALFN; e += 256) {
+        const int M = e >> 4, d = e & 15;
+        const int j = (M + a0 + d) & (HALFN - 1);
+        E[d][j] = Cb[(size_t)M * HALFN + j];
+    }
+    const uint32_t* qv = qp + ((size_t)q * 4 * 2 + r) * N + h * HALFN;  // vector v at qv + v * 2N
+    for (int y = tid; y < HALFN; y += 256) {
+        const int sl = T->gpos_slot[h * HALFN + y];
+        const uint32_t g0 = qv[y], g1 = qv[2 * N + y];
+        const uint32_t g1p = sub_mod(g1, mul_mod(two26, g0, p, K), p);
+        const uint32_t ga1p = sub_mod(T->gka[1][r][sl], mul_mod(two26, T->gka[0][r][sl], p, K), p);
+        for (int st = 0; st < nst; ++st) {
+            base[2 * st][y] = qv[(2 + st) * 2 * N + y];
+            mult[2 * st][y] = T->qa[st][r][sl];
+            base[2 * st + 1][y] = g1p;
+            mult[2 * st + 1][y] = ga1p;
+        }
+    }
+    if (tid < 16) csum[tid] = 0;
+    __syncthreads();
+    // 32 (g, ai, hh) combinations x 8 threads, each over (s', c) items
+    const int combo = tid & 31, part = tid >> 5;
+    const int gl = combo >> 3, ai = (combo >> 1) & 3, hh = combo & 1;
+    const int d = gl * 4 + ai;
+    const int g = blockIdx.x * 4 + gl;
+    const int nsp = 2 * nst;
+    unsigned long long sum = 0;
+    uint32_t* Wq = W + (size_t)q * (k1w_tile_bytes(nst) / 4);
+    for (int it = part; it < nsp * 32; it += 8) {
+        const int sp = it >> 5, c = it & 31;
+        uint32_t w[16];
+#pragma unroll
+        for (int kk = 0; kk < 16; ++kk) {
+            const int j = 32 * c + 16 * hh + kk;
+            const uint32_t x = add_mod(base[sp][j], mul_mod(mult[sp][j], E[d][j], p, K), p);
+            sum += x;
+            w[kk] = limbs8(x);
+        }
+        uint32_t* dst = Wq + ((((size_t)(r * 2 + h) * (HALFN / 4) + g) * nsp + sp) * 32 + c) * 128 + (ai * 2 + hh) * 4;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            u32x4 o;
+            o.x = ((w[0] >> (8 * v)) & 255) | (((w[1] >> (8 * v)) & 255) << 8) | (((w[2] >> (8 * v)) & 255) << 16) | (((w[3] >> (8 * v)) & 255) << 24);
+            o.y = ((w[4] >> (8 * v)) & 255) | (((w[5] >> (8 * v)) & 255) << 8) | (((w[6] >> (8 * v)) & 255) << 16) | (((w[7] >> (8 * v)) & 255) << 24);
+            o.z = ((w[8] >> (8 * v)) & 255) | (((w[9] >> (8 * v)) & 255) << 8) | (((w[10] >> (8 * v)) & 255) << 16) | (((w[11] >> (8 * v)) & 255) << 24);
+            o.w = ((w[12] >> (8 * v)) & 255) | (((w[13] >> (8 * v)) & 255) << 8) | (((w[14] >> (8 * v)) & 255) << 16) | (((w[15] >> (8 * v)) & 255) << 24);
+            *reinterpret_cast<u32x4*>(dst + v * 32) = o;
+        }
+    }
+    atomicAdd(&csum[d], sum);
+    __syncthreads();
+    if (tid < 16) {
+        uint32_t cv = 0;
+        if (r == 0) {
+            const uint64_t m = ((1ull << 25) % p) * (csum[tid] % p) % p;
+            cv = (uint32_t)((p - m) % p);
+        }
+        Wq[(size_t)4 * (HALFN / 4) * nsp * 32 * 128 + (r * 2 + h) * HALFN + a0 + tid] = cv;
+    }
+}
+
+// The two-stream inner product: the k1d arithmetic (v_dot4 over 4 digit
+// pairs, 7 weight accumulators) on the TE and IPGK1 K-steps only. A block is
+// one a-group of 8 shards (one wave each): the weights of that a-group for
+// every K-step (nsp x 32 tiles of 512 B) are staged in LDS once and read
+// like the query digits of k1d, so the loop has no barrier. Grid x = a-group:
+// the blocks in flight cover all a-groups of a few shard groups, so each XCD
+// holds the 64 KiB weight tiles of its 32 a-groups in L2 while the shard
+// groups pass (measured: 86% of HBM peak, against 80-84% with the shard
+// group fastest, which keeps pages of all 256 shards open at once).
+template <int R, int PD, int U>
+__device__ __forceinline__ void k1w_body(const uint32_t* const* __restrict__ slabs, int nshards, const uint32_t* __restrict__ W,
+                                         uint32_t* __restrict__ outb, int nst, const u32x4* wl, int g, int sg, int half) {
+    const int S = 3 * nst, nsp = 2 * nst;
+    const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63, h = l >> 5;
+    const int ai = (l & 31) >> 3;
+    const int nk = nsp * 32;
+    const int sh = sg * 8 + wave;
+    if (sh >= nshards) return;
+    const ARow arow = a_row(slabs[sh], R, half, g, S);
+    int acc[7];
+#pragma unroll
+    for (int w = 0; w < 7; ++w) acc[w] = 0;
+    // K-step q: s' = q / 32 -> slab stream s = 3 st + (0: TE, 2: IPGK1), chunk
+    // q % 32, i.e. slab K-step q + 32 * ((s' + 1) / 2). Each shard walks the
+    // K-steps from its own starting point (the integer sums do not depend on
+    // the order), so the 8 shards of a block, which stream the same a-group,
+    // read at different offsets. Loads are unconditional (index clamped to the
+    // last step).
+    const int rot = (sh * 37) & (nk - 1);
+    ATile abuf[PD];
+#pragma unroll
+    for (int d = 0; d < PD; ++d) {
+        const int q = (d + rot) & (nk - 1);
+        load_a(abuf[d], arow, R, l, q + 32 * (((q >> 5) + 1) >> 1));
+    }
+    const int lb = ai * 2 + h;
+    // U steps per loop iteration (U a multiple of PD dividing nk): the body is
+    // long enough that the compiler keeps the PD-step prefetch within it
+    for (int kk0 = 0; kk0 < nk; kk0 += U) {
+#pragma unroll
+        for (int dd = 0; dd < U; ++dd) {
+            const int d = dd % PD;
+            const int kk = kk0 + dd;
+            v4i a[4];
+            a[0] = abuf[d].d[0];
+            a[1] = abuf[d].d[1];
+            a[2] = abuf[d].d[2];
+            a[3] = R == 0 ? top_digits(abuf[d].top) : top_digits1(abuf[d].top);
+            const int qn = (min(kk + PD, nk - 1) + rot) & (nk - 1);
+            load_a(abuf[d], arow, R, l, qn + 32 * (((qn >> 5) + 1) >> 1));
+            // keep the prefetch here: without this the scheduler moves each
+            // load down to its use and only one step stays in flight
+            __builtin_amdgcn_sched_barrier(0);
+            u32x4 b[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) b[v] = wl[((kk + rot) & (nk - 1)) * 32 + v * 8 + lb];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    int t = acc[u + v];
+                    t = __builtin_amdgcn_sdot4(a[u].x, (int)b[v].x, t, false);
+                    t = __builtin_amdgcn_sdot4(a[u].y, (int)b[v].y, t, false);
+                    t = __builtin_amdgcn_sdot4(a[u].z, (int)b[v].z, t, false);
+                    t = __builtin_amdgcn_sdot4(a[u].w, (int)b[v].w, t, false);
+                    acc[u + v] = t;
+                }
+        }
+    }
+    const uint32_t p = mod_p(R);
+    const int K = mod_k(R);
+    uint32_t cw[7];
+    weight_consts(p, cw);
+    const int row = l & 31;
+    const int a = g * 4 + (row >> 3), lane8 = row & 7;
+    const uint32_t* corr = W + (size_t)4 * (HALFN / 4) * nsp * 32 * 128;
+    int64_t t = corr[(R * 2 + half) * HALFN + a];
+#pragma unroll
+    for (int w = 0; w < 7; ++w) {
+        const int tot = acc[w] + __shfl_xor(acc[w], 32);
+        t += (int64_t)tot * cw[w];
+    }
+    if (h == 0) outb[(((size_t)sh * 2 + R) * N + half * HALFN + a) * LANES + lane8] = from_i64(t, p, K);
+}
+
+template <int PD, int U>
+__global__ __launch_bounds__(512) void k1w_kernel(const uint32_t* const* __restrict__ slabs, int nshards,
+                                                  const uint32_t* __restrict__ W, uint32_t* __restrict__ outb, int nst) {
+    extern __shared__ u32x4 wl[];  // [nsp * 32 K-steps][32]
+    const int g = blockIdx.x, sg = blockIdx.y;
+    const int half = blockIdx.z & 1, r = blockIdx.z >> 1;
+    const int nk = 2 * nst * 32;
+    const u32x4* src = reinterpret_cast<const u32x4*>(W) + ((size_t)(r * 2 + half) * (HALFN / 4) + g) * nk * 32;
+    for (int i = threadIdx.x; i < nk * 32; i += 512) wl[i] = src[i];
+    __syncthreads();
+    if (r == 0) k1w_body<0, PD, U>(slabs, nshards, W, outb, nst, wl, g, sg, half);
+    else k1w_body<1, PD, U>(slabs, nshards, W, outb, nst, wl, g, sg, half);
+}
+
+void launch_k1w(const uint32_t* const* slabs, int nshards, const uint32_t* qp, uint32_t* scan, uint32_t* tiles,
+                uint32_t* outb, int nst, const DevTables* T, hipStream_t s, hipEvent_t main_start) {
+    hipLaunchKernelGGL(k1w_scan_kernel, dim3(HALFN / 64, 2, 2), dim3(64), 0, s, qp, scan, T);
+    hipLaunchKernelGGL(k1w_tiles_kernel, dim3(HALFN / 16, 2, 2), dim3(256), 0, s, qp, scan, tiles, nst, T);
+    if (main_start) (void)hipEventRecord(main_start, s);
+    const size_t lds = (size_t)2 * nst * 32 * 512;
+    hipLaunchKernelGGL((k1w_kernel<4, 32>), dim3(HALFN / 4, (nshards + 7) / 8, 4), dim3(512), lds, s, slabs, nshards, tiles,
+                       outb, nst);
+}
+
 void launch_relayout(const uint32_t* rows_tmp, uint32_t* slab, int L, int nst, hipStream_t s) {
     if (L == HALFN)
         hipLaunchKernelGGL(relayout_mb_kernel, dim3(32 * 32, 4, 3 * nst), dim3(256), 0, s, rows_tmp, slab, nst);

@@ -138,6 +138,12 @@ class Context:
         up at once, which makes the call fail with DNSPIR_E_HIP)."""
         check(lib().dnspir_ctx_set_exchange_polls(self.h, int(max_polls)), "ctx_set_exchange_polls")
 
+    def set_k1_mode(self, mode):
+        """Single-query inner product: 3 streams all three shard arrays, 2
+        streams TE and IPGK1 with per-query weights in place of IPGK0, 0
+        (default) picks 2 for fleets of 8 or more shards. Same bytes."""
+        check(lib().dnspir_ctx_set_k1_mode(self.h, int(mode)), "ctx_set_k1_mode")
+
     def __del__(self):
         h = getattr(self, "h", None)
         if h is not None and h.value:

@@ -114,6 +114,15 @@ int dnspir_ctx_set_graphs(dnspir_ctx* ctx, int on);
  * 0 makes every wait give up at once (to exercise that error path). No
  * counterpart in the reference. */
 int dnspir_ctx_set_exchange_polls(dnspir_ctx* ctx, int max_polls);
+/* Inner product of single queries on shards with loop length N/2. mode 3:
+ * stream all three shard arrays (TE, IPGK0, IPGK1), as main_pir_loop does
+ * (base_pir.rs:1153-1208). mode 2: stream TE and IPGK1 only and fold IPGK0
+ * into per-query weights (IPGK0 = sigma(a_i) - 2^26 IPGK1, with the chain
+ * a_i linear in the stored rows), a third fewer bytes. mode 0 (default):
+ * mode 2 for fleets of 8 or more shards, else mode 3. Every mode gives the
+ * same bytes. Batches of 2 or more queries always stream three arrays. No
+ * counterpart in the reference. */
+int dnspir_ctx_set_k1_mode(dnspir_ctx* ctx, int mode);
 
 /* ---- two-level composition: perform_batched_double_pir (double_pir.rs:126-228)
  * fleets: random-access shard lists (shards may repeat). q0/q1: query blocks

@@ -154,6 +154,45 @@ def test_process_queries_bytes_and_decrypt(pir, l):
         assert np.array_equal(pir.process_reply(reps[0], sk), want)
 
 
+@pytest.mark.parametrize("l", [1024, 2048])
+def test_two_stream_inner_product_matches_three_streams(pir, l):
+    """The single-query inner product over the TE and IPGK1 arrays only (IPGK0
+    folded into per-query weights through the key-switch identity) gives the
+    oracle's bytes, like the three-array one, on both slots; the automatic
+    choice agrees with both."""
+    g, o, e = _shard(pir, l, 1, 0)
+    g2, o2, e2 = _shard(pir, l, 2, 0)
+    fg, fo = [g, g2, g], [o, o2, o]
+    ls = orc.choose_index_group_secondary(3)
+    rng = pir.Rng(bytes([7] * 32))
+    q0, sk0 = pir.prepare_query(rng, 8 * l - 1, 1, l, ls)
+    q1, sk1 = pir.prepare_query(rng, 5, 2, l, ls)
+    oreps, _, _ = orc.process_queries(fo, fo, q0, q1)
+    got = {}
+    for mode in (2, 3, 0):
+        ctx = pir.Context()
+        ctx.set_k1_mode(mode)
+        _say(f"[parity] two-stream K1 l={l} mode={mode}")
+        got[mode], tm = pir.process_queries(fg, fg, q0, q1, ctx=ctx)
+        if mode == 2:
+            assert tm.bytes_streamed * 3 == 2 * 2 * 3 * _lib_slab_bytes(pir, l)
+    assert got[2] == oreps and got[3] == oreps and got[0] == oreps
+    assert np.array_equal(pir.process_reply(got[2][0], sk0), e2[8 * l - 1])
+    assert np.array_equal(pir.process_reply(got[2][1], sk1), e[5])
+    # raw ring elements of the composition, no wire rounding in between
+    ctx = pir.Context()
+    ctx.set_k1_mode(2)
+    qb = _block(70, l == N, ls == N)
+    go, _ = pir.perform_batched_double_pir(fg, None, qb, None, ctx=ctx)
+    oo, _, _ = orc.double_pir(fo, None, qb, None, ls)
+    assert np.array_equal(go[0], oo[0])
+
+
+def _lib_slab_bytes(pir, l):
+    from dnspir_amd import lib
+    return lib().dnspir_required_bytes(l) - 196608
+
+
 def test_process_queries_two_slots(pir):
     g, o, e = _shard(pir, 1024, 1, 0)
     rng = pir.Rng(bytes([6] * 32))

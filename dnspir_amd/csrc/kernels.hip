@@ -1769,33 +1769,55 @@ __global__ __launch_bounds__(256) void k1d_kernel(const uint32_t* const* __restr
 // fleet share them.
 // ---------------------------------------------------------------------------
 
-// C[q][r][h][M][j] (u32): one thread per (q, r, h, j), the recurrence over M
-__global__ __launch_bounds__(256) void k1w_scan_kernel(const uint32_t* __restrict__ qp, uint32_t* __restrict__ C,
-                                                       const DevTables* __restrict__ T) {
+// C[r][h][M][j] (u32) for one query. The recurrence over M (c += gkb0[j-M] P,
+// P *= gka0[j-M]) is affine in (c, P), so it is split into 16 segments of 64
+// steps: each thread (j, segment) first forms its segment's own product and
+// sum from (0, 1), the segments' starting values come from a prefix over the
+// 16 segments, then each thread runs its segment again from the true start
+// and writes it. Block: 64 positions j x 16 segments.
+__global__ __launch_bounds__(1024) void k1w_scan_kernel(const uint32_t* __restrict__ qp, uint32_t* __restrict__ C,
+                                                        const DevTables* __restrict__ T) {
+    constexpr int SEG = 16, LEN = HALFN / SEG;
     __shared__ uint32_t g0s[HALFN], gas[HALFN];
-    const int h = blockIdx.y, r = blockIdx.z & 1, q = blockIdx.z >> 1;
+    __shared__ uint32_t segc[SEG][64], segp[SEG][64];
+    const int h = blockIdx.y, r = blockIdx.z;
     const uint32_t p = mod_p(r);
     const int K = mod_k(r);
-    const uint32_t* g0 = qp + (((size_t)q * 4 + 0) * 2 + r) * N + h * HALFN;
-    for (int y = threadIdx.x; y < HALFN; y += blockDim.x) {
+    const int tid = threadIdx.x, jj = tid & 63, seg = tid >> 6;
+    const uint32_t* g0 = qp + (size_t)r * N + h * HALFN;  // vector 0 (gkb0) of the query
+    for (int y = tid; y < HALFN; y += 1024) {
         g0s[y] = g0[y];
         gas[y] = T->gka[0][r][T->gpos_slot[h * HALFN + y]];
     }
     __syncthreads();
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t* out = C + ((size_t)(q * 2 + r) * 2 + h) * HALFN * HALFN + j;
+    const int j = blockIdx.x * 64 + jj;
+    const int m0 = seg * LEN;  // steps M = m0 .. m0 + LEN - 1 (M = 0 contributes nothing)
     uint32_t c = 0, P = 1;
-    out[0] = 0;
-    for (int M = 1; M < HALFN; ++M) {
+    for (int M = (seg == 0 ? 1 : m0); M < m0 + LEN; ++M) {
+        const int y = (j - M) & (HALFN - 1);
+        c = add_mod(c, mul_mod(g0s[y], P, p, K), p);
+        P = mul_mod(P, gas[y], p, K);
+    }
+    segc[seg][jj] = c;
+    segp[seg][jj] = P;
+    __syncthreads();
+    // start of this segment: fold the earlier segments, (c, P) -> (c + P c_k, P P_k)
+    uint32_t c0 = 0, P0 = 1;
+    for (int k = 0; k < seg; ++k) {
+        c0 = add_mod(c0, mul_mod(P0, segc[k][jj], p, K), p);
+        P0 = mul_mod(P0, segp[k][jj], p, K);
+    }
+    uint32_t* out = C + ((size_t)r * 2 + h) * HALFN * HALFN + j;
+    if (seg == 0) out[0] = 0;
+    c = c0;
+    P = P0;
+    for (int M = (seg == 0 ? 1 : m0); M < m0 + LEN; ++M) {
         const int y = (j - M) & (HALFN - 1);
         c = add_mod(c, mul_mod(g0s[y], P, p, K), p);
         P = mul_mod(P, gas[y], p, K);
         out[(size_t)M * HALFN] = c;
     }
 }
-
-// bytes of the weight tiles of one query: [r][h][g][s'][c] tiles of 512 B
-// (s' = 2 st + {0: TE, 1: IPGK1}), then the corrections [r][h][a] (u32)
 
 // Weight tiles in the layout k1w_kernel reads: tile (r, h, g, s', c) holds at
 // byte v*128 + (ai*2 + hh)*16 + kk digit v of the weight of a = 4g + ai,
@@ -1975,7 +1997,7 @@ __global__ __launch_bounds__(512) void k1w_kernel(const uint32_t* const* __restr
 
 void launch_k1w(const uint32_t* const* slabs, int nshards, const uint32_t* qp, uint32_t* scan, uint32_t* tiles,
                 uint32_t* outb, int nst, const DevTables* T, hipStream_t s, hipEvent_t main_start) {
-    hipLaunchKernelGGL(k1w_scan_kernel, dim3(HALFN / 64, 2, 2), dim3(64), 0, s, qp, scan, T);
+    hipLaunchKernelGGL(k1w_scan_kernel, dim3(HALFN / 64, 2, 2), dim3(1024), 0, s, qp, scan, T);
     hipLaunchKernelGGL(k1w_tiles_kernel, dim3(HALFN / 16, 2, 2), dim3(256), 0, s, qp, scan, tiles, nst, T);
     if (main_start) (void)hipEventRecord(main_start, s);
     const size_t lds = (size_t)2 * nst * 32 * 512;

@@ -255,8 +255,7 @@ int run_primary(dnspir_ctx* c, const uint32_t* const* slabs, int len, int l_p, c
     if (bytes) *bytes = (uint64_t)len * slab_bytes(l_p);
     if (L_p == N / 2) {
         if ((rc = c->outb.ensure((size_t)nq * len * 2 * N * LANES * 4))) return rc;
-        const bool two = nq == 1 && (c->k1_mode == 2 || (c->k1_mode == 0 && len >= K1W_MIN_SHARDS));
-        if (two) {
+        if (use_k1w(c, l_p, (size_t)len, nq)) {
             if ((rc = c->wscan.ensure(k1w_scan_bytes())) || (rc = c->wtile.ensure(k1w_tile_bytes(nst_p)))) return rc;
             launch_k1w(slabs, len, qp, c->wscan.as<uint32_t>(), c->wtile.as<uint32_t>(), c->outb.as<uint32_t>(), nst_p,
                        g.tables, s, k1_start);
@@ -886,9 +885,11 @@ int dne::batch_reserve(dnspir_ctx* c, int l_p, size_t len, int nq, size_t qlen) 
         (size_t)nl * N * 8,                          // bcoef
         (size_t)nl * N * 4,                          // a17o
         (size_t)nl * N * 4,                          // b12o
+        use_k1w(c, l_p, len, nq) ? k1w_scan_bytes() : 0,         // wscan
+        use_k1w(c, l_p, len, nq) ? k1w_tile_bytes(nst_p) : 0,    // wtile
     };
-    DevBuf* bufs[] = {&c->qcoef, &c->qs, &c->qp, &c->se, &c->ql, &c->outb, &c->M,
-                      &c->te, &c->aout, &c->cout, &c->acoef, &c->bcoef, &c->a17o, &c->b12o};
+    DevBuf* bufs[] = {&c->qcoef, &c->qs, &c->qp, &c->se, &c->ql, &c->outb, &c->M, &c->te,
+                      &c->aout, &c->cout, &c->acoef, &c->bcoef, &c->a17o, &c->b12o, &c->wscan, &c->wtile};
     for (size_t i = 0; i < sizeof(sizes) / sizeof(sizes[0]); ++i)
         if ((rc = bufs[i]->ensure(sizes[i]))) return rc;
     return 0;

@@ -159,8 +159,8 @@ struct dnspir_ctx {
         if (h_err) (void)hipHostFree(h_err);
     }
 
-    dne::DevBuf* scratch_bufs[17] = {&se, &M, &te, &aout, &cout, &qcoef, &qs, &qp, &acc,
-                                     &outb, &ql, &acoef, &bcoef, &a17o, &b12o, &qbytes, &rbytes};
+    dne::DevBuf* scratch_bufs[19] = {&se,    &M,     &te,    &aout, &cout, &qcoef,  &qs,     &qp,    &acc,  &outb,
+                                     &ql,    &acoef, &bcoef, &a17o, &b12o, &qbytes, &rbytes, &wscan, &wtile};
     // carve all scratch from `a` from now on (nullptr = back to hipMalloc);
     // storage the buffers held before is released or forgotten
     void bind_arena(dne::Arena* a) {
@@ -183,6 +183,11 @@ constexpr int PAIR_MAX_CHAINS = 128;
 // product for single queries (below, preparing the weights costs about what
 // the third stream would)
 constexpr int K1W_MIN_SHARDS = 8;
+// whether a one-query primary pass over `len` shards of group order l_p on
+// this context takes the two-stream inner product
+inline bool use_k1w(const dnspir_ctx* c, int l_p, size_t len, int nq) {
+    return nq == 1 && 2 * l_p >= N && (c->k1_mode == 2 || (c->k1_mode == 0 && len >= (size_t)K1W_MIN_SHARDS));
+}
 // after the stream sync that follows a call: DNSPIR_E_HIP if that call's
 // two-CU chain timed out waiting for a partner block
 int check_pair(dnspir_ctx* c);

@@ -163,3 +163,32 @@ def test_batch_graph_replays_match_eager(pir):
         assert single[0] == eager[k][rep * rb:(rep + 1) * rb].tobytes()
         assert np.array_equal(pir.process_reply(single[0], sk), e[si][pi])
     assert ctx.set_graphs(True) is False  # capture worked: no eager fallback
+
+
+def test_one_query_batches_two_stream_graph_replays(pir):
+    """One-query batches on a fleet of 9 entries take the two-array inner
+    product inside the captured graph (its weight buffers are reserved before
+    capture); every replay equals the three-array single-query reply."""
+    import torch
+    g, _, e = _fleet(pir, 1024, 3)
+    fleet = [g[k % 3] for k in range(9)]
+    qs = _queries(pir, 4, 1024, 9, 61)
+    qlen = len(qs[0][2])
+    rb = pir.reply_bytes()
+    qdev = torch.zeros(qlen, dtype=torch.uint8, device="cuda")
+    rdev = torch.zeros(rb, dtype=torch.uint8, device="cuda")
+    ctx = pir.Context()
+    ref = pir.Context()
+    ref.set_k1_mode(3)
+    for rep, (pi, si, q, sk) in enumerate(qs + qs[:2]):
+        qdev.copy_(torch.from_numpy(np.frombuffer(q, np.uint8).copy()))
+        rdev.zero_()
+        torch.cuda.synchronize()
+        _say(f"[batch] one-query batch replay {rep}")
+        pir.process_batch_device(fleet, qdev, qlen, 1, rdev, ctx=ctx)
+        got = rdev.cpu().numpy().tobytes()
+        want, _ = pir.process_queries(fleet, None, q, None, ctx=ref)
+        assert got == want[0], rep
+        assert np.array_equal(pir.process_reply(got, sk), e[si % 3][pi])
+    assert ctx.set_graphs(True) is False  # capture worked: no eager fallback
+

@@ -106,6 +106,11 @@ def test_batch_scratch_bytes():
     assert all(x > 0 for x in b) and b == sorted(b)
     # the secondary database alone: 3 lanes x l_s columns x N u16 per query
     assert b[1] - b[0] > 3 * 2048 * 2048 * 2
+    # one query on 8 or more shards reserves the two-array weights (16 MiB
+    # table + tiles and corrections: 32 MiB + 16 KiB per conjugate-free stream)
+    w = pir.batch_scratch_bytes(1024, 8, 1) - pir.batch_scratch_bytes(1024, 7, 1)
+    assert w >= (16 << 20) + (32 << 20) + (16 << 10)
+    assert pir.batch_scratch_bytes(1024, 7, 2) - pir.batch_scratch_bytes(1024, 6, 2) < (1 << 20)
     assert pir.batch_scratch_bytes(512, 4, 1) == 0      # rotated shard layout
     assert pir.batch_scratch_bytes(1024, 0, 1) == 0
     assert pir.batch_scratch_bytes(1024, 257, 1) == 0

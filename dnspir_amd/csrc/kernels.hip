@@ -1898,13 +1898,14 @@ __global__ __launch_bounds__(256) void k1w_tiles_kernel(const uint32_t* __restri
 
 // The two-stream inner product: the k1d arithmetic (v_dot4 over 4 digit
 // pairs, 7 weight accumulators) on the TE and IPGK1 K-steps only. A block is
-// one a-group of 8 shards (one wave each): the weights of that a-group for
+// one a-group of 16 shards (one wave each): the weights of that a-group for
 // every K-step (nsp x 32 tiles of 512 B) are staged in LDS once and read
 // like the query digits of k1d, so the loop has no barrier. Grid x = a-group:
 // the blocks in flight cover all a-groups of a few shard groups, so each XCD
 // holds the 64 KiB weight tiles of its 32 a-groups in L2 while the shard
 // groups pass (measured: 86% of HBM peak, against 80-84% with the shard
-// group fastest, which keeps pages of all 256 shards open at once).
+// group fastest, which keeps pages of all 256 shards open at once; 16
+// shards per block fetch the weight tiles half as often as 8).
 template <int R, int PD, int U>
 __device__ __forceinline__ void k1w_body(const uint32_t* const* __restrict__ slabs, int nshards, const uint32_t* __restrict__ W,
                                          uint32_t* __restrict__ outb, int nst, const u32x4* wl, int g, int sg, int half) {
@@ -1912,7 +1913,7 @@ __device__ __forceinline__ void k1w_body(const uint32_t* const* __restrict__ sla
     const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63, h = l >> 5;
     const int ai = (l & 31) >> 3;
     const int nk = nsp * 32;
-    const int sh = sg * 8 + wave;
+    const int sh = sg * 16 + wave;
     if (sh >= nshards) return;
     const ARow arow = a_row(slabs[sh], R, half, g, S);
     int acc[7];
@@ -1921,7 +1922,7 @@ __device__ __forceinline__ void k1w_body(const uint32_t* const* __restrict__ sla
     // K-step q: s' = q / 32 -> slab stream s = 3 st + (0: TE, 2: IPGK1), chunk
     // q % 32, i.e. slab K-step q + 32 * ((s' + 1) / 2). Each shard walks the
     // K-steps from its own starting point (the integer sums do not depend on
-    // the order), so the 8 shards of a block, which stream the same a-group,
+    // the order), so the 16 shards of a block, which stream the same a-group,
     // read at different offsets. Loads are unconditional (index clamped to the
     // last step).
     const int rot = (sh * 37) & (nk - 1);
@@ -1982,14 +1983,14 @@ __device__ __forceinline__ void k1w_body(const uint32_t* const* __restrict__ sla
 }
 
 template <int PD, int U>
-__global__ __launch_bounds__(512) void k1w_kernel(const uint32_t* const* __restrict__ slabs, int nshards,
+__global__ __launch_bounds__(1024) void k1w_kernel(const uint32_t* const* __restrict__ slabs, int nshards,
                                                   const uint32_t* __restrict__ W, uint32_t* __restrict__ outb, int nst) {
     extern __shared__ u32x4 wl[];  // [nsp * 32 K-steps][32]
     const int g = blockIdx.x, sg = blockIdx.y;
     const int half = blockIdx.z & 1, r = blockIdx.z >> 1;
     const int nk = 2 * nst * 32;
     const u32x4* src = reinterpret_cast<const u32x4*>(W) + ((size_t)(r * 2 + half) * (HALFN / 4) + g) * nk * 32;
-    for (int i = threadIdx.x; i < nk * 32; i += 512) wl[i] = src[i];
+    for (int i = threadIdx.x; i < nk * 32; i += 1024) wl[i] = src[i];
     __syncthreads();
     if (r == 0) k1w_body<0, PD, U>(slabs, nshards, W, outb, nst, wl, g, sg, half);
     else k1w_body<1, PD, U>(slabs, nshards, W, outb, nst, wl, g, sg, half);
@@ -2001,7 +2002,7 @@ void launch_k1w(const uint32_t* const* slabs, int nshards, const uint32_t* qp, u
     hipLaunchKernelGGL(k1w_tiles_kernel, dim3(HALFN / 16, 2, 2), dim3(256), 0, s, qp, scan, tiles, nst, T);
     if (main_start) (void)hipEventRecord(main_start, s);
     const size_t lds = (size_t)2 * nst * 32 * 512;
-    hipLaunchKernelGGL((k1w_kernel<4, 32>), dim3(HALFN / 4, (nshards + 7) / 8, 4), dim3(512), lds, s, slabs, nshards, tiles,
+    hipLaunchKernelGGL((k1w_kernel<4, 32>), dim3(HALFN / 4, (nshards + 15) / 16, 4), dim3(1024), lds, s, slabs, nshards, tiles,
                        outb, nst);
 }
 

@@ -88,6 +88,9 @@ SIGNATURES = {
     "dnspir_shard_create": (ci, [ci, ci, vp, sz, ctypes.POINTER(vp)]),
     "dnspir_shard_destroy": (ci, [vp]),
     "dnspir_shard_group_order": (ci, [vp]),
+    "dnspir_shard_save": (ci, [vp, ctypes.c_char_p]),
+    "dnspir_shard_file_info": (ci, [ctypes.c_char_p, ctypes.POINTER(ci), ctypes.POINTER(ci), ctypes.POINTER(sz)]),
+    "dnspir_shard_load": (ci, [ctypes.c_char_p, vp, sz, ctypes.POINTER(vp)]),
     "dnspir_shard_set_db": (ci, [vp, u16p, sz]),
     "dnspir_shard_set_db_device": (ci, [vp, vp, sz]),
     "dnspir_shard_get_a": (ci, [vp, u64p]),

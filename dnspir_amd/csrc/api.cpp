@@ -44,6 +44,7 @@ struct Global {
     std::mutex mu;
     int device = -1;
     DevTables* tables = nullptr;
+    uint64_t layout = 0;  // fingerprint of the device shard layout (persisted shard files)
     hipStream_t pre_stream = nullptr;
     DevBuf pre_entries, pre_M, pre_te, pre_ip, pre_aout, pre_idx;
     dnspir_ctx* default_ctx = nullptr;
@@ -472,6 +473,13 @@ int dnspir_init(int device) {
         delete h;
         return rc;
     }
+    // what a persisted shard's bytes depend on besides its entries: the slot
+    // order of the NTT (Galois positions -> slots), the roots, the residue
+    // packing of the slab
+    uint64_t fp = 0xcbf29ce484222325ull ^ 0x2625u;
+    for (int i = 0; i < N; ++i) fp = (fp ^ h->gpos_slot[i]) * 0x100000001b3ull;
+    for (int r = 0; r < 2; ++r) fp = (fp ^ h->psi[r][1]) * 0x100000001b3ull;
+    fp = (fp ^ (MB_BLOCK_BYTES_P1 * 65536ull + MB_BLOCK_BYTES_P2)) * 0x100000001b3ull;
     DevTables* d = nullptr;
     if (hipMalloc(&d, sizeof(DevTables)) != hipSuccess) {
         delete h;
@@ -483,6 +491,7 @@ int dnspir_init(int device) {
     }
     delete h;
     g.tables = d;
+    g.layout = fp;
     g.device = device;
     HIP_OK(hipStreamCreateWithFlags(&g.pre_stream, hipStreamNonBlocking));
     g.default_ctx = new dnspir_ctx;
@@ -519,6 +528,151 @@ int dnspir_shard_destroy(dnspir_shard* s) {
 }
 
 int dnspir_shard_group_order(const dnspir_shard* s) { return s ? s->l : 0; }
+
+// ---- persisted shards: a preprocessed shard's device state in a file --------
+// 64-byte header, then the dnspir_required_bytes(l) bytes of the shard's
+// device memory (slab, a-part, its 17-bit form) as they lie in HBM.
+namespace {
+constexpr char SHARD_MAGIC[8] = {'D', 'N', 'S', 'P', 'I', 'R', 'S', 'H'};
+constexpr uint32_t SHARD_FILE_VERSION = 1;
+struct ShardFileHeader {
+    char magic[8];
+    uint32_t version, n;
+    int32_t group_order, used_seeds;
+    uint64_t layout;         // fingerprint of the device layout that wrote it
+    uint64_t payload_bytes;  // dnspir_required_bytes(group_order)
+    uint64_t checksum;       // of the payload, over 64-bit words
+    uint64_t reserved[2];
+};
+static_assert(sizeof(ShardFileHeader) == 64, "shard file header");
+constexpr size_t SHARD_FILE_CHUNK = (size_t)32 << 20;
+
+struct PayloadHash {
+    uint64_t h = 0x84222325cbf29ce4ull;
+    void add(const void* p, size_t n) {  // n is a multiple of 8
+        const uint64_t* w = static_cast<const uint64_t*>(p);
+        for (size_t i = 0; i < n / 8; ++i) {
+            h = (h ^ w[i]) * 0x100000001b3ull;
+            h ^= h >> 29;
+        }
+    }
+};
+
+int read_header(FILE* f, ShardFileHeader* hd) {
+    if (fread(hd, sizeof(*hd), 1, f) != 1) return DNSPIR_E_FORMAT;
+    if (memcmp(hd->magic, SHARD_MAGIC, 8) != 0 || hd->version != SHARD_FILE_VERSION || hd->n != (uint32_t)N ||
+        !valid_group(hd->group_order) || (hd->used_seeds != 0 && hd->used_seeds != 1) ||
+        hd->payload_bytes != dnspir_required_bytes(hd->group_order))
+        return DNSPIR_E_FORMAT;
+    return 0;
+}
+
+struct PinnedChunk {
+    void* p = nullptr;
+    ~PinnedChunk() {
+        if (p) (void)hipHostFree(p);
+    }
+};
+}  // namespace
+
+int dnspir_shard_save(const dnspir_shard* sh, const char* path) {
+    if (!sh || !path) return DNSPIR_E_ARG;
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (!sh->has_db) return DNSPIR_E_ARG;
+    std::lock_guard<std::mutex> lk(g.mu);
+    HIP_OK(hipStreamSynchronize(g.pre_stream));
+    const size_t total = dnspir_required_bytes(sh->l);
+    PinnedChunk buf;
+    HIP_OK(hipHostMalloc(&buf.p, std::min(total, SHARD_FILE_CHUNK), hipHostMallocDefault));
+    FILE* f = fopen(path, "wb");
+    if (!f) return DNSPIR_E_IO;
+    ShardFileHeader hd;
+    memset(&hd, 0, sizeof(hd));
+    memcpy(hd.magic, SHARD_MAGIC, 8);
+    hd.version = SHARD_FILE_VERSION;
+    hd.n = N;
+    hd.group_order = sh->l;
+    hd.used_seeds = sh->seeds;
+    hd.layout = g.layout;
+    hd.payload_bytes = total;
+    bool ok = fwrite(&hd, sizeof(hd), 1, f) == 1;
+    PayloadHash hash;
+    for (size_t off = 0; ok && off < total; off += SHARD_FILE_CHUNK) {
+        const size_t n = std::min(SHARD_FILE_CHUNK, total - off);
+        if (hipMemcpy(buf.p, reinterpret_cast<const char*>(sh->slab) + off, n, hipMemcpyDeviceToHost) != hipSuccess) {
+            fclose(f);
+            HIP_OK(hipGetLastError());
+            return DNSPIR_E_HIP;
+        }
+        hash.add(buf.p, n);
+        ok = fwrite(buf.p, 1, n, f) == n;
+    }
+    hd.checksum = hash.h;
+    ok = ok && fseek(f, 0, SEEK_SET) == 0 && fwrite(&hd, sizeof(hd), 1, f) == 1;
+    ok = (fclose(f) == 0) && ok;
+    return ok ? 0 : DNSPIR_E_IO;
+}
+
+int dnspir_shard_file_info(const char* path, int* group_order, int* used_seeds, size_t* bytes) {
+    if (!path) return DNSPIR_E_ARG;
+    FILE* f = fopen(path, "rb");
+    if (!f) return DNSPIR_E_IO;
+    ShardFileHeader hd;
+    const int rc = read_header(f, &hd);
+    fclose(f);
+    if (rc) return rc;
+    if (group_order) *group_order = hd.group_order;
+    if (used_seeds) *used_seeds = hd.used_seeds;
+    if (bytes) *bytes = (size_t)hd.payload_bytes;
+    return 0;
+}
+
+int dnspir_shard_load(const char* path, void* dev_mem, size_t bytes, dnspir_shard** out) {
+    if (!path || !dev_mem || !out) return DNSPIR_E_ARG;
+    int rc = ensure_init();
+    if (rc) return rc;
+    FILE* f = fopen(path, "rb");
+    if (!f) return DNSPIR_E_IO;
+    ShardFileHeader hd;
+    if ((rc = read_header(f, &hd)) || hd.layout != g.layout) {
+        fclose(f);
+        return rc ? rc : DNSPIR_E_FORMAT;
+    }
+    dnspir_shard* s = nullptr;
+    if ((rc = dnspir_shard_create(hd.group_order, hd.used_seeds, dev_mem, bytes, &s))) {
+        fclose(f);
+        return rc;
+    }
+    const size_t total = hd.payload_bytes;
+    PinnedChunk buf;
+    if (hipHostMalloc(&buf.p, std::min(total, SHARD_FILE_CHUNK), hipHostMallocDefault) != hipSuccess) {
+        fclose(f);
+        delete s;
+        HIP_OK(hipGetLastError());
+        return DNSPIR_E_OOM;
+    }
+    PayloadHash hash;
+    rc = 0;
+    for (size_t off = 0; !rc && off < total; off += SHARD_FILE_CHUNK) {
+        const size_t n = std::min(SHARD_FILE_CHUNK, total - off);
+        if (fread(buf.p, 1, n, f) != n) rc = DNSPIR_E_FORMAT;  // short file
+        else {
+            hash.add(buf.p, n);
+            if (hipMemcpy(static_cast<char*>(dev_mem) + off, buf.p, n, hipMemcpyHostToDevice) != hipSuccess) rc = DNSPIR_E_HIP;
+        }
+    }
+    if (!rc && (hash.h != hd.checksum || fgetc(f) != EOF)) rc = DNSPIR_E_FORMAT;
+    fclose(f);
+    if (rc) {
+        delete s;
+        if (rc == DNSPIR_E_HIP) HIP_OK(hipGetLastError());
+        return rc;
+    }
+    s->has_db = true;
+    *out = s;
+    return 0;
+}
 
 static int set_db_impl(dnspir_shard* sh, const uint16_t* entries, size_t n_entries, bool on_device) {
     if (!sh || !entries) return DNSPIR_E_ARG;

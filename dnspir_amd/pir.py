@@ -11,6 +11,7 @@ borrows a slab, base_pir.rs:280-300); here a torch uint8 tensor on the GPU
 holds it.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -40,10 +41,18 @@ def required_bytes(group_order):
     return lib().dnspir_required_bytes(group_order)
 
 
+def shard_file_info(path):
+    """(group_order, used_seeds, payload bytes) from a shard file's header."""
+    l, seeds, nb = ctypes.c_int(), ctypes.c_int(), ctypes.c_size_t()
+    check(lib().dnspir_shard_file_info(os.fsencode(path), ctypes.byref(l), ctypes.byref(seeds), ctypes.byref(nb)),
+          "dnspir_shard_file_info")
+    return l.value, seeds.value, nb.value
+
+
 class PIRDatabase:
     """One preprocessed shard (base_pir.rs:225-1247) resident in HBM."""
 
-    def __init__(self, group_order=N, used_seeds=0, device=0, memory=None):
+    def __init__(self, group_order=N, used_seeds=0, device=0, memory=None, _load_path=None):
         import torch
         init(device)
         self.group_order = group_order
@@ -55,10 +64,26 @@ class PIRDatabase:
         self.memory = memory  # keeps the slab alive
         base = memory.data_ptr()
         aligned = (base + 255) & ~255
+        # the bytes the caller's memory actually holds past the aligned start
+        avail = max(0, memory.numel() * memory.element_size() - (aligned - base))
         h = ctypes.c_void_p()
-        check(lib().dnspir_shard_create(group_order, used_seeds, ctypes.c_void_p(aligned),
-                                        nbytes + 256 - (aligned - base), ctypes.byref(h)), "dnspir_shard_create")
+        if _load_path is None:
+            check(lib().dnspir_shard_create(group_order, used_seeds, ctypes.c_void_p(aligned), avail, ctypes.byref(h)),
+                  "dnspir_shard_create")
+        else:
+            check(lib().dnspir_shard_load(os.fsencode(_load_path), ctypes.c_void_p(aligned), avail, ctypes.byref(h)),
+                  "dnspir_shard_load")
         self.h = h
+
+    def save(self, path):
+        """Writes the preprocessed shard's device state to `path` (checkpoint)."""
+        check(lib().dnspir_shard_save(self.h, os.fsencode(path)), "dnspir_shard_save")
+
+    @classmethod
+    def load(cls, path, device=0, memory=None):
+        """A shard resumed from a file written by save(), without preprocessing."""
+        l, seeds, _ = shard_file_info(path)
+        return cls(l, seeds, device, memory, _load_path=path)
 
     def __del__(self):
         h = getattr(self, "h", None)

@@ -35,6 +35,8 @@ extern "C" {
 #define DNSPIR_E_NOINIT (-10)     /* dnspir_init not called / no device */
 #define DNSPIR_E_NOKEY (-11)      /* keyless query, but the client's Galois keys are not cached */
 #define DNSPIR_E_CLOSED (-12)     /* the server is shutting down */
+#define DNSPIR_E_IO (-13)         /* a shard file could not be opened, written or closed */
+#define DNSPIR_E_FORMAT (-14)     /* not a shard file of this build's layout, or corrupt / truncated */
 
 typedef struct dnspir_shard dnspir_shard;
 typedef struct dnspir_ctx dnspir_ctx;
@@ -85,6 +87,21 @@ int dnspir_shard_set_db(dnspir_shard* shard, const uint16_t* entries, size_t n_e
  * runs on the library's own stream: work that produces `entries_dev` on other
  * streams must be complete before the call. */
 int dnspir_shard_set_db_device(dnspir_shard* shard, const uint16_t* entries_dev, size_t n_entries);
+/* Persisted shards (checkpoint / resume of preprocessing; no counterpart in
+ * the reference, whose server preprocesses at start-up, server.rs:664-739).
+ * dnspir_shard_save writes a preprocessed shard's device state to `path`: a
+ * 64-byte header (magic "DNSPIRSH", version, N, group order, used seeds, a
+ * fingerprint of this build's device layout, payload size, checksum), then the
+ * dnspir_required_bytes(group_order) bytes of its device memory.
+ * dnspir_shard_file_info reads the header (no device needed).
+ * dnspir_shard_load checks the header against this build, copies the state
+ * into caller-owned device memory as dnspir_shard_create takes it, verifies
+ * the checksum and returns a shard ready for queries. DNSPIR_E_IO: the file
+ * cannot be opened or written; DNSPIR_E_FORMAT: wrong magic, version, layout
+ * or size, a truncated file or a checksum mismatch. */
+int dnspir_shard_save(const dnspir_shard* shard, const char* path);
+int dnspir_shard_file_info(const char* path, int* group_order, int* used_seeds, size_t* bytes);
+int dnspir_shard_load(const char* path, void* dev_mem, size_t bytes, dnspir_shard** out);
 /* PIRDatabase::get_a (base_pir.rs:897-903): out [8][N] host */
 int dnspir_shard_get_a(const dnspir_shard* shard, uint64_t* out);
 /* PIRDatabase::perform_pir (base_pir.rs:945-952): qb [N], qb_conj [N] (NULL

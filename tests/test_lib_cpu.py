@@ -125,3 +125,30 @@ def test_server_needs_a_device():
     assert _lib.lib().dnspir_server_create(ctypes.byref(bad), ctypes.byref(h)) == -5
     bad = _lib.ServerConfig(1, 1, 1 << 20, 0, -1.0)
     assert _lib.lib().dnspir_server_create(ctypes.byref(bad), ctypes.byref(h)) in (-5, -10)
+
+
+def _shard_header(l=1024, seeds=0, magic=b"DNSPIRSH", version=1, n=2048, payload=None):
+    import struct
+    if payload is None:
+        payload = _lib.lib().dnspir_required_bytes(l)
+    return struct.pack("<8sIIiiQQQQQ", magic, version, n, l, seeds, 0x1234, payload, 0, 0, 0)
+
+
+def test_shard_file_header(tmp_path):
+    """Shard files: the header is read without a device; wrong magic, version,
+    ring size or payload size, a truncated header and a missing file are
+    rejected with DNSPIR_E_FORMAT / DNSPIR_E_IO."""
+    good = tmp_path / "good.shard"
+    good.write_bytes(_shard_header(2048, 1))
+    assert pir.shard_file_info(str(good)) == (2048, 1, _lib.lib().dnspir_required_bytes(2048))
+    for name, data in [("magic", _shard_header(magic=b"DNSPIRXX")), ("version", _shard_header(version=2)),
+                       ("ring", _shard_header(n=4096)), ("size", _shard_header(payload=12345)),
+                       ("group", _shard_header(l=1536, payload=0)), ("short", _shard_header()[:40])]:
+        f = tmp_path / f"{name}.shard"
+        f.write_bytes(data)
+        with pytest.raises(_lib.DnspirError) as ei:
+            pir.shard_file_info(str(f))
+        assert ei.value.code == -14, name
+    with pytest.raises(_lib.DnspirError) as ei:
+        pir.shard_file_info(str(tmp_path / "missing.shard"))
+    assert ei.value.code == -13

@@ -257,8 +257,9 @@ int run_primary(dnspir_ctx* c, const uint32_t* const* slabs, int len, int l_p, c
     if (L_p == N / 2) {
         if ((rc = c->outb.ensure((size_t)nq * len * 2 * N * LANES * 4))) return rc;
         if (use_k1w(c, l_p, (size_t)len, nq)) {
-            if ((rc = c->wscan.ensure(k1w_scan_bytes())) || (rc = c->wtile.ensure(k1w_tile_bytes(nst_p)))) return rc;
-            launch_k1w(slabs, len, qp, c->wscan.as<uint32_t>(), c->wtile.as<uint32_t>(), c->outb.as<uint32_t>(), nst_p,
+            if ((rc = c->wscan.ensure(nq * k1w_scan_bytes())) || (rc = c->wtile.ensure(nq * k1w_tile_bytes(nst_p))))
+                return rc;
+            launch_k1w(slabs, len, qp, nq, c->wscan.as<uint32_t>(), c->wtile.as<uint32_t>(), c->outb.as<uint32_t>(), nst_p,
                        g.tables, s, k1_start);
             if (bytes) *bytes = *bytes / 3 * 2;  // the TE and IPGK1 streams
         } else {
@@ -370,18 +371,24 @@ int run_double(dnspir_ctx* c, const dnspir_shard* const* fleets[2], const size_t
     uint64_t streamed = 0, ref_streamed = 0;
     c->k1_launches = 0;
     c->k1_timed = true;
+    // both slots on the same fleet: one two-query pass over the shards (the
+    // two-stream inner product takes two queries; qp holds them back to back)
+    const bool joint = present[0] && present[1] && lens[0] == lens[1] &&
+                       std::equal(fleets[0], fleets[0] + lens[0], fleets[1]) && use_k1w(c, l_p, lens[0], 2);
     for (int sl = 0; sl < 2; ++sl) {
         if (!present[sl]) continue;
         const int len = (int)lens[sl];
-        bool ob = false;
+        bool ob = true;
         uint64_t bytes = 0;
-        if ((rc = run_primary(c, slabs[sl], len, l_p, c->qp.as<uint32_t>() + (size_t)sl * 4 * 2 * N, 1, &ob,
-                              c->k1_launches == 0 ? c->ev[3] : nullptr, &bytes)))
-            return rc;
-        if (c->k1_launches == 0) HIP_OK(hipEventRecord(c->ev[4], s));
-        ++c->k1_launches;
+        if (!joint || sl == 0) {
+            if ((rc = run_primary(c, slabs[sl], len, l_p, c->qp.as<uint32_t>() + (size_t)sl * 4 * 2 * N, joint ? 2 : 1, &ob,
+                                  c->k1_launches == 0 ? c->ev[3] : nullptr, &bytes)))
+                return rc;
+            if (c->k1_launches == 0) HIP_OK(hipEventRecord(c->ev[4], s));
+            ++c->k1_launches;
+        }
         EpiLaunch e;
-        if (ob) e.outb = c->outb.as<uint32_t>();
+        if (ob) e.outb = c->outb.as<uint32_t>() + (joint ? (size_t)sl * len * 2 * N * LANES : 0);
         else e.acc = c->acc.as<unsigned long long>();
         e.nshards = len;
         e.nq = 1;
@@ -1039,8 +1046,8 @@ int dne::batch_reserve(dnspir_ctx* c, int l_p, size_t len, int nq, size_t qlen) 
         (size_t)nl * N * 8,                          // bcoef
         (size_t)nl * N * 4,                          // a17o
         (size_t)nl * N * 4,                          // b12o
-        use_k1w(c, l_p, len, nq) ? k1w_scan_bytes() : 0,         // wscan
-        use_k1w(c, l_p, len, nq) ? k1w_tile_bytes(nst_p) : 0,    // wtile
+        use_k1w(c, l_p, len, nq) ? nq * k1w_scan_bytes() : 0,         // wscan
+        use_k1w(c, l_p, len, nq) ? nq * k1w_tile_bytes(nst_p) : 0,    // wtile
     };
     DevBuf* bufs[] = {&c->qcoef, &c->qs, &c->qp, &c->se, &c->ql, &c->outb, &c->M, &c->te,
                       &c->aout, &c->cout, &c->acoef, &c->bcoef, &c->a17o, &c->b12o, &c->wscan, &c->wtile};

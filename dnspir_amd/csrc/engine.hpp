@@ -183,10 +183,11 @@ constexpr int PAIR_MAX_CHAINS = 128;
 // product for single queries (below, preparing the weights costs about what
 // the third stream would)
 constexpr int K1W_MIN_SHARDS = 8;
-// whether a one-query primary pass over `len` shards of group order l_p on
-// this context takes the two-stream inner product
+// whether a primary pass of nq <= 2 queries over `len` shards of group order
+// l_p on this context takes the two-stream inner product
 inline bool use_k1w(const dnspir_ctx* c, int l_p, size_t len, int nq) {
-    return nq == 1 && 2 * l_p >= N && (c->k1_mode == 2 || (c->k1_mode == 0 && len >= (size_t)K1W_MIN_SHARDS));
+    return (nq == 1 || nq == 2) && 2 * l_p >= N &&
+           (c->k1_mode == 2 || (c->k1_mode == 0 && len >= (size_t)K1W_MIN_SHARDS));
 }
 // after the stream sync that follows a call: DNSPIR_E_HIP if that call's
 // two-CU chain timed out waiting for a partner block

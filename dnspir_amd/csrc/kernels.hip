@@ -1769,7 +1769,7 @@ __global__ __launch_bounds__(256) void k1d_kernel(const uint32_t* const* __restr
 // fleet share them.
 // ---------------------------------------------------------------------------
 
-// C[r][h][M][j] (u32) for one query. The recurrence over M (c += gkb0[j-M] P,
+// C[q][r][h][M][j] (u32). The recurrence over M (c += gkb0[j-M] P,
 // P *= gka0[j-M]) is affine in (c, P), so it is split into 16 segments of 64
 // steps: each thread (j, segment) first forms its segment's own product and
 // sum from (0, 1), the segments' starting values come from a prefix over the
@@ -1780,11 +1780,11 @@ __global__ __launch_bounds__(1024) void k1w_scan_kernel(const uint32_t* __restri
     constexpr int SEG = 16, LEN = HALFN / SEG;
     __shared__ uint32_t g0s[HALFN], gas[HALFN];
     __shared__ uint32_t segc[SEG][64], segp[SEG][64];
-    const int h = blockIdx.y, r = blockIdx.z;
+    const int h = blockIdx.y, r = blockIdx.z & 1, q = blockIdx.z >> 1;
     const uint32_t p = mod_p(r);
     const int K = mod_k(r);
     const int tid = threadIdx.x, jj = tid & 63, seg = tid >> 6;
-    const uint32_t* g0 = qp + (size_t)r * N + h * HALFN;  // vector 0 (gkb0) of the query
+    const uint32_t* g0 = qp + ((size_t)q * 4 * 2 + r) * N + h * HALFN;  // vector 0 (gkb0) of query q
     for (int y = tid; y < HALFN; y += 1024) {
         g0s[y] = g0[y];
         gas[y] = T->gka[0][r][T->gpos_slot[h * HALFN + y]];
@@ -1807,7 +1807,7 @@ __global__ __launch_bounds__(1024) void k1w_scan_kernel(const uint32_t* __restri
         c0 = add_mod(c0, mul_mod(P0, segc[k][jj], p, K), p);
         P0 = mul_mod(P0, segp[k][jj], p, K);
     }
-    uint32_t* out = C + ((size_t)r * 2 + h) * HALFN * HALFN + j;
+    uint32_t* out = C + ((size_t)(q * 2 + r) * 2 + h) * HALFN * HALFN + j;
     if (seg == 0) out[0] = 0;
     c = c0;
     P = P0;
@@ -1898,7 +1898,7 @@ __global__ __launch_bounds__(256) void k1w_tiles_kernel(const uint32_t* __restri
 
 // The two-stream inner product: the k1d arithmetic (v_dot4 over 4 digit
 // pairs, 7 weight accumulators) on the TE and IPGK1 K-steps only. A block is
-// one a-group of 16 shards (one wave each): the weights of that a-group for
+// one a-group of 16 shards (one wave each; 8 for two queries): the weights of that a-group for
 // every K-step (nsp x 32 tiles of 512 B) are staged in LDS once and read
 // like the query digits of k1d, so the loop has no barrier. Grid x = a-group:
 // the blocks in flight cover all a-groups of a few shard groups, so each XCD
@@ -1906,19 +1906,22 @@ __global__ __launch_bounds__(256) void k1w_tiles_kernel(const uint32_t* __restri
 // groups pass (measured: 86% of HBM peak, against 80-84% with the shard
 // group fastest, which keeps pages of all 256 shards open at once; 16
 // shards per block fetch the weight tiles half as often as 8).
-template <int R, int PD, int U>
+template <int R, int PD, int U, int NQ, int SPB>
 __device__ __forceinline__ void k1w_body(const uint32_t* const* __restrict__ slabs, int nshards, const uint32_t* __restrict__ W,
-                                         uint32_t* __restrict__ outb, int nst, const u32x4* wl, int g, int sg, int half) {
+                                         uint32_t* __restrict__ outb, int nst, size_t q_stride, const u32x4* wl, int g, int sg,
+                                         int half) {
     const int S = 3 * nst, nsp = 2 * nst;
     const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63, h = l >> 5;
     const int ai = (l & 31) >> 3;
     const int nk = nsp * 32;
-    const int sh = sg * 16 + wave;
+    const int sh = sg * SPB + wave;
     if (sh >= nshards) return;
     const ARow arow = a_row(slabs[sh], R, half, g, S);
-    int acc[7];
+    int acc[NQ][7];
 #pragma unroll
-    for (int w = 0; w < 7; ++w) acc[w] = 0;
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+        for (int w = 0; w < 7; ++w) acc[q][w] = 0;
     // K-step q: s' = q / 32 -> slab stream s = 3 st + (0: TE, 2: IPGK1), chunk
     // q % 32, i.e. slab K-step q + 32 * ((s' + 1) / 2). Each shard walks the
     // K-steps from its own starting point (the integer sums do not depend on
@@ -1950,20 +1953,23 @@ __device__ __forceinline__ void k1w_body(const uint32_t* const* __restrict__ sla
             // keep the prefetch here: without this the scheduler moves each
             // load down to its use and only one step stays in flight
             __builtin_amdgcn_sched_barrier(0);
-            u32x4 b[4];
 #pragma unroll
-            for (int v = 0; v < 4; ++v) b[v] = wl[((kk + rot) & (nk - 1)) * 32 + v * 8 + lb];
+            for (int q = 0; q < NQ; ++q) {
+                u32x4 b[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
+                for (int v = 0; v < 4; ++v) b[v] = wl[(q * nk + ((kk + rot) & (nk - 1))) * 32 + v * 8 + lb];
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    int t = acc[u + v];
-                    t = __builtin_amdgcn_sdot4(a[u].x, (int)b[v].x, t, false);
-                    t = __builtin_amdgcn_sdot4(a[u].y, (int)b[v].y, t, false);
-                    t = __builtin_amdgcn_sdot4(a[u].z, (int)b[v].z, t, false);
-                    t = __builtin_amdgcn_sdot4(a[u].w, (int)b[v].w, t, false);
-                    acc[u + v] = t;
-                }
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        int t = acc[q][u + v];
+                        t = __builtin_amdgcn_sdot4(a[u].x, (int)b[v].x, t, false);
+                        t = __builtin_amdgcn_sdot4(a[u].y, (int)b[v].y, t, false);
+                        t = __builtin_amdgcn_sdot4(a[u].z, (int)b[v].z, t, false);
+                        t = __builtin_amdgcn_sdot4(a[u].w, (int)b[v].w, t, false);
+                        acc[q][u + v] = t;
+                    }
+            }
         }
     }
     const uint32_t p = mod_p(R);
@@ -1972,38 +1978,52 @@ __device__ __forceinline__ void k1w_body(const uint32_t* const* __restrict__ sla
     weight_consts(p, cw);
     const int row = l & 31;
     const int a = g * 4 + (row >> 3), lane8 = row & 7;
-    const uint32_t* corr = W + (size_t)4 * (HALFN / 4) * nsp * 32 * 128;
-    int64_t t = corr[(R * 2 + half) * HALFN + a];
 #pragma unroll
-    for (int w = 0; w < 7; ++w) {
-        const int tot = acc[w] + __shfl_xor(acc[w], 32);
-        t += (int64_t)tot * cw[w];
+    for (int q = 0; q < NQ; ++q) {
+        const uint32_t* corr = W + q * (k1w_tile_bytes(nst) / 4) + (size_t)4 * (HALFN / 4) * nsp * 32 * 128;
+        int64_t t = corr[(R * 2 + half) * HALFN + a];
+#pragma unroll
+        for (int w = 0; w < 7; ++w) {
+            const int tot = acc[q][w] + __shfl_xor(acc[q][w], 32);
+            t += (int64_t)tot * cw[w];
+        }
+        if (h == 0)
+            outb[(size_t)q * q_stride + (((size_t)sh * 2 + R) * N + half * HALFN + a) * LANES + lane8] = from_i64(t, p, K);
     }
-    if (h == 0) outb[(((size_t)sh * 2 + R) * N + half * HALFN + a) * LANES + lane8] = from_i64(t, p, K);
 }
 
-template <int PD, int U>
-__global__ __launch_bounds__(1024) void k1w_kernel(const uint32_t* const* __restrict__ slabs, int nshards,
-                                                  const uint32_t* __restrict__ W, uint32_t* __restrict__ outb, int nst) {
-    extern __shared__ u32x4 wl[];  // [nsp * 32 K-steps][32]
+template <int PD, int U, int NQ, int SPB>
+__global__ __launch_bounds__(64 * SPB) void k1w_kernel(const uint32_t* const* __restrict__ slabs, int nshards,
+                                                       const uint32_t* __restrict__ W, uint32_t* __restrict__ outb, int nst,
+                                                       size_t q_stride) {
+    extern __shared__ u32x4 wl[];  // [NQ][nsp * 32 K-steps][32]
     const int g = blockIdx.x, sg = blockIdx.y;
     const int half = blockIdx.z & 1, r = blockIdx.z >> 1;
     const int nk = 2 * nst * 32;
-    const u32x4* src = reinterpret_cast<const u32x4*>(W) + ((size_t)(r * 2 + half) * (HALFN / 4) + g) * nk * 32;
-    for (int i = threadIdx.x; i < nk * 32; i += 1024) wl[i] = src[i];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const u32x4* src = reinterpret_cast<const u32x4*>(W + q * (k1w_tile_bytes(nst) / 4)) +
+                           ((size_t)(r * 2 + half) * (HALFN / 4) + g) * nk * 32;
+        for (int i = threadIdx.x; i < nk * 32; i += 64 * SPB) wl[q * nk * 32 + i] = src[i];
+    }
     __syncthreads();
-    if (r == 0) k1w_body<0, PD, U>(slabs, nshards, W, outb, nst, wl, g, sg, half);
-    else k1w_body<1, PD, U>(slabs, nshards, W, outb, nst, wl, g, sg, half);
+    if (r == 0) k1w_body<0, PD, U, NQ, SPB>(slabs, nshards, W, outb, nst, q_stride, wl, g, sg, half);
+    else k1w_body<1, PD, U, NQ, SPB>(slabs, nshards, W, outb, nst, q_stride, wl, g, sg, half);
 }
 
-void launch_k1w(const uint32_t* const* slabs, int nshards, const uint32_t* qp, uint32_t* scan, uint32_t* tiles,
+void launch_k1w(const uint32_t* const* slabs, int nshards, const uint32_t* qp, int nq, uint32_t* scan, uint32_t* tiles,
                 uint32_t* outb, int nst, const DevTables* T, hipStream_t s, hipEvent_t main_start) {
-    hipLaunchKernelGGL(k1w_scan_kernel, dim3(HALFN / 64, 2, 2), dim3(1024), 0, s, qp, scan, T);
-    hipLaunchKernelGGL(k1w_tiles_kernel, dim3(HALFN / 16, 2, 2), dim3(256), 0, s, qp, scan, tiles, nst, T);
+    hipLaunchKernelGGL(k1w_scan_kernel, dim3(HALFN / 64, 2, 2 * nq), dim3(1024), 0, s, qp, scan, T);
+    hipLaunchKernelGGL(k1w_tiles_kernel, dim3(HALFN / 16, 2, 2 * nq), dim3(256), 0, s, qp, scan, tiles, nst, T);
     if (main_start) (void)hipEventRecord(main_start, s);
-    const size_t lds = (size_t)2 * nst * 32 * 512;
-    hipLaunchKernelGGL((k1w_kernel<4, 32>), dim3(HALFN / 4, (nshards + 15) / 16, 4), dim3(1024), lds, s, slabs, nshards, tiles,
-                       outb, nst);
+    const size_t lds = (size_t)nq * 2 * nst * 32 * 512;
+    const size_t qstride = (size_t)nshards * 2 * N * LANES;
+    if (nq == 1)
+        hipLaunchKernelGGL((k1w_kernel<4, 32, 1, 16>), dim3(HALFN / 4, (nshards + 15) / 16, 4), dim3(1024), lds, s, slabs,
+                           nshards, tiles, outb, nst, qstride);
+    else  // two queries: both weight sets in LDS (128 KiB at full group order), 8 shards per block
+        hipLaunchKernelGGL((k1w_kernel<4, 32, 2, 8>), dim3(HALFN / 4, (nshards + 7) / 8, 4), dim3(512), lds, s, slabs,
+                           nshards, tiles, outb, nst, qstride);
 }
 
 void launch_relayout(const uint32_t* rows_tmp, uint32_t* slab, int L, int nst, hipStream_t s) {

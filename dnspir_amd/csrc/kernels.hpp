@@ -38,15 +38,15 @@ size_t qlimb_bytes(int nq, int nst);
 // main_start (optional) is recorded right before the inner-product launch
 void launch_k1b(const uint32_t* const* slabs, int nshards, const uint32_t* qp, int nq, uint32_t* ql, uint32_t* outb, int nst,
                 hipStream_t s, hipEvent_t main_start = nullptr);
-// Two-stream K1 for one query on limb-block shards (the outb of launch_k1b
-// with nq = 1): per-query weights folded from the IPGK0 stream, then an inner
-// product over the TE and IPGK1 streams only. scan: k1w_scan_bytes(), tiles:
-// k1w_tile_bytes(nst).
+// Two-stream K1 for one or two queries on limb-block shards (the outb of
+// launch_k1b): per-query weights folded from the IPGK0 stream, then an inner
+// product over the TE and IPGK1 streams only. scan: nq * k1w_scan_bytes(),
+// tiles: nq * k1w_tile_bytes(nst).
 __host__ __device__ constexpr size_t k1w_scan_bytes() { return (size_t)4 * dn::HALFN * dn::HALFN * 4; }
 __host__ __device__ constexpr size_t k1w_tile_bytes(int nst) {
     return (size_t)4 * (dn::HALFN / 4) * 2 * nst * 32 * 512 + (size_t)4 * dn::HALFN * 4;
 }
-void launch_k1w(const uint32_t* const* slabs, int nshards, const uint32_t* qp, uint32_t* scan, uint32_t* tiles,
+void launch_k1w(const uint32_t* const* slabs, int nshards, const uint32_t* qp, int nq, uint32_t* scan, uint32_t* tiles,
                 uint32_t* outb, int nst, const DevTables* T, hipStream_t s, hipEvent_t main_start = nullptr);
 // batched inner-product candidates (all exact), chosen per (column tiles, streams)
 int k1b_variant_count();

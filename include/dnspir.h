@@ -137,8 +137,9 @@ int dnspir_ctx_set_exchange_polls(dnspir_ctx* ctx, int max_polls);
  * into per-query weights (IPGK0 = sigma(a_i) - 2^26 IPGK1, with the chain
  * a_i linear in the stored rows), a third fewer bytes. mode 0 (default):
  * mode 2 for fleets of 8 or more shards, else mode 3. Every mode gives the
- * same bytes. Batches of 2 or more queries always stream three arrays. No
- * counterpart in the reference. */
+ * same bytes. Two queries over one fleet (both slots of a call on the same
+ * shards, or a two-query batch) share one pass; batches of 3 or more queries
+ * always stream three arrays. No counterpart in the reference. */
 int dnspir_ctx_set_k1_mode(dnspir_ctx* ctx, int mode);
 
 /* ---- two-level composition: perform_batched_double_pir (double_pir.rs:126-228)

@@ -166,9 +166,9 @@ def test_batch_graph_replays_match_eager(pir):
 
 
 def test_one_query_batches_two_stream_graph_replays(pir):
-    """One-query batches on a fleet of 9 entries take the two-array inner
-    product inside the captured graph (its weight buffers are reserved before
-    capture); every replay equals the three-array single-query reply."""
+    """One- and two-query batches on a fleet of 9 entries take the two-array
+    inner product inside the captured graph (its weight buffers are reserved
+    before capture); every replay equals the three-array single-query reply."""
     import torch
     g, _, e = _fleet(pir, 1024, 3)
     fleet = [g[k % 3] for k in range(9)]
@@ -191,4 +191,19 @@ def test_one_query_batches_two_stream_graph_replays(pir):
         assert got == want[0], rep
         assert np.array_equal(pir.process_reply(got, sk), e[si % 3][pi])
     assert ctx.set_graphs(True) is False  # capture worked: no eager fallback
+    # two-query batches: one two-query pass over the shards
+    q2 = torch.zeros(2 * qlen, dtype=torch.uint8, device="cuda")
+    r2 = torch.zeros(2 * rb, dtype=torch.uint8, device="cuda")
+    for rep in range(3):
+        pair = [qs[rep], qs[rep + 1]]
+        q2.copy_(torch.from_numpy(np.frombuffer(pair[0][2] + pair[1][2], np.uint8).copy()))
+        r2.zero_()
+        torch.cuda.synchronize()
+        _say(f"[batch] two-query batch replay {rep}")
+        pir.process_batch_device(fleet, q2, qlen, 2, r2, ctx=ctx)
+        got = r2.cpu().numpy().tobytes()
+        for k, (pi, si, q, sk) in enumerate(pair):
+            want, _ = pir.process_queries(fleet, None, q, None, ctx=ref)
+            assert got[k * rb:(k + 1) * rb] == want[0], (rep, k)
+    assert ctx.set_graphs(True) is False
 

@@ -265,6 +265,61 @@ __device__ __forceinline__ void inv_pass2_last(uint32_t* buf, const uint32_t* tw
     }
 }
 
+// The same radix-4 passes with each twiddle stored beside its Shoup companion
+// (tw2[k] = (w_k, wq_k), 16-byte aligned): one 8-byte LDS read for the first
+// stage's twiddle, one 16-byte read for the second stage's adjacent pair, so
+// a group issues 2 twiddle reads instead of 6.
+template <bool PAD = false>
+__device__ __forceinline__ void fwd_pass4gi(uint32_t* buf, const uint2* tw2, uint32_t p, int t, int g) {
+    const int m = 1 << t, lq = LOGN - t - 2, q = 1 << lq;
+    const int i = g >> lq, j0 = (i << (lq + 2)) + (g & (q - 1));
+    uint32_t v0 = buf[lx<PAD>(j0)], v1 = buf[lx<PAD>(j0 + q)], v2 = buf[lx<PAD>(j0 + 2 * q)], v3 = buf[lx<PAD>(j0 + 3 * q)];
+    const uint2 wa = tw2[m + i];
+    bf_ct(v0, v2, wa.x, wa.y, p);
+    bf_ct(v1, v3, wa.x, wa.y, p);
+    const uint4 wb = *reinterpret_cast<const uint4*>(tw2 + 2 * m + 2 * i);
+    bf_ct(v0, v1, wb.x, wb.y, p);
+    bf_ct(v2, v3, wb.z, wb.w, p);
+    buf[lx<PAD>(j0)] = v0; buf[lx<PAD>(j0 + q)] = v1; buf[lx<PAD>(j0 + 2 * q)] = v2; buf[lx<PAD>(j0 + 3 * q)] = v3;
+}
+template <bool PAD = false>
+__device__ __forceinline__ void inv_pass4gi(uint32_t* buf, const uint2* tw2, uint32_t p, int t, int g) {
+    const int m = 1 << t, lq = LOGN - t - 2, q = 1 << lq;
+    const int i = g >> lq, j0 = (i << (lq + 2)) + (g & (q - 1));
+    uint32_t v0 = buf[lx<PAD>(j0)], v1 = buf[lx<PAD>(j0 + q)], v2 = buf[lx<PAD>(j0 + 2 * q)], v3 = buf[lx<PAD>(j0 + 3 * q)];
+    const uint4 wb = *reinterpret_cast<const uint4*>(tw2 + 2 * m + 2 * i);
+    bf_gs(v0, v1, wb.x, wb.y, p);
+    bf_gs(v2, v3, wb.z, wb.w, p);
+    const uint2 wa = tw2[m + i];
+    bf_gs(v0, v2, wa.x, wa.y, p);
+    bf_gs(v1, v3, wa.x, wa.y, p);
+    buf[lx<PAD>(j0)] = v0; buf[lx<PAD>(j0 + q)] = v1; buf[lx<PAD>(j0 + 2 * q)] = v2; buf[lx<PAD>(j0 + 3 * q)] = v3;
+}
+template <bool PAD = false>
+__device__ __forceinline__ void fwd_pass2_lasti(uint32_t* buf, const uint2* tw2, uint32_t p, int g) {
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int i = g + 512 * k;
+        uint32_t x = buf[lx<PAD>(2 * i)], y = buf[lx<PAD>(2 * i + 1)];
+        const uint2 w = tw2[1024 + i];
+        bf_ct(x, y, w.x, w.y, p);
+        buf[lx<PAD>(2 * i)] = x;
+        buf[lx<PAD>(2 * i + 1)] = y;
+    }
+}
+template <bool PAD = false>
+__device__ __forceinline__ void inv_pass2_lasti(uint32_t* buf, const uint2* tw2, uint32_t p, int g) {
+    const uint2 w = tw2[1];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int j = g + 512 * k;
+        uint32_t x = buf[lx<PAD>(j)], y = buf[lx<PAD>(j + N / 2)];
+        bf_gs(x, y, w.x, w.y, p);
+        buf[lx<PAD>(j)] = x;
+        buf[lx<PAD>(j + N / 2)] = y;
+    }
+}
+
 // whole-block transforms by 256 threads (g = thread index) on an LDS array;
 // inputs in [0, 2p), outputs in [0, p) (forward) / [0, 2p) (inverse, which
 // leaves the N factor in)
@@ -1493,7 +1548,7 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
                                                           const int* __restrict__ lane_slot, int L, int nlanes, int nchains,
                                                           uint32_t* xbuf, unsigned* err, int spin_limit,
                                                           const DevTables* __restrict__ T) {
-    __shared__ uint32_t tw[2][N], twq[2][N];  // [fwd/inv][k], modulus r
+    __shared__ __attribute__((aligned(16))) uint2 tw2[2][N];  // [fwd/inv][k] = (twiddle, Shoup companion), modulus r
     __shared__ uint32_t sa[N];
     __shared__ uint32_t sc[2][N];      // [buffer][slot]
     __shared__ uint32_t xa[NPAD];      // inverse transform of sigma_5(a)
@@ -1506,8 +1561,8 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
     const uint32_t p = mod_p(r);
     const int K = mod_k(r);
     for (int k = tid; k < N; k += 1024) {
-        tw[0][k] = T->psi[r][k]; twq[0][k] = T->psiq[r][k];
-        tw[1][k] = T->ipsi[r][k]; twq[1][k] = T->ipsiq[r][k];
+        tw2[0][k] = make_uint2(T->psi[r][k], T->psiq[r][k]);
+        tw2[1][k] = make_uint2(T->ipsi[r][k], T->ipsiq[r][k]);
         sa[k] = 0;
         sc[0][k] = 0;
     }
@@ -1548,25 +1603,24 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
     for (int i = 0; i < L; ++i) {
         // ---- A: inverse NTT of sigma_5(a) mod p_r (threads 0..511, radix 4) ----
         if (tid < 512) {
-            const uint32_t* itw = tw[1];
-            const uint32_t* itwq = twq[1];
             const int ii = gA;
             uint32_t v0 = sa[sigA[0]], v1 = sa[sigA[1]], v2 = sa[sigA[2]], v3 = sa[sigA[3]];
-            bf_gs(v0, v1, itw[1024 + 2 * ii], itwq[1024 + 2 * ii], p);
-            bf_gs(v2, v3, itw[1025 + 2 * ii], itwq[1025 + 2 * ii], p);
-            const uint32_t w = itw[512 + ii], wq = itwq[512 + ii];
-            bf_gs(v0, v2, w, wq, p);
-            bf_gs(v1, v3, w, wq, p);
+            const uint4 wb = *reinterpret_cast<const uint4*>(&tw2[1][1024 + 2 * ii]);
+            bf_gs(v0, v1, wb.x, wb.y, p);
+            bf_gs(v2, v3, wb.z, wb.w, p);
+            const uint2 wa = tw2[1][512 + ii];
+            bf_gs(v0, v2, wa.x, wa.y, p);
+            bf_gs(v1, v3, wa.x, wa.y, p);
             const int b4 = lx<true>(4 * ii);
             xa[b4] = v0; xa[b4 + 1] = v1; xa[b4 + 2] = v2; xa[b4 + 3] = v3;
         }
         lds_barrier();
 #pragma unroll
         for (int t = 7; t >= 1; t -= 2) {
-            if (tid < 512) inv_pass4g<true>(xa, tw[1], twq[1], p, t, gA);
+            if (tid < 512) inv_pass4gi<true>(xa, tw2[1], p, t, gA);
             lds_barrier();
         }
-        if (tid < 512) inv_pass2_last<true>(xa, tw[1], twq[1], p, gA);
+        if (tid < 512) inv_pass2_lasti<true>(xa, tw2[1], p, gA);
         lds_barrier();
         // ---- exchange with the partner (other modulus) ----
         const uint32_t tag = (uint32_t)(i % 31 + 1) << 27;
@@ -1634,10 +1688,10 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
         // ---- C: forward NTT of D1 mod p_r (threads 0..511, radix 4) ----
 #pragma unroll
         for (int t = 0; t <= 8; t += 2) {
-            if (tid < 512) fwd_pass4g<true>(xd[0], tw[0], twq[0], p, t, gA);
+            if (tid < 512) fwd_pass4gi<true>(xd[0], tw2[0], p, t, gA);
             lds_barrier();
         }
-        if (tid < 512) fwd_pass2_last<true>(xd[0], tw[0], twq[0], p, gA);
+        if (tid < 512) fwd_pass2_lasti<true>(xd[0], tw2[0], p, gA);
         lds_barrier();
         // ---- D: update a mod p_r (c follows in the next step's exchange) ----
 #pragma unroll

@@ -295,6 +295,50 @@ __device__ __forceinline__ void inv_pass4gi(uint32_t* buf, const uint2* tw2, uin
     bf_gs(v1, v3, wa.x, wa.y, p);
     buf[lx<PAD>(j0)] = v0; buf[lx<PAD>(j0 + q)] = v1; buf[lx<PAD>(j0 + 2 * q)] = v2; buf[lx<PAD>(j0 + 3 * q)] = v3;
 }
+// radix-8 forward pass (stages 3P..3P+2) and the last radix-4 forward pass
+// (stages 9, 10 for groups g, g + 256) on the interleaved twiddle table
+template <bool PAD = false>
+__device__ __forceinline__ void fwd_pass8i(uint32_t* buf, const uint2* tw2, uint32_t p, int P, int g) {
+    const int m = 1 << (3 * P);
+    const int q = N >> (3 * P + 3);
+    const int i = g / q, r = g % q;
+    const int j0 = i * 8 * q + r;
+    uint32_t v[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) v[c] = buf[lx<PAD>(j0 + c * q)];
+    const uint2 w1 = tw2[m + i];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) bf_ct(v[c], v[c + 4], w1.x, w1.y, p);
+    const uint4 w2 = *reinterpret_cast<const uint4*>(tw2 + 2 * m + 2 * i);
+    bf_ct(v[0], v[2], w2.x, w2.y, p);
+    bf_ct(v[1], v[3], w2.x, w2.y, p);
+    bf_ct(v[4], v[6], w2.z, w2.w, p);
+    bf_ct(v[5], v[7], w2.z, w2.w, p);
+    const uint4 w3a = *reinterpret_cast<const uint4*>(tw2 + 4 * m + 4 * i);
+    const uint4 w3b = *reinterpret_cast<const uint4*>(tw2 + 4 * m + 4 * i + 2);
+    bf_ct(v[0], v[1], w3a.x, w3a.y, p);
+    bf_ct(v[2], v[3], w3a.z, w3a.w, p);
+    bf_ct(v[4], v[5], w3b.x, w3b.y, p);
+    bf_ct(v[6], v[7], w3b.z, w3b.w, p);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) buf[lx<PAD>(j0 + c * q)] = v[c];
+}
+template <bool PAD = false>
+__device__ __forceinline__ void fwd_pass4i(uint32_t* buf, const uint2* tw2, uint32_t p, int g) {
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int i = g + 256 * k;
+        const int b4 = lx<PAD>(4 * i);
+        uint32_t v0 = buf[b4], v1 = buf[b4 + 1], v2 = buf[b4 + 2], v3 = buf[b4 + 3];
+        const uint2 wa = tw2[512 + i];
+        bf_ct(v0, v2, wa.x, wa.y, p);
+        bf_ct(v1, v3, wa.x, wa.y, p);
+        const uint4 wb = *reinterpret_cast<const uint4*>(tw2 + 1024 + 2 * i);
+        bf_ct(v0, v1, wb.x, wb.y, p);
+        bf_ct(v2, v3, wb.z, wb.w, p);
+        buf[b4] = v0; buf[b4 + 1] = v1; buf[b4 + 2] = v2; buf[b4 + 3] = v3;
+    }
+}
 template <bool PAD = false>
 __device__ __forceinline__ void fwd_pass2_lasti(uint32_t* buf, const uint2* tw2, uint32_t p, int g) {
 #pragma unroll
@@ -521,7 +565,7 @@ __global__ __launch_bounds__(1024) void chain_kernel(const uint32_t* __restrict_
                                                      uint32_t* __restrict__ a_out, uint32_t* __restrict__ c_out,
                                                      const uint32_t* __restrict__ qs, const int* __restrict__ lane_slot,
                                                      int L, int nlanes, int qa_base, const DevTables* __restrict__ T) {
-    __shared__ uint32_t tw[2][2][N], twq[2][2][N];  // [fwd/inv][r][k]
+    __shared__ __attribute__((aligned(16))) uint2 tw2[2][2][N];  // [fwd/inv][r][k] = (twiddle, Shoup companion)
     __shared__ uint32_t sa[2][N];
     __shared__ uint32_t sc[2][2][N];                // [buffer][r][slot]
     __shared__ uint32_t xb[2][NPAD];  // transform buffers [r], padded index lx<true>
@@ -530,8 +574,8 @@ __global__ __launch_bounds__(1024) void chain_kernel(const uint32_t* __restrict_
     for (int k = tid; k < N; k += 1024) {
 #pragma unroll
         for (int r = 0; r < 2; ++r) {
-            tw[0][r][k] = T->psi[r][k]; twq[0][r][k] = T->psiq[r][k];
-            tw[1][r][k] = T->ipsi[r][k]; twq[1][r][k] = T->ipsiq[r][k];
+            tw2[0][r][k] = make_uint2(T->psi[r][k], T->psiq[r][k]);
+            tw2[1][r][k] = make_uint2(T->ipsi[r][k], T->ipsiq[r][k]);
             sa[r][k] = 0;
             sc[0][r][k] = 0;
         }
@@ -577,15 +621,15 @@ __global__ __launch_bounds__(1024) void chain_kernel(const uint32_t* __restrict_
         // ---- A: inverse NTTs of sigma_5(a) (512 threads per modulus, radix 4) ----
         {
             const uint32_t p = mod_p(rA);
-            const uint32_t* itw = tw[1][rA];
-            const uint32_t* itwq = twq[1][rA];
+            const uint2* itw2 = tw2[1][rA];
             const int ii = gA;
             uint32_t v0 = sa[rA][sigA[0]], v1 = sa[rA][sigA[1]], v2 = sa[rA][sigA[2]], v3 = sa[rA][sigA[3]];
-            bf_gs(v0, v1, itw[1024 + 2 * ii], itwq[1024 + 2 * ii], p);
-            bf_gs(v2, v3, itw[1025 + 2 * ii], itwq[1025 + 2 * ii], p);
-            const uint32_t w = itw[512 + ii], wq = itwq[512 + ii];
-            bf_gs(v0, v2, w, wq, p);
-            bf_gs(v1, v3, w, wq, p);
+            const uint4 wb = *reinterpret_cast<const uint4*>(itw2 + 1024 + 2 * ii);
+            bf_gs(v0, v1, wb.x, wb.y, p);
+            bf_gs(v2, v3, wb.z, wb.w, p);
+            const uint2 wa = itw2[512 + ii];
+            bf_gs(v0, v2, wa.x, wa.y, p);
+            bf_gs(v1, v3, wa.x, wa.y, p);
             uint32_t* b = xb[rA];
             const int b4 = lx<true>(4 * ii);
             b[b4] = v0; b[b4 + 1] = v1; b[b4 + 2] = v2; b[b4 + 3] = v3;
@@ -596,10 +640,10 @@ __global__ __launch_bounds__(1024) void chain_kernel(const uint32_t* __restrict_
             asm volatile("" : "+v"(gAx));  // pass addresses recomputed per step (VGPRs)
 #pragma unroll
             for (int t = 7; t >= 1; t -= 2) {
-                inv_pass4g<true>(xb[rA], tw[1][rA], twq[1][rA], mod_p(rA), t, gAx);
+                inv_pass4gi<true>(xb[rA], tw2[1][rA], mod_p(rA), t, gAx);
                 lds_barrier();
             }
-            inv_pass2_last<true>(xb[rA], tw[1][rA], twq[1][rA], mod_p(rA), gAx);
+            inv_pass2_lasti<true>(xb[rA], tw2[1][rA], mod_p(rA), gAx);
             lds_barrier();
         }
         // ---- B: CRT, centered lift, digit D1 (base_pir.rs:544-570) ----
@@ -637,10 +681,10 @@ __global__ __launch_bounds__(1024) void chain_kernel(const uint32_t* __restrict_
             asm volatile("" : "+v"(gCx));  // pass addresses recomputed per step (VGPRs)
 #pragma unroll  // constant pass index
             for (int P = 0; P < 3; ++P) {
-                if (tid < 512) fwd_pass8<true>(xb[rC], tw[0][rC], twq[0][rC], p, P, gCx);
+                if (tid < 512) fwd_pass8i<true>(xb[rC], tw2[0][rC], p, P, gCx);
                 lds_barrier();
             }
-            if (tid < 512) fwd_pass4<true>(xb[rC], tw[0][rC], twq[0][rC], p, gCx);
+            if (tid < 512) fwd_pass4i<true>(xb[rC], tw2[0][rC], p, gCx);
         }
         lds_barrier();
         // ---- D: update a (and c) per slot ----

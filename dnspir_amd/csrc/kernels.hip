@@ -111,6 +111,16 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Orders the LDS accesses of one wave: a wave's LDS instructions execute in
+// issue order, so only the compiler has to be kept from moving loads and
+// stores across this point. For passes whose data stays within one wave.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    asm volatile("" ::: "memory");
+}
+
 // LDS index of element j in a transform buffer. Padded buffers (N + N/8
 // words) insert 4 words after every 32: the radix-8 pass at stride 4 then
 // hits 32 distinct banks per half-wave instead of 4, and 4-aligned groups
@@ -344,6 +354,39 @@ __device__ __forceinline__ void fwd_pass2_lasti(uint32_t* buf, const uint2* tw2,
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
         const int i = g + 512 * k;
+        uint32_t x = buf[lx<PAD>(2 * i)], y = buf[lx<PAD>(2 * i + 1)];
+        const uint2 w = tw2[1024 + i];
+        bf_ct(x, y, w.x, w.y, p);
+        buf[lx<PAD>(2 * i)] = x;
+        buf[lx<PAD>(2 * i + 1)] = y;
+    }
+}
+// fwd_pass4i with the groups of wave w (g = thread index of 256) in elements
+// [512w, 512w + 512), where the radix-8 passes at strides 32 and 4 leave them
+template <bool PAD = false>
+__device__ __forceinline__ void fwd_pass4iw(uint32_t* buf, const uint2* tw2, uint32_t p, int g) {
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int i = ((g >> 6) << 7) + (g & 63) + 64 * k;
+        const int b4 = lx<PAD>(4 * i);
+        uint32_t v0 = buf[b4], v1 = buf[b4 + 1], v2 = buf[b4 + 2], v3 = buf[b4 + 3];
+        const uint2 wa = tw2[512 + i];
+        bf_ct(v0, v2, wa.x, wa.y, p);
+        bf_ct(v1, v3, wa.x, wa.y, p);
+        const uint4 wb = *reinterpret_cast<const uint4*>(tw2 + 1024 + 2 * i);
+        bf_ct(v0, v1, wb.x, wb.y, p);
+        bf_ct(v2, v3, wb.z, wb.w, p);
+        buf[b4] = v0; buf[b4 + 1] = v1; buf[b4 + 2] = v2; buf[b4 + 3] = v3;
+    }
+}
+// the last forward pass with the groups of wave w in elements [256w, 256w + 256)
+// (g = thread index of 512): after the stride-128 stage the passes of a wave
+// touch only its own block, so they need no block barrier
+template <bool PAD = false>
+__device__ __forceinline__ void fwd_pass2_lastw(uint32_t* buf, const uint2* tw2, uint32_t p, int g) {
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int i = ((g >> 6) << 7) + (g & 63) + 64 * k;
         uint32_t x = buf[lx<PAD>(2 * i)], y = buf[lx<PAD>(2 * i + 1)];
         const uint2 w = tw2[1024 + i];
         bf_ct(x, y, w.x, w.y, p);
@@ -634,15 +677,19 @@ __global__ __launch_bounds__(1024) void chain_kernel(const uint32_t* __restrict_
             const int b4 = lx<true>(4 * ii);
             b[b4] = v0; b[b4 + 1] = v1; b[b4 + 2] = v2; b[b4 + 3] = v3;
         }
-        lds_barrier();
         {
+            // the gather and the stride-4..128 passes of a wave stay in its 256
+            // elements; the stride-256/512 and stride-1024 passes cross waves
             int gAx = gA;
             asm volatile("" : "+v"(gAx));  // pass addresses recomputed per step (VGPRs)
 #pragma unroll
-            for (int t = 7; t >= 1; t -= 2) {
+            for (int t = 7; t >= 3; t -= 2) {
+                wave_lds_sync();
                 inv_pass4gi<true>(xb[rA], tw2[1][rA], mod_p(rA), t, gAx);
-                lds_barrier();
             }
+            lds_barrier();
+            inv_pass4gi<true>(xb[rA], tw2[1][rA], mod_p(rA), 1, gAx);
+            lds_barrier();
             inv_pass2_lasti<true>(xb[rA], tw2[1][rA], mod_p(rA), gAx);
             lds_barrier();
         }
@@ -679,12 +726,16 @@ __global__ __launch_bounds__(1024) void chain_kernel(const uint32_t* __restrict_
             const uint32_t p = mod_p(rC);
             int gCx = gC;
             asm volatile("" : "+v"(gCx));  // pass addresses recomputed per step (VGPRs)
-#pragma unroll  // constant pass index
-            for (int P = 0; P < 3; ++P) {
-                if (tid < 512) fwd_pass8i<true>(xb[rC], tw2[0][rC], p, P, gCx);
-                lds_barrier();
+            // after the stride-256 pass a wave's passes stay in its 512 elements
+            if (tid < 512) fwd_pass8i<true>(xb[rC], tw2[0][rC], p, 0, gCx);
+            lds_barrier();
+            if (tid < 512) {
+                fwd_pass8i<true>(xb[rC], tw2[0][rC], p, 1, gCx);
+                wave_lds_sync();
+                fwd_pass8i<true>(xb[rC], tw2[0][rC], p, 2, gCx);
+                wave_lds_sync();
+                fwd_pass4iw<true>(xb[rC], tw2[0][rC], p, gCx);
             }
-            if (tid < 512) fwd_pass4i<true>(xb[rC], tw2[0][rC], p, gCx);
         }
         lds_barrier();
         // ---- D: update a (and c) per slot ----
@@ -1646,6 +1697,9 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
 #pragma unroll 1
     for (int i = 0; i < L; ++i) {
         // ---- A: inverse NTT of sigma_5(a) mod p_r (threads 0..511, radix 4) ----
+        // The gather and the stride-4..128 passes of wave w stay in elements
+        // [256w, 256w + 256), so only the last two passes wait at block barriers
+        // (the forward transform below likewise from its stride-32 pass on).
         if (tid < 512) {
             const int ii = gA;
             uint32_t v0 = sa[sigA[0]], v1 = sa[sigA[1]], v2 = sa[sigA[2]], v3 = sa[sigA[3]];
@@ -1657,13 +1711,16 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
             bf_gs(v1, v3, wa.x, wa.y, p);
             const int b4 = lx<true>(4 * ii);
             xa[b4] = v0; xa[b4 + 1] = v1; xa[b4 + 2] = v2; xa[b4 + 3] = v3;
+            wave_lds_sync();
+            inv_pass4gi<true>(xa, tw2[1], p, 7, gA);
+            wave_lds_sync();
+            inv_pass4gi<true>(xa, tw2[1], p, 5, gA);
+            wave_lds_sync();
+            inv_pass4gi<true>(xa, tw2[1], p, 3, gA);
         }
         lds_barrier();
-#pragma unroll
-        for (int t = 7; t >= 1; t -= 2) {
-            if (tid < 512) inv_pass4gi<true>(xa, tw2[1], p, t, gA);
-            lds_barrier();
-        }
+        if (tid < 512) inv_pass4gi<true>(xa, tw2[1], p, 1, gA);
+        lds_barrier();
         if (tid < 512) inv_pass2_lasti<true>(xa, tw2[1], p, gA);
         lds_barrier();
         // ---- exchange with the partner (other modulus) ----
@@ -1683,10 +1740,14 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
             c_update(sc[cur], sc[cur ^ 1], xd[0], sga, te, cb0, cb1, cqb, sig_upd, p, K, tid);
             cur ^= 1;
         }
+        // both words in one round trip; only a word not yet tagged is polled again
+        uint32_t u2[2];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) u2[k] = __hip_atomic_load(theirs + tid + 1024 * k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
             const int j = tid + 1024 * k;
-            uint32_t u = __hip_atomic_load(theirs + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            uint32_t u = u2[k];
             if (spin_limit <= 0) {  // no waiting at all (exercises the timeout path)
                 atomicOr(err, 1u);
                 u = tag;
@@ -1730,12 +1791,19 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
         }
         lds_barrier();
         // ---- C: forward NTT of D1 mod p_r (threads 0..511, radix 4) ----
-#pragma unroll
-        for (int t = 0; t <= 8; t += 2) {
-            if (tid < 512) fwd_pass4gi<true>(xd[0], tw2[0], p, t, gA);
-            lds_barrier();
+        if (tid < 512) fwd_pass4gi<true>(xd[0], tw2[0], p, 0, gA);
+        lds_barrier();
+        if (tid < 512) fwd_pass4gi<true>(xd[0], tw2[0], p, 2, gA);
+        lds_barrier();
+        if (tid < 512) {
+            fwd_pass4gi<true>(xd[0], tw2[0], p, 4, gA);
+            wave_lds_sync();
+            fwd_pass4gi<true>(xd[0], tw2[0], p, 6, gA);
+            wave_lds_sync();
+            fwd_pass4gi<true>(xd[0], tw2[0], p, 8, gA);
+            wave_lds_sync();
+            fwd_pass2_lastw<true>(xd[0], tw2[0], p, gA);
         }
-        if (tid < 512) fwd_pass2_lasti<true>(xd[0], tw2[0], p, gA);
         lds_barrier();
         // ---- D: update a mod p_r (c follows in the next step's exchange) ----
 #pragma unroll

@@ -1334,7 +1334,7 @@ int k1b_get_choice(int ct, int nst) {
 // sum of the chains' a (and c) over the streams -> inverse NTT -> CRT ->
 // coefficients mod q and their 17-bit (a) / 12-bit (b) mod switch.
 // One block per lane index. a_out/c_out [st][li][r][slot].
-__global__ __launch_bounds__(256) void chain_out_kernel(const uint32_t* __restrict__ a_out, const uint32_t* __restrict__ c_out,
+__global__ __launch_bounds__(1024) void chain_out_kernel(const uint32_t* __restrict__ a_out, const uint32_t* __restrict__ c_out,
                                                         int nst, int nlanes, uint64_t* __restrict__ a_coef,
                                                         uint64_t* __restrict__ b_coef, uint32_t* __restrict__ a17,
                                                         uint32_t* __restrict__ b12, const DevTables* __restrict__ T) {
@@ -2255,7 +2255,8 @@ void launch_diff(const uint32_t* a, const uint32_t* b, size_t n, unsigned* flag,
 
 void launch_chain_out(const uint32_t* a_out, const uint32_t* c_out, int nst, int nlanes, uint64_t* a_coef, uint64_t* b_coef,
                       uint32_t* a17, uint32_t* b12, const DevTables* T, hipStream_t s) {
-    hipLaunchKernelGGL(chain_out_kernel, dim3(nlanes), dim3(256), 0, s, a_out, c_out, nst, nlanes, a_coef, b_coef, a17, b12, T);
+    // one block per lane: four inverse transforms of 256 threads each
+    hipLaunchKernelGGL(chain_out_kernel, dim3(nlanes), dim3(1024), 0, s, a_out, c_out, nst, nlanes, a_coef, b_coef, a17, b12, T);
 }
 
 void launch_qprep(const uint64_t* qcoef, uint32_t* qs, uint32_t* qp, int nq, const DevTables* T, hipStream_t s) {

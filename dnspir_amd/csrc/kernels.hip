@@ -469,12 +469,12 @@ __global__ __launch_bounds__(256) void k5_col_ntt_kernel(const uint16_t* __restr
 // tile [l][TS] (TS interleaved columns), in registers per group of 2^R
 // elements; lazy values in [0, 2p), Shoup products by psi^-(2N/2^(lh+1) j),
 // read from the LDS table dtw[2^lh + j] (companions in dtwq).
-template <int TS, int R>
+template <int TS, int R, int NT>
 __device__ __forceinline__ void k5_dif_pass(uint32_t* tile, int logl, int top, const uint32_t* dtw, const uint32_t* dtwq, int tid) {
     constexpr int G = 1 << R;
     const int sh = top - R + 1;  // group stride 2^sh in rows
     const int ngroups = (1 << logl) / G * TS;
-    for (int g = tid; g < ngroups; g += 256) {
+    for (int g = tid; g < ngroups; g += NT) {
         const int d = g % TS, gi = g / TS;
         const int jlow = gi & ((1 << sh) - 1), blk = gi >> sh;
         const int base = (blk << (top + 1)) + jlow;
@@ -496,40 +496,43 @@ __device__ __forceinline__ void k5_dif_pass(uint32_t* tile, int logl, int top, c
     }
 }
 
+// K5R_NT threads per block: the 64 KiB tile allows two blocks per CU, so a
+// block brings 16 waves to hide the LDS and strided-load latency
+constexpr int K5R_NT = 1024;
 template <int TS>
-__global__ __launch_bounds__(256) void k5_rows_kernel(uint32_t* __restrict__ M, int l, int logl, int kq, int conj,
-                                                      const DevTables* __restrict__ T) {
+__global__ __launch_bounds__(K5R_NT) void k5_rows_kernel(uint32_t* __restrict__ M, int l, int logl, int kq, int conj,
+                                                         const DevTables* __restrict__ T) {
     extern __shared__ uint32_t tile[];  // [l][TS], l * TS <= 16384
     __shared__ uint32_t dtw[16384 / TS], dtwq[16384 / TS];  // DIF twiddles, l entries
     const int s0 = blockIdx.x * TS, li = blockIdx.y;
     uint32_t* base = M + (size_t)li * l * N;
     const int tid = threadIdx.x;
     const int lstep = LOGN + 1 - logl;  // 2N / l = 2^lstep
-    for (int t = tid + 1; t < l; t += 256) {  // stage lh, position j: t = 2^lh + j
+    for (int t = tid + 1; t < l; t += K5R_NT) {  // stage lh, position j: t = 2^lh + j
         const int lh = 31 - __clz(t), j = t - (1 << lh);
         const int e = (M2N - (((M2N >> (lh + 1)) * j) & (M2N - 1))) & (M2N - 1);
         dtw[t] = T->psipow[0][e];
         dtwq[t] = T->psipowq[0][e];
     }
-    // each thread keeps one slot of the tile (256 is a multiple of TS)
+    // each thread keeps one slot of the tile (K5R_NT is a multiple of TS)
     const int d = tid % TS, s = s0 + d;
     const int z = (T->slot_exp[s] * kq) & (M2N - 1);
     const int coset = z & ((1 << lstep) - 1);
-    for (int j = tid / TS; j < l; j += 256 / TS) {
+    for (int j = tid / TS; j < l; j += K5R_NT / TS) {
         const int e = (M2N - ((coset * j) & (M2N - 1))) & (M2N - 1);
         tile[j * TS + d] = mul_shoup(base[(size_t)j * N + s], T->psipow[0][e], T->psipowq[0][e], P1);
     }
     __syncthreads();
     int top = logl - 1;
     while (top >= 2) {
-        k5_dif_pass<TS, 3>(tile, logl, top, dtw, dtwq, tid);
+        k5_dif_pass<TS, 3, K5R_NT>(tile, logl, top, dtw, dtwq, tid);
         __syncthreads();
         top -= 3;
     }
-    if (top == 1) k5_dif_pass<TS, 2>(tile, logl, top, dtw, dtwq, tid);
-    else if (top == 0) k5_dif_pass<TS, 1>(tile, logl, top, dtw, dtwq, tid);
+    if (top == 1) k5_dif_pass<TS, 2, K5R_NT>(tile, logl, top, dtw, dtwq, tid);
+    else if (top == 0) k5_dif_pass<TS, 1, K5R_NT>(tile, logl, top, dtw, dtwq, tid);
     if (top >= 0) __syncthreads();
-    for (int i = tid / TS; i < l; i += 256 / TS) {
+    for (int i = tid / TS; i < l; i += K5R_NT / TS) {
         int y;
         if (conj && i >= N / 2) y = (z * T->pow5[i - N / 2]) & (M2N - 1);
         else y = ((M2N - z) * T->pow5[i]) & (M2N - 1);
@@ -1590,11 +1593,11 @@ void launch_k5(const uint16_t* entries, const long long* ent_off, long long col_
     hipLaunchKernelGGL(k5_col_ntt_kernel, dim3(l, nlanes), dim3(256), 0, s, entries, ent_off, col_stride, M, l, ncols_nonzero, T);
     // tiles of 64 KiB: TS slots x l rows, TS * 4 bytes contiguous per row
     if (l <= 512)
-        hipLaunchKernelGGL(k5_rows_kernel<32>, dim3(N / 32, nlanes), dim3(256), (size_t)l * 32 * 4, s, M, l, logl, kq, conj, T);
+        hipLaunchKernelGGL(k5_rows_kernel<32>, dim3(N / 32, nlanes), dim3(K5R_NT), (size_t)l * 32 * 4, s, M, l, logl, kq, conj, T);
     else if (l == 1024)
-        hipLaunchKernelGGL(k5_rows_kernel<16>, dim3(N / 16, nlanes), dim3(256), (size_t)l * 16 * 4, s, M, l, logl, kq, conj, T);
+        hipLaunchKernelGGL(k5_rows_kernel<16>, dim3(N / 16, nlanes), dim3(K5R_NT), (size_t)l * 16 * 4, s, M, l, logl, kq, conj, T);
     else
-        hipLaunchKernelGGL(k5_rows_kernel<8>, dim3(N / 8, nlanes), dim3(256), (size_t)l * 8 * 4, s, M, l, logl, kq, conj, T);
+        hipLaunchKernelGGL(k5_rows_kernel<8>, dim3(N / 8, nlanes), dim3(K5R_NT), (size_t)l * 8 * 4, s, M, l, logl, kq, conj, T);
     hipLaunchKernelGGL(k5_cols_final_kernel, dim3(l, nlanes), dim3(256), 0, s, M, te_tmp, l, L, conj, nlanes, scale, T);
 }
 

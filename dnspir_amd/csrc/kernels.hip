@@ -1944,22 +1944,24 @@ __global__ __launch_bounds__(256) void k1d_kernel(const uint32_t* const* __restr
 // sum from (0, 1), the segments' starting values come from a prefix over the
 // 16 segments, then each thread runs its segment again from the true start
 // and writes it. Block: 64 positions j x 16 segments.
-__global__ __launch_bounds__(1024) void k1w_scan_kernel(const uint32_t* __restrict__ qp, uint32_t* __restrict__ C,
+__global__ __launch_bounds__(256) void k1w_scan_kernel(const uint32_t* __restrict__ qp, uint32_t* __restrict__ C,
                                                         const DevTables* __restrict__ T) {
-    constexpr int SEG = 16, LEN = HALFN / SEG;
+    // 16 positions x 16 segments per block: 4 x HALFN / 16 blocks per query
+    // spread the sequential steps over every CU
+    constexpr int SEG = 16, LEN = HALFN / SEG, JB = 16;
     __shared__ uint32_t g0s[HALFN], gas[HALFN];
-    __shared__ uint32_t segc[SEG][64], segp[SEG][64];
+    __shared__ uint32_t segc[SEG][JB], segp[SEG][JB];
     const int h = blockIdx.y, r = blockIdx.z & 1, q = blockIdx.z >> 1;
     const uint32_t p = mod_p(r);
     const int K = mod_k(r);
-    const int tid = threadIdx.x, jj = tid & 63, seg = tid >> 6;
+    const int tid = threadIdx.x, jj = tid % JB, seg = tid / JB;
     const uint32_t* g0 = qp + ((size_t)q * 4 * 2 + r) * N + h * HALFN;  // vector 0 (gkb0) of query q
-    for (int y = tid; y < HALFN; y += 1024) {
+    for (int y = tid; y < HALFN; y += 256) {
         g0s[y] = g0[y];
         gas[y] = T->gka[0][r][T->gpos_slot[h * HALFN + y]];
     }
     __syncthreads();
-    const int j = blockIdx.x * 64 + jj;
+    const int j = blockIdx.x * JB + jj;
     const int m0 = seg * LEN;  // steps M = m0 .. m0 + LEN - 1 (M = 0 contributes nothing)
     uint32_t c = 0, P = 1;
     for (int M = (seg == 0 ? 1 : m0); M < m0 + LEN; ++M) {
@@ -1993,7 +1995,10 @@ __global__ __launch_bounds__(1024) void k1w_scan_kernel(const uint32_t* __restri
 // j = 32c + 16hh + kk (balanced digits, as the query limbs of the batched
 // kernels). corr[a] = -2^25 sum_(s',j) W[a][j] mod p1 for the biased top
 // digits (zero mod p2). One block per (16 a's, h, r, q).
-__global__ __launch_bounds__(256) void k1w_tiles_kernel(const uint32_t* __restrict__ qp, const uint32_t* __restrict__ C,
+// 1024 threads: the 99 KiB of LDS holds one block per CU, so the block brings
+// all 16 of its waves
+constexpr int K1W_TILES_NT = 1024;
+__global__ __launch_bounds__(K1W_TILES_NT) void k1w_tiles_kernel(const uint32_t* __restrict__ qp, const uint32_t* __restrict__ C,
                                                         uint32_t* __restrict__ W, int nst, const DevTables* __restrict__ T) {
     __shared__ uint32_t E[16][HALFN + 1];  // E[d][j] = C[(j - a0 - d) mod L][j]; +1 spreads the rows over banks
     __shared__ uint32_t base[4][HALFN], mult[4][HALFN];
@@ -2004,13 +2009,13 @@ __global__ __launch_bounds__(256) void k1w_tiles_kernel(const uint32_t* __restri
     const int K = mod_k(r);
     const uint32_t two26 = (uint32_t)((1ull << 26) % p);
     const uint32_t* Cb = C + ((size_t)(q * 2 + r) * 2 + h) * HALFN * HALFN;
-    for (int e = tid; e < 16 * HALFN; e += 256) {
+    for (int e = tid; e < 16 * HALFN; e += K1W_TILES_NT) {
         const int M = e >> 4, d = e & 15;
         const int j = (M + a0 + d) & (HALFN - 1);
         E[d][j] = Cb[(size_t)M * HALFN + j];
     }
     const uint32_t* qv = qp + ((size_t)q * 4 * 2 + r) * N + h * HALFN;  // vector v at qv + v * 2N
-    for (int y = tid; y < HALFN; y += 256) {
+    for (int y = tid; y < HALFN; y += K1W_TILES_NT) {
         const int sl = T->gpos_slot[h * HALFN + y];
         const uint32_t g0 = qv[y], g1 = qv[2 * N + y];
         const uint32_t g1p = sub_mod(g1, mul_mod(two26, g0, p, K), p);
@@ -2024,7 +2029,8 @@ __global__ __launch_bounds__(256) void k1w_tiles_kernel(const uint32_t* __restri
     }
     if (tid < 16) csum[tid] = 0;
     __syncthreads();
-    // 32 (g, ai, hh) combinations x 8 threads, each over (s', c) items
+    // 32 (g, ai, hh) combinations x K1W_TILES_NT / 32 threads, each over (s', c) items
+    constexpr int NPART = K1W_TILES_NT / 32;
     const int combo = tid & 31, part = tid >> 5;
     const int gl = combo >> 3, ai = (combo >> 1) & 3, hh = combo & 1;
     const int d = gl * 4 + ai;
@@ -2032,7 +2038,7 @@ __global__ __launch_bounds__(256) void k1w_tiles_kernel(const uint32_t* __restri
     const int nsp = 2 * nst;
     unsigned long long sum = 0;
     uint32_t* Wq = W + (size_t)q * (k1w_tile_bytes(nst) / 4);
-    for (int it = part; it < nsp * 32; it += 8) {
+    for (int it = part; it < nsp * 32; it += NPART) {
         const int sp = it >> 5, c = it & 31;
         uint32_t w[16];
 #pragma unroll
@@ -2182,8 +2188,8 @@ __global__ __launch_bounds__(64 * SPB) void k1w_kernel(const uint32_t* const* __
 
 void launch_k1w(const uint32_t* const* slabs, int nshards, const uint32_t* qp, int nq, uint32_t* scan, uint32_t* tiles,
                 uint32_t* outb, int nst, const DevTables* T, hipStream_t s, hipEvent_t main_start) {
-    hipLaunchKernelGGL(k1w_scan_kernel, dim3(HALFN / 64, 2, 2 * nq), dim3(1024), 0, s, qp, scan, T);
-    hipLaunchKernelGGL(k1w_tiles_kernel, dim3(HALFN / 16, 2, 2 * nq), dim3(256), 0, s, qp, scan, tiles, nst, T);
+    hipLaunchKernelGGL(k1w_scan_kernel, dim3(HALFN / 16, 2, 2 * nq), dim3(256), 0, s, qp, scan, T);
+    hipLaunchKernelGGL(k1w_tiles_kernel, dim3(HALFN / 16, 2, 2 * nq), dim3(K1W_TILES_NT), 0, s, qp, scan, tiles, nst, T);
     if (main_start) (void)hipEventRecord(main_start, s);
     const size_t lds = (size_t)nq * 2 * nst * 32 * 512;
     const size_t qstride = (size_t)nshards * 2 * N * LANES;

@@ -172,7 +172,9 @@ int dne::ctx_presize(dnspir_ctx* c, int max_nq) {
     int rc;
     if ((rc = c->ptrs.ensure(PTR_SLOTS * sizeof(void*)))) return rc;
     if ((rc = c->xch.ensure((chain_pair_xbuf_words(PAIR_MAX_CHAINS) + 64) * 4))) return rc;
+    const size_t cap = c->idx.n;
     if ((rc = c->idx.ensure((size_t)3 * max_nq * (sizeof(long long) + 2 * sizeof(int)) + 64))) return rc;
+    if (c->idx.n != cap) c->idx_uploaded = nullptr;  // the lane lists are uploaded again
     return c->keyp.ensure((size_t)2 * max_nq * sizeof(void*));
 }
 
@@ -231,14 +233,21 @@ int check_fleet(const dnspir_shard* const* fleet, size_t len, int l_p) {
 int upload_lanes(dnspir_ctx* c, const std::vector<long long>& off, const std::vector<int>& ncols,
                  const std::vector<int>& slot, const long long** d_off, const int** d_ncols, const int** d_slot) {
     const size_t n = off.size();
+    const size_t cap = c->idx.n;
     int rc = c->idx.ensure(n * (sizeof(long long) + 2 * sizeof(int)) + 64);
     if (rc) return rc;
-    c->h_idx.assign(2 * n, 0);
-    memcpy(c->h_idx.data(), off.data(), n * sizeof(long long));
-    int* ip = reinterpret_cast<int*>(c->h_idx.data() + n);
+    if (c->idx.n != cap) c->idx_uploaded = nullptr;  // reallocated (possibly at the same address)
+    std::vector<long long> h(2 * n, 0);
+    memcpy(h.data(), off.data(), n * sizeof(long long));
+    int* ip = reinterpret_cast<int*>(h.data() + n);
     memcpy(ip, ncols.data(), n * sizeof(int));
     memcpy(ip + n, slot.data(), n * sizeof(int));
-    HIP_OK(hipMemcpyAsync(c->idx.p, c->h_idx.data(), 2 * n * sizeof(long long), hipMemcpyHostToDevice, c->stream));
+    // the same lists already in this buffer (repeated calls on one fleet): no copy
+    if (c->idx.p != c->idx_uploaded || h != c->h_idx) {
+        c->h_idx.assign(h.begin(), h.end());
+        HIP_OK(hipMemcpyAsync(c->idx.p, c->h_idx.data(), 2 * n * sizeof(long long), hipMemcpyHostToDevice, c->stream));
+        c->idx_uploaded = c->idx.p;
+    }
     *d_off = c->idx.as<long long>();
     *d_ncols = reinterpret_cast<const int*>(c->idx.as<long long>() + n);
     *d_slot = *d_ncols + n;

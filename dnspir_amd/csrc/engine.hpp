@@ -141,6 +141,7 @@ struct dnspir_ctx {
     std::vector<const dnspir_shard*> cached[2];
     std::vector<uint64_t> stage;
     std::vector<long long> h_idx;  // host staging of the lane lists (outlives the async copy)
+    const void* idx_uploaded = nullptr;  // idx buffer that holds h_idx
     std::vector<const void*> h_keyp;
     bool k1_timed = true;  // ev[3]/ev[4] bracket the first inner-product launch
     // batched pipeline captured as two HIP graphs (primary, secondary phase),

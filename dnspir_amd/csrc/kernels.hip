@@ -1699,6 +1699,13 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
     uint32_t sga[2] = {0, 0};  // sigma_5(a) at slots tid, tid + 1024, same steps
 #pragma unroll 1
     for (int i = 0; i < L; ++i) {
+        // this step's TE row, in flight through A
+        uint32_t tn[2];
+        {
+            const uint32_t* te_row = te_base + (size_t)i * 2 * N + (size_t)r * N;
+#pragma unroll
+            for (int k = 0; k < 2; ++k) tn[k] = te_row[tid + 1024 * k];
+        }
         // ---- A: inverse NTT of sigma_5(a) mod p_r (threads 0..511, radix 4) ----
         // The gather and the stride-4..128 passes of wave w stay in elements
         // [256w, 256w + 256), so only the last two passes wait at block barriers
@@ -1743,6 +1750,15 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
             c_update(sc[cur], sc[cur ^ 1], xd[0], sga, te, cb0, cb1, cqb, sig_upd, p, K, tid);
             cur ^= 1;
         }
+        // this step's sigma_5(a) (sa is rewritten in D) and TE row, and the two
+        // products of the a-update that do not need the digits
+        uint64_t pa[2];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            te[k] = tn[k];
+            sga[k] = sa[sig_upd[k]];
+            pa[k] = (uint64_t)sga[k] * cg0[k] + (uint64_t)te[k] * cqa[k];
+        }
         // both words in one round trip; only a word not yet tagged is polled again
         uint32_t u2[2];
 #pragma unroll
@@ -1768,14 +1784,6 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
             }
             oth[k] = u & 0x07FFFFFFu;
         }
-        // this step's TE row (used in D): in flight through B and C, whose
-        // barriers order LDS only
-        const uint32_t* te_row = te_base + (size_t)i * 2 * N + (size_t)r * N;
-#pragma unroll
-        for (int k = 0; k < 2; ++k) te[k] = te_row[tid + 1024 * k];
-        // sigma_5(a) of this step (sa is rewritten in D)
-#pragma unroll
-        for (int k = 0; k < 2; ++k) sga[k] = sa[sig_upd[k]];
         // ---- B: CRT, centered lift, digits (base_pir.rs:544-570): D1 mod p_r ----
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
@@ -1813,7 +1821,7 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
         for (int k = 0; k < 2; ++k) {
             const int s = tid + 1024 * k;
             const uint32_t d1 = xd[0][lx<true>(s)];  // NTT(D1), [0, 4p)
-            sa[s] = red64((uint64_t)sga[k] * cg0[k] + (uint64_t)d1 * cg1[k] + (uint64_t)te[k] * cqa[k], p, K);
+            sa[s] = red64(pa[k] + (uint64_t)d1 * cg1[k], p, K);
         }
         lds_barrier();
     }

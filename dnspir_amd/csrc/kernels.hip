@@ -2081,14 +2081,14 @@ __global__ __launch_bounds__(K1W_TILES_NT) void k1w_tiles_kernel(const uint32_t*
 
 // The two-stream inner product: the k1d arithmetic (v_dot4 over 4 digit
 // pairs, 7 weight accumulators) on the TE and IPGK1 K-steps only. A block is
-// one a-group of 16 shards (one wave each; 8 for two queries): the weights of that a-group for
-// every K-step (nsp x 32 tiles of 512 B) are staged in LDS once and read
-// like the query digits of k1d, so the loop has no barrier. Grid x = a-group:
-// the blocks in flight cover all a-groups of a few shard groups, so each XCD
-// holds the 64 KiB weight tiles of its 32 a-groups in L2 while the shard
-// groups pass (measured: 86% of HBM peak, against 80-84% with the shard
-// group fastest, which keeps pages of all 256 shards open at once; 16
-// shards per block fetch the weight tiles half as often as 8).
+// one a-group (and r, half) that streams groups of 16 shards, one wave per
+// shard (8 for two queries): the weights of that a-group for every K-step
+// (nsp x 32 tiles of 512 B) are staged in LDS once and read like the query
+// digits of k1d, so the loop has no barrier, and the block walks every shard
+// group against them (the weight tiles are fetched once per launch). Grid x =
+// a-group: the blocks in flight cover all a-groups of one (r, half) as the
+// shard groups pass (measured: 86% of HBM peak, against 80-84% with the
+// shard group fastest, which keeps pages of all 256 shards open at once).
 template <int R, int PD, int U, int NQ, int SPB>
 __device__ __forceinline__ void k1w_body(const uint32_t* const* __restrict__ slabs, int nshards, const uint32_t* __restrict__ W,
                                          uint32_t* __restrict__ outb, int nst, size_t q_stride, const u32x4* wl, int g, int sg,
@@ -2190,10 +2190,17 @@ __global__ __launch_bounds__(64 * SPB) void k1w_kernel(const uint32_t* const* __
         for (int i = threadIdx.x; i < nk * 32; i += 64 * SPB) wl[q * nk * 32 + i] = src[i];
     }
     __syncthreads();
-    if (r == 0) k1w_body<0, PD, U, NQ, SPB>(slabs, nshards, W, outb, nst, q_stride, wl, g, sg, half);
-    else k1w_body<1, PD, U, NQ, SPB>(slabs, nshards, W, outb, nst, q_stride, wl, g, sg, half);
+    // shard groups sg, sg + gridDim.y, ...: the weights stay in LDS across them
+    const int nsg = (nshards + SPB - 1) / SPB;
+    for (int sgi = sg; sgi < nsg; sgi += gridDim.y) {
+        if (r == 0) k1w_body<0, PD, U, NQ, SPB>(slabs, nshards, W, outb, nst, q_stride, wl, g, sgi, half);
+        else k1w_body<1, PD, U, NQ, SPB>(slabs, nshards, W, outb, nst, q_stride, wl, g, sgi, half);
+    }
 }
 
+// blocks per (a-group, r, half): one block fills the a-group's weights into
+// LDS once and streams every shard group against them
+constexpr int K1W_SG_BLOCKS = 1;
 void launch_k1w(const uint32_t* const* slabs, int nshards, const uint32_t* qp, int nq, uint32_t* scan, uint32_t* tiles,
                 uint32_t* outb, int nst, const DevTables* T, hipStream_t s, hipEvent_t main_start) {
     hipLaunchKernelGGL(k1w_scan_kernel, dim3(HALFN / 16, 2, 2 * nq), dim3(256), 0, s, qp, scan, T);
@@ -2201,11 +2208,12 @@ void launch_k1w(const uint32_t* const* slabs, int nshards, const uint32_t* qp, i
     if (main_start) (void)hipEventRecord(main_start, s);
     const size_t lds = (size_t)nq * 2 * nst * 32 * 512;
     const size_t qstride = (size_t)nshards * 2 * N * LANES;
+    const int gy1 = std::min(K1W_SG_BLOCKS, (nshards + 15) / 16), gy2 = std::min(K1W_SG_BLOCKS, (nshards + 7) / 8);
     if (nq == 1)
-        hipLaunchKernelGGL((k1w_kernel<4, 32, 1, 16>), dim3(HALFN / 4, (nshards + 15) / 16, 4), dim3(1024), lds, s, slabs,
+        hipLaunchKernelGGL((k1w_kernel<4, 32, 1, 16>), dim3(HALFN / 4, gy1, 4), dim3(1024), lds, s, slabs,
                            nshards, tiles, outb, nst, qstride);
     else  // two queries: both weight sets in LDS (128 KiB at full group order), 8 shards per block
-        hipLaunchKernelGGL((k1w_kernel<4, 32, 2, 8>), dim3(HALFN / 4, (nshards + 7) / 8, 4), dim3(512), lds, s, slabs,
+        hipLaunchKernelGGL((k1w_kernel<4, 32, 2, 8>), dim3(HALFN / 4, gy2, 4), dim3(512), lds, s, slabs,
                            nshards, tiles, outb, nst, qstride);
 }
 

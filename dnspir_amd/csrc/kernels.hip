@@ -588,17 +588,19 @@ __global__ __launch_bounds__(256) void k5_cols_final_kernel(const uint32_t* __re
 // ---------------------------------------------------------------------------
 // K4/K6: the sequential a-chain (preprocess_loop, base_pir.rs:819-862, with
 // partial_gal :496-594). One block of 1024 threads per (lane, stream). Step i:
-//   A  gather sigma_5(a) and inverse NTT in p1 and p2     (threads 0..511)
-//   B  CRT, centered lift, balanced base-2^26 digits D0, D1 (all threads)
-//   C  forward NTTs of D0, D1 in p1 and p2                  (256 threads each)
-//   D  a <- D0 gka0 + D1 gka1 + qa TE_i (per slot)
+//   A  gather sigma_5(a) and inverse NTT in p1 and p2     (512 threads each)
+//   B  CRT, centered lift, top balanced base-2^26 digit D1  (all threads)
+//   C  forward NTTs of D1 in p1 and p2                      (256 threads each)
+//   D  a <- D0 gka0 + D1 gka1 + qa TE_i (per slot), with NTT(D0) =
+//      sigma_5(a) - 2^26 NTT(D1): a <- sigma_5(a) gka0 + D1 (gka1 - 2^26 gka0) + qa TE_i
 // SEC additionally runs the secondary b-recurrence in gathered form,
 // c <- sigma_5(c) + D0 gkb0 + D1 gkb1 + TE_i qb (main_pir_loop,
 // base_pir.rs:1085-1247), so the digits never leave the chip; PRE stores the
 // digits as the shard's IPGK rows.
-// NTT passes are radix-8 (radix-4 for the last two stages) over registers with
-// LDS exchanges; values stay in [0, 2p) between butterflies (Harvey's lazy
-// reduction with Shoup multiplications); twiddles live in LDS.
+// NTT passes are radix-4 (inverse) and radix-8 (forward) over registers with
+// LDS exchanges; values stay in [0, 2p) / [0, 4p) between butterflies (Harvey's
+// lazy reduction with Shoup multiplications); twiddles live in LDS. Passes
+// whose data stays within one wave order LDS per wave, not per block.
 // ---------------------------------------------------------------------------
 constexpr uint32_t NINV_Q2INV_P1 = 67055624u;  // N^-1 p2^-1 mod p1
 constexpr uint32_t NINV_Q1INV_P2 = 8189u;      // N^-1 p1^-1 mod p2
@@ -1632,10 +1634,10 @@ __device__ __forceinline__ void c_update(const uint32_t* __restrict__ c_in, uint
 // The secondary chain on two CUs per chain (latency form for a few queries):
 // block (chain, r) keeps a and c mod p_r. Each step it inverse-transforms
 // sigma_5(a) mod p_r, swaps the 2048 coefficients with its partner block (the
-// other modulus) through xbuf, computes the CRT digits of every coefficient,
-// forward-transforms the two digit vectors mod p_r and updates a and c mod
-// p_r: half the transforms of chain_kernel<true> per block, plus one
-// exchange. Partners are blocks b and b ^ 8 (one XCD under round-robin
+// other modulus) through xbuf, computes the top gadget digit D1 of every
+// coefficient, forward-transforms it mod p_r (NTT(D0) = sigma_5(a) - 2^26
+// NTT(D1) needs no transform) and updates a and c mod p_r: half the
+// transforms of chain_kernel<true> per block, plus one exchange. Partners are blocks b and b ^ 8 (one XCD under round-robin
 // dispatch). Each exchanged word carries a step tag in bits 27-31 (values
 // are < 2p < 2^27), so a block waits for exactly its partner's words of this
 // step and needs no fence; xbuf starts zeroed (tag 0 is never used). Every

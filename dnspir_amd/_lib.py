@@ -27,6 +27,8 @@ ERRORS = {
     -10: "DNSPIR_E_NOINIT",
     -11: "DNSPIR_E_NOKEY",
     -12: "DNSPIR_E_CLOSED",
+    -13: "DNSPIR_E_IO",
+    -14: "DNSPIR_E_FORMAT",
 }
 E_NOKEY = -11
 QUERY_KEYLESS = 1
@@ -59,7 +61,12 @@ class ServerStats(ctypes.Structure):
     _fields_ = [("completed", ctypes.c_uint64), ("failed", ctypes.c_uint64), ("batches", ctypes.c_uint64),
                 ("key_hits", ctypes.c_uint64), ("key_inserts", ctypes.c_uint64), ("key_evictions", ctypes.c_uint64),
                 ("key_misses", ctypes.c_uint64), ("admission_waits", ctypes.c_uint64),
-                ("scratch_in_use_peak", ctypes.c_size_t), ("inflight_peak", ctypes.c_int)]
+                ("scratch_in_use_peak", ctypes.c_size_t), ("inflight_peak", ctypes.c_int),
+                ("pair_refused", ctypes.c_uint64), ("pair_retries", ctypes.c_uint64)]
+
+
+class CtxStats(ctypes.Structure):
+    _fields_ = [("pair_runs", ctypes.c_uint64), ("pair_refused", ctypes.c_uint64), ("pair_retries", ctypes.c_uint64)]
 
 
 class TuneEntry(ctypes.Structure):
@@ -102,12 +109,13 @@ SIGNATURES = {
     "dnspir_ctx_set_graphs": (ci, [vp, ci]),
     "dnspir_ctx_set_exchange_polls": (ci, [vp, ci]),
     "dnspir_ctx_set_k1_mode": (ci, [vp, ci]),
+    "dnspir_ctx_get_stats": (ci, [vp, ctypes.POINTER(CtxStats)]),
     "dnspir_double_pir": (ci, [vp, vp, sz, vp, sz, u64p, u64p, u64p, u64p, ctypes.POINTER(Timing)]),
     "dnspir_process_queries": (ci, [vp, vp, sz, vp, sz, u8p, sz, u8p, sz, u8p, u8p, ctypes.POINTER(Timing)]),
     "dnspir_process_queries_device": (ci, [vp, vp, sz, vp, sz, vp, sz, vp, sz, vp, vp, ctypes.POINTER(Timing)]),
     "dnspir_process_batch": (ci, [vp, vp, sz, u8p, sz, ci, u8p, ctypes.POINTER(Timing)]),
     "dnspir_process_batch_device": (ci, [vp, vp, sz, vp, sz, ci, vp, ctypes.POINTER(Timing)]),
-    "dnspir_database_shape": (None, [sz, ip, ip]),
+    "dnspir_database_shape": (ci, [sz, ip, ip]),
     "dnspir_num_databases": (sz, [sz]),
     "dnspir_choose_index_group_secondary": (ci, [sz]),
     "dnspir_query_bytes": (sz, [ci, ci]),

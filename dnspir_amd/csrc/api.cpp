@@ -48,6 +48,9 @@ struct Global {
     hipStream_t pre_stream = nullptr;
     DevBuf pre_entries, pre_M, pre_te, pre_ip, pre_aout, pre_idx;
     dnspir_ctx* default_ctx = nullptr;
+    // blocks of two-CU chain launches in flight (all contexts) and the CU count
+    std::mutex pair_mu;
+    int pair_used = 0, pair_cap = 0;
 };
 Global g;
 
@@ -205,16 +208,20 @@ int upload_fleet(dnspir_ctx* c, int slot, const dnspir_shard* const* fleet, size
     const uint32_t** base = c->ptrs.as<const uint32_t*>() + slot * 512;
     *slabs_dev = base;
     *a17_dev = base + 256;
-    std::vector<const dnspir_shard*> cur(fleet, fleet + len);
-    if (cur != c->cached[slot]) {
-        std::vector<const uint32_t*> h(512, nullptr);
-        for (size_t i = 0; i < len; ++i) {
-            h[i] = fleet[i]->slab;
-            h[256 + i] = fleet[i]->a17;
-        }
+    // the device list is compared by the pointers it holds, not by shard
+    // handles: a handle freed and allocated again may come back at the same
+    // address for a shard in other device memory
+    std::vector<const uint32_t*> h(512, nullptr);
+    for (size_t i = 0; i < len; ++i) {
+        h[i] = fleet[i]->slab;
+        h[256 + i] = fleet[i]->a17;
+    }
+    if (c->cached_base[slot] != (const void*)base || h != c->cached[slot]) {
+        c->cached[slot].clear();  // a failed copy leaves nothing trusted
         HIP_OK(hipMemcpyAsync(base, h.data(), 512 * sizeof(void*), hipMemcpyHostToDevice, c->stream));
         HIP_OK(hipStreamSynchronize(c->stream));
-        c->cached[slot] = cur;
+        c->cached[slot].swap(h);
+        c->cached_base[slot] = base;
     }
     return 0;
 }
@@ -288,8 +295,6 @@ int run_primary(dnspir_ctx* c, const uint32_t* const* slabs, int len, int l_p, c
     return 0;
 }
 
-// Secondary phase for `nlanes` lanes whose entries are described by the lane
-// lists; the reply (a, b per lane) lands in c->acoef/bcoef/a17o/b12o.
 }  // namespace
 
 // queues the copy of the two-CU chain's timeout flag to pinned host memory
@@ -300,22 +305,79 @@ static int copy_pair_flag(dnspir_ctx* c) {
     return 0;
 }
 
-// after a sync: the exchange flag of the last two-CU chain launch
-int dne::check_pair(dnspir_ctx* c) {
-    if (!c->pair_pending) return 0;
-    c->pair_pending = false;
-    if (*c->h_err) {
-        snprintf(g_last_error, sizeof(g_last_error), "secondary chain: partner exchange timed out");
-        return DNSPIR_E_HIP;
-    }
-    return 0;
-}
-
+// ---- the device's budget of two-CU chain blocks -------------------------------
+// Each block of chain_pair_kernel spins on its partner block, so a wait can
+// end only once the partner is resident. One such block takes 85 KB of LDS,
+// so at most one fits a CU (160 KB). If the pair blocks in flight on the
+// device never outnumber its CUs, every waiting block's partner finds a CU as
+// the blocks of other kernels (which never wait) retire. g.pair_used counts
+// the pair blocks reserved by every context of the process; a call that
+// would exceed the CU count runs its chains on one CU each instead.
 namespace {
 
+int pair_blocks(int nchains) { return 16 * ((nchains + 7) / 8); }  // launch_chain_pair's grid
+
+void pair_release(dnspir_ctx* c) {
+    if (!c->pair_held) return;
+    std::lock_guard<std::mutex> lk(g.pair_mu);
+    g.pair_used -= c->pair_held;
+    c->pair_held = 0;
+}
+
+// a reservation left by a call that returned early (its kernels may still run)
+void pair_settle(dnspir_ctx* c) {
+    if (!c->pair_held) return;
+    (void)hipStreamSynchronize(c->stream);
+    pair_release(c);
+}
+
+// reserves the blocks of a two-CU chain launch for nchains chains
+bool pair_acquire(dnspir_ctx* c, int nchains) {
+    pair_settle(c);
+    const int b = pair_blocks(nchains);
+    std::lock_guard<std::mutex> lk(g.pair_mu);
+    if (g.pair_used + b > g.pair_cap) {
+        ++c->stats.pair_refused;
+        return false;
+    }
+    g.pair_used += b;
+    c->pair_held = b;
+    ++c->stats.pair_runs;
+    return true;
+}
+
+// records a reply encoding that follows the secondary phase (repeated by a
+// retry of the phase)
+void note_encode(dnspir_ctx* c, int lane0, int nq, uint8_t* reply) {
+    dnspir_ctx::SecondaryCall& sc = c->last_sec;
+    if (sc.n_enc >= 2) return;
+    sc.enc_lane0[sc.n_enc] = lane0;
+    sc.enc_nq[sc.n_enc] = nq;
+    sc.enc_reply[sc.n_enc] = reply;
+    ++sc.n_enc;
+}
+
+// the arguments of a secondary phase, for check_pair's retry
+void note_secondary(dnspir_ctx* c, const uint16_t* entries, long long col_stride, const long long* d_off,
+                    const int* d_ncols, const int* d_slot, int nlanes, int l_s) {
+    dnspir_ctx::SecondaryCall& sc = c->last_sec;
+    sc.entries = entries;
+    sc.col_stride = col_stride;
+    sc.d_off = d_off;
+    sc.d_ncols = d_ncols;
+    sc.d_slot = d_slot;
+    sc.nlanes = nlanes;
+    sc.l_s = l_s;
+    sc.n_enc = 0;
+}
+
+// Secondary phase for `nlanes` lanes whose entries are described by the lane
+// lists; the reply (a, b per lane) lands in c->acoef/bcoef/a17o/b12o.
 // pair: run the chains on two CUs each (chain_pair_kernel, lower latency for a
-// few chains); the caller sets c->pair_pending and checks the exchange flag
-// after its sync (check_pair)
+// few chains; the caller holds their pair_acquire reservation, queues
+// copy_pair_flag and calls check_pair after its sync). The arguments are kept
+// in c->last_sec so that check_pair can run the phase again on one CU per
+// chain.
 int run_secondary(dnspir_ctx* c, const uint16_t* entries, long long col_stride, const long long* d_off, const int* d_ncols,
                   const int* d_slot, int nlanes, int l_s, bool pair = false) {
     const int L_s = loop_len(l_s), nst_s = nstreams(l_s);
@@ -329,6 +391,7 @@ int run_secondary(dnspir_ctx* c, const uint16_t* entries, long long col_stride, 
     if ((rc = c->bcoef.ensure((size_t)nlanes * N * 8))) return rc;
     if ((rc = c->a17o.ensure((size_t)nlanes * N * 4))) return rc;
     if ((rc = c->b12o.ensure((size_t)nlanes * N * 4))) return rc;
+    note_secondary(c, entries, col_stride, d_off, d_ncols, d_slot, nlanes, l_s);
     launch_k5(entries, d_off, col_stride, c->M.as<uint32_t>(), c->te.as<uint32_t>(), l_s, d_ncols, nlanes, g.tables, s);
     if (pair) {
         // xch: the timeout flag in word 0, the exchange words from word 64
@@ -345,6 +408,35 @@ int run_secondary(dnspir_ctx* c, const uint16_t* entries, long long col_stride, 
                      c->bcoef.as<uint64_t>(), c->a17o.as<uint32_t>(), c->b12o.as<uint32_t>(), g.tables, s);
     return 0;
 }
+
+}  // namespace
+
+// After the sync that follows a call: if that call's two-CU chain timed out
+// waiting for a partner block, its outputs are invalid; the same secondary
+// phase (and the reply encodings noted for it) runs again with one CU per
+// chain, which never waits, and the stream is synchronised again.
+int dne::check_pair(dnspir_ctx* c, bool* retried) {
+    if (retried) *retried = false;
+    const bool pending = c->pair_pending;
+    c->pair_pending = false;
+    const bool timed_out = pending && *c->h_err != 0;
+    pair_release(c);
+    if (!timed_out) return 0;
+    ++c->stats.pair_retries;
+    const dnspir_ctx::SecondaryCall sc = c->last_sec;
+    int rc = run_secondary(c, sc.entries, sc.col_stride, sc.d_off, sc.d_ncols, sc.d_slot, sc.nlanes, sc.l_s, false);
+    if (rc) return rc;
+    for (int i = 0; i < sc.n_enc; ++i)
+        launch_encode_reply(c->a17o.as<uint32_t>(), c->b12o.as<uint32_t>(), sc.enc_lane0[i], sc.enc_nq[i], sc.enc_reply[i],
+                            c->stream);
+    c->last_sec = sc;
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(c->stream));
+    if (retried) *retried = true;
+    return 0;
+}
+
+namespace {
 
 // perform_batched_double_pir on ring-element queries already in c->qcoef
 // ([2][6][N]); leaves the reply in c->acoef/bcoef/a17o/b12o with lane index
@@ -410,10 +502,11 @@ int run_double(dnspir_ctx* c, const dnspir_shard* const* fleets[2], const size_t
         ref_streamed += (uint64_t)len * ref_slab_bytes(l_p);
     }
     HIP_OK(hipEventRecord(c->ev[1], s));
+    const bool pair = pair_acquire(c, nstreams(l_s) * (int)off.size());
     if ((rc = run_secondary(c, c->se.as<uint16_t>(), (long long)LANES * N, d_off, d_ncols, d_slot, (int)off.size(), l_s,
-                            true)))
+                            pair)))
         return rc;
-    if ((rc = copy_pair_flag(c))) return rc;
+    if (pair && (rc = copy_pair_flag(c))) return rc;
     HIP_OK(hipEventRecord(c->ev[2], s));
     if (tm) {
         tm->bytes_streamed = streamed;
@@ -472,6 +565,10 @@ const char* dnspir_strerror(int code) {
         case DNSPIR_E_HIP: return "HIP runtime error";
         case DNSPIR_E_DECODE: return "reply digits out of range";
         case DNSPIR_E_NOINIT: return "dnspir_init has not been called";
+        case DNSPIR_E_NOKEY: return "keyless query, but the client's Galois keys are not cached";
+        case DNSPIR_E_CLOSED: return "the server is shutting down";
+        case DNSPIR_E_IO: return "a shard file could not be opened, written or closed";
+        case DNSPIR_E_FORMAT: return "not a shard file of this build's layout, or corrupt or truncated";
         default: return "unknown error";
     }
 }
@@ -509,6 +606,13 @@ int dnspir_init(int device) {
     g.tables = d;
     g.layout = fp;
     g.device = device;
+    {
+        hipDeviceProp_t prop;
+        HIP_OK(hipGetDeviceProperties(&prop, device));
+        std::lock_guard<std::mutex> pk(g.pair_mu);
+        g.pair_cap = prop.multiProcessorCount;
+        g.pair_used = 0;
+    }
     HIP_OK(hipStreamCreateWithFlags(&g.pre_stream, hipStreamNonBlocking));
     g.default_ctx = new dnspir_ctx;
     return ctx_init(g.default_ctx, nullptr);
@@ -825,6 +929,7 @@ int dnspir_ctx_create(void* stream, dnspir_ctx** out) {
 int dnspir_ctx_destroy(dnspir_ctx* c) {
     if (!c) return DNSPIR_E_ARG;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    pair_release(c);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->own && c->stream) (void)hipStreamDestroy(c->stream);
@@ -849,6 +954,18 @@ int dnspir_ctx_set_exchange_polls(dnspir_ctx* c, int max_polls) {
     return 0;
 }
 
+int dnspir_ctx_get_stats(dnspir_ctx* c, dnspir_ctx_stats* out) {
+    if (!out) return DNSPIR_E_ARG;
+    if (!c) {
+        int rc = ensure_init();
+        if (rc) return rc;
+        c = default_ctx();
+    }
+    std::lock_guard<std::mutex> lk(c->mu);
+    *out = c->stats;
+    return 0;
+}
+
 int dnspir_ctx_set_k1_mode(dnspir_ctx* c, int mode) {
     if (!c || (mode != 0 && mode != 2 && mode != 3)) return DNSPIR_E_ARG;
     std::lock_guard<std::mutex> lk(c->mu);
@@ -857,21 +974,24 @@ int dnspir_ctx_set_k1_mode(dnspir_ctx* c, int mode) {
 }
 
 int dnspir_choose_index_group_secondary(size_t num_shards) {
+    if (num_shards > (size_t)(N / LANES)) return DNSPIR_E_SHAPE;  // pir_wrapper.rs:168
     int log2l = 0;
     while (((size_t)1 << log2l) < num_shards) ++log2l;
     return LANES << log2l;
 }
 
-void dnspir_database_shape(size_t n, int* lp, int* ls) {
+int dnspir_database_shape(size_t n, int* lp, int* ls) {
+    if (n < 1 || n > (size_t)N * N) return DNSPIR_E_SHAPE;  // pir_wrapper.rs:221-222
     const int l_p = n <= (size_t)N * N / 16 ? N / 2 : N;  // pir_wrapper.rs:188, 223-227
     const size_t shards = (n + (size_t)LANES * l_p - 1) / ((size_t)LANES * l_p);
     if (lp) *lp = l_p;
     if (ls) *ls = dnspir_choose_index_group_secondary(shards);
+    return 0;
 }
 
 size_t dnspir_num_databases(size_t n) {
     int lp = 0;
-    dnspir_database_shape(n, &lp, nullptr);
+    if (dnspir_database_shape(n, &lp, nullptr)) return 0;
     return (n + (size_t)LANES * lp - 1) / ((size_t)LANES * lp);
 }
 
@@ -908,12 +1028,13 @@ int dnspir_double_pir(dnspir_ctx* c, const dnspir_shard* const* f0, size_t len0,
                                   hipMemcpyHostToDevice, c->stream));
     if ((rc = run_double(c, fleets, lens, present, tm))) return rc;
     HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(c->stream));
+    if ((rc = check_pair(c))) return rc;
     const size_t nl = 3 * ((present[0] ? 1 : 0) + (present[1] ? 1 : 0));  // lanes carrying replies
     std::vector<uint64_t> a(nl * N), b(nl * N);
     HIP_OK(hipMemcpyAsync(a.data(), c->acoef.p, a.size() * 8, hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipMemcpyAsync(b.data(), c->bcoef.p, b.size() * 8, hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
-    if ((rc = check_pair(c))) return rc;
     uint64_t* outs[2] = {out0, out1};
     int li = 0;
     for (int sl = 0; sl < 2; ++sl) {
@@ -935,7 +1056,6 @@ static int process_device_impl(dnspir_ctx* c, const dnspir_shard* const* f0, siz
     if (!f0 || len0 == 0 || !f0[0]) return DNSPIR_E_SHAPE;
     if (len0 < len1) return DNSPIR_E_SHAPE;
     const int l_p = f0[0]->l;
-    const int l_s = dnspir_choose_index_group_secondary(len0);
     const bool present[2] = {q0 != nullptr, q1 != nullptr};
     const dnspir_shard* const* fleets[2] = {f0, f1};
     const size_t lens[2] = {len0, len1};
@@ -943,6 +1063,8 @@ static int process_device_impl(dnspir_ctx* c, const dnspir_shard* const* f0, siz
     const size_t ql[2] = {ql0, ql1};
     for (int sl = 0; sl < 2; ++sl)
         if (present[sl] && (rc = check_fleet(fleets[sl], lens[sl], l_p))) return rc;
+    const int l_s = dnspir_choose_index_group_secondary(len0);
+    if (l_s < 0) return l_s;
     if ((rc = c->qcoef.ensure((size_t)LANES * 6 * N * 8))) return rc;
     const int conj_p = l_p == N, conj_s = l_s == N;
     for (int sl = 0; sl < 2; ++sl) {
@@ -967,7 +1089,10 @@ static int process_device_impl(dnspir_ctx* c, const dnspir_shard* const* f0, siz
     int li = 0;
     for (int sl = 0; sl < 2; ++sl) {
         if (!present[sl]) continue;
-        if (rs[sl]) launch_encode_reply(c->a17o.as<uint32_t>(), c->b12o.as<uint32_t>(), li, 1, rs[sl], c->stream);
+        if (rs[sl]) {
+            launch_encode_reply(c->a17o.as<uint32_t>(), c->b12o.as<uint32_t>(), li, 1, rs[sl], c->stream);
+            note_encode(c, li, 1, rs[sl]);
+        }
         li += 3;
     }
     HIP_OK(hipGetLastError());
@@ -1085,8 +1210,8 @@ int dne::batch_impl(dnspir_ctx* c, const dnspir_shard* const* fleet, size_t len,
     if ((rc = batch_reserve(c, l_p, len, nq, 0))) return rc;
     // few chains run on two CUs each; their exchange buffer is sized once for
     // the largest such batch (no allocation inside a graph capture)
-    const bool pair = 3 * nq * nstreams(l_s) <= PAIR_MAX_CHAINS;
-    if (pair && (rc = c->xch.ensure((chain_pair_xbuf_words(PAIR_MAX_CHAINS) + 64) * 4))) return rc;
+    const int nchains = 3 * nq * nstreams(l_s);
+    if (nchains <= PAIR_MAX_CHAINS && (rc = c->xch.ensure((chain_pair_xbuf_words(PAIR_MAX_CHAINS) + 64) * 4))) return rc;
     const uint32_t* const* slabs;
     const uint32_t* const* a17s;
     if ((rc = upload_fleet(c, 0, fleet, len, &slabs, &a17s))) return rc;
@@ -1101,6 +1226,9 @@ int dne::batch_impl(dnspir_ctx* c, const dnspir_shard* const* fleet, size_t len,
     const long long* d_off;
     const int *d_ncols, *d_slot;
     if ((rc = upload_lanes(c, loff, ncols, slot, &d_off, &d_ncols, &d_slot))) return rc;
+    // the pair form only while the device's pair budget has room (taken last,
+    // after every early return above)
+    const bool pair = nchains <= PAIR_MAX_CHAINS && pair_acquire(c, nchains);
     const bool use_keys = keys && (keys->src || keys->dst);
     if (use_keys) {
         // cached keys replace the decoded ones; fresh ones are stored
@@ -1145,6 +1273,7 @@ int dne::batch_impl(dnspir_ctx* c, const dnspir_shard* const* fleet, size_t len,
         int r2;
         if ((r2 = run_secondary(c, c->se.as<uint16_t>(), 3LL * N, d_off, d_ncols, d_slot, 3 * nq, l_s, pair))) return r2;
         launch_encode_reply(c->a17o.as<uint32_t>(), c->b12o.as<uint32_t>(), 0, nq, rdev, s);
+        note_encode(c, 0, nq, rdev);
         return 0;
     };
     c->k1_launches = (nq + 63) / 64;
@@ -1154,7 +1283,8 @@ int dne::batch_impl(dnspir_ctx* c, const dnspir_shard* const* fleet, size_t len,
         std::vector<uintptr_t> key = {(uintptr_t)qdev, qlen, (uintptr_t)nq, (uintptr_t)rdev, len, (uintptr_t)l_p,
                                       (uintptr_t)use_keys, (uintptr_t)c->keyp.p, (uintptr_t)c->ptrs.p, (uintptr_t)c->idx.p,
                                       (uintptr_t)k1b_get_choice(1, nstreams(l_p)), (uintptr_t)k1b_get_choice(2, nstreams(l_p)),
-                                      (uintptr_t)c->xch.p, (uintptr_t)c->h_err};
+                                      (uintptr_t)c->xch.p, (uintptr_t)c->h_err, (uintptr_t)use_k1w(c, l_p, len, nq),
+                                      (uintptr_t)pair, (uintptr_t)c->exchange_polls};
         for (auto* b : c->scratch_bufs) key.push_back((uintptr_t)b->p);
         if (!c->graph_exec[0] || key != c->graph_key) {
             c->drop_graphs();
@@ -1176,6 +1306,9 @@ int dne::batch_impl(dnspir_ctx* c, const dnspir_shard* const* fleet, size_t len,
             }
         }
         if (c->graph_exec[0] && c->graph_exec[1]) {
+            // what the captured secondary phase does, for check_pair's retry
+            note_secondary(c, c->se.as<uint16_t>(), 3LL * N, d_off, d_ncols, d_slot, 3 * nq, l_s);
+            note_encode(c, 0, nq, rdev);
             HIP_OK(hipEventRecord(c->ev[0], s));
             HIP_OK(hipGraphLaunch(c->graph_exec[0], s));
             HIP_OK(hipEventRecord(c->ev[1], s));

@@ -136,9 +136,25 @@ struct dnspir_ctx {
     unsigned* h_err = nullptr;  // pinned copy of the chain timeout flag
     int exchange_polls = 1 << 21;  // bound of each exchange wait of the two-CU chain
     bool pair_pending = false;  // h_err is to be checked after the next sync
+    int pair_held = 0;          // two-CU chain blocks this context has reserved on the device
+    // the last secondary phase's arguments and the reply encodings that
+    // followed it (repeated with one CU per chain when a partner wait timed out)
+    struct SecondaryCall {
+        const uint16_t* entries = nullptr;
+        long long col_stride = 0;
+        const long long* d_off = nullptr;
+        const int *d_ncols = nullptr, *d_slot = nullptr;
+        int nlanes = 0, l_s = 0;
+        int n_enc = 0, enc_lane0[2] = {0, 0}, enc_nq[2] = {0, 0};
+        uint8_t* enc_reply[2] = {nullptr, nullptr};
+    } last_sec;
+    dnspir_ctx_stats stats{};
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // start, primary end, end, k1 start, k1 end
     int k1_launches = 0;
-    std::vector<const dnspir_shard*> cached[2];
+    // the slab / a17 pointer lists last uploaded to ptrs (per fleet slot) and
+    // the device address they were uploaded to
+    std::vector<const uint32_t*> cached[2];
+    const void* cached_base[2] = {nullptr, nullptr};
     std::vector<uint64_t> stage;
     std::vector<long long> h_idx;  // host staging of the lane lists (outlives the async copy)
     const void* idx_uploaded = nullptr;  // idx buffer that holds h_idx
@@ -190,9 +206,10 @@ inline bool use_k1w(const dnspir_ctx* c, int l_p, size_t len, int nq) {
     return (nq == 1 || nq == 2) && 2 * l_p >= N &&
            (c->k1_mode == 2 || (c->k1_mode == 0 && len >= (size_t)K1W_MIN_SHARDS));
 }
-// after the stream sync that follows a call: DNSPIR_E_HIP if that call's
-// two-CU chain timed out waiting for a partner block
-int check_pair(dnspir_ctx* c);
+// after the stream sync that follows a call: releases the call's two-CU
+// chain reservation; if that chain timed out waiting for a partner block,
+// runs the secondary phase again with one CU per chain (*retried = true)
+int check_pair(dnspir_ctx* c, bool* retried = nullptr);
 // sizes the persistent pointer / lane-list buffers for batches of up to max_nq
 int ctx_presize(dnspir_ctx* c, int max_nq);
 

@@ -135,6 +135,7 @@ struct Slot {
     std::thread th;
     uint8_t* hq = nullptr;  // pinned query staging
     uint8_t* hr = nullptr;  // pinned reply staging
+    uint64_t refused_seen = 0, retries_seen = 0;  // ctx.stats already added to the server's
 };
 
 }  // namespace
@@ -334,11 +335,24 @@ struct dnspir_server {
             }
         }
         cv_work.notify_all();
-        if (!rc && (hipMemcpyAsync(sl->hr, c->rbytes.p, (size_t)nb * REPLY_BYTES, hipMemcpyDeviceToHost, c->stream) !=
-                        hipSuccess ||
-                    hipStreamSynchronize(c->stream) != hipSuccess))
-            rc = DNSPIR_E_HIP;
-        if (!rc) rc = check_pair(c);
+        auto fetch = [&] {
+            return hipMemcpyAsync(sl->hr, c->rbytes.p, (size_t)nb * REPLY_BYTES, hipMemcpyDeviceToHost, c->stream) ==
+                           hipSuccess &&
+                       hipStreamSynchronize(c->stream) == hipSuccess
+                       ? 0
+                       : DNSPIR_E_HIP;
+        };
+        if (!rc) rc = fetch();
+        bool retried = false;
+        if (!rc) rc = check_pair(c, &retried);
+        if (!rc && retried) rc = fetch();  // the replies were encoded again
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            st.pair_refused += c->stats.pair_refused - sl->refused_seen;
+            st.pair_retries += c->stats.pair_retries - sl->retries_seen;
+            sl->refused_seen = c->stats.pair_refused;
+            sl->retries_seen = c->stats.pair_retries;
+        }
         if (rc == DNSPIR_E_HIP) fprintf(stderr, "dnspir server: %s\n", dnspir_last_error());
         if (!rc)
             for (int i = 0; i < nb; ++i)
@@ -535,20 +549,44 @@ int dnspir_server_add_database(dnspir_server* s, const dnspir_shard* const* flee
     return 0;
 }
 
-// starts the slot workers on first use (after the databases are known)
+// releases what slot_init set up (no worker thread runs on it)
+static void slot_free(Slot* sl) {
+    if (sl->ctx.stream) (void)hipStreamSynchronize(sl->ctx.stream);
+    for (auto& e : sl->ctx.ev)
+        if (e) (void)hipEventDestroy(e), e = nullptr;
+    if (sl->ctx.own && sl->ctx.stream) (void)hipStreamDestroy(sl->ctx.stream);
+    sl->ctx.stream = nullptr;
+    if (sl->hq) (void)hipHostFree(sl->hq);
+    if (sl->hr) (void)hipHostFree(sl->hr);
+    sl->hq = sl->hr = nullptr;
+}
+
+static int slot_init(dnspir_server* s, Slot* sl) {
+    int rc = ctx_init(&sl->ctx, nullptr);
+    if (rc) return rc;
+    if ((rc = ctx_presize(&sl->ctx, s->cfg.max_batch))) return rc;
+    HIP_OK(hipHostMalloc((void**)&sl->hq, (size_t)s->cfg.max_batch * s->max_qlen, hipHostMallocDefault));
+    HIP_OK(hipHostMalloc((void**)&sl->hr, (size_t)s->cfg.max_batch * REPLY_BYTES, hipHostMallocDefault));
+    return 0;
+}
+
+// starts the slot workers on first use (after the databases are known); the
+// slots become visible only once every one of them is set up, so a failure
+// leaves the server without slots (the next submit tries again)
 static int start_slots(dnspir_server* s) {
     if (!s->slots.empty()) return 0;
     if (s->dbs.empty()) return DNSPIR_E_ARG;
+    std::vector<std::unique_ptr<Slot>> made;
     for (int i = 0; i < s->cfg.slots; ++i) {
         std::unique_ptr<Slot> sl(new (std::nothrow) Slot);
-        if (!sl) return DNSPIR_E_OOM;
-        int rc = ctx_init(&sl->ctx, nullptr);
-        if (rc) return rc;
-        if ((rc = ctx_presize(&sl->ctx, s->cfg.max_batch))) return rc;
-        HIP_OK(hipHostMalloc((void**)&sl->hq, (size_t)s->cfg.max_batch * s->max_qlen, hipHostMallocDefault));
-        HIP_OK(hipHostMalloc((void**)&sl->hr, (size_t)s->cfg.max_batch * REPLY_BYTES, hipHostMallocDefault));
-        s->slots.push_back(std::move(sl));
+        int rc = sl ? slot_init(s, sl.get()) : DNSPIR_E_OOM;
+        if (sl) made.push_back(std::move(sl));
+        if (rc) {
+            for (auto& m : made) slot_free(m.get());
+            return rc;
+        }
     }
+    s->slots = std::move(made);
     for (auto& sl : s->slots) {
         Slot* p = sl.get();
         p->th = std::thread([s, p] { s->worker(p); });
@@ -636,14 +674,7 @@ int dnspir_server_destroy(dnspir_server* s) {
     s->cv_done.notify_all();
     for (auto& sl : s->slots)
         if (sl->th.joinable()) sl->th.join();
-    for (auto& sl : s->slots) {
-        if (sl->ctx.stream) (void)hipStreamSynchronize(sl->ctx.stream);
-        for (auto& e : sl->ctx.ev)
-            if (e) (void)hipEventDestroy(e);
-        if (sl->ctx.own && sl->ctx.stream) (void)hipStreamDestroy(sl->ctx.stream);
-        if (sl->hq) (void)hipHostFree(sl->hq);
-        if (sl->hr) (void)hipHostFree(sl->hr);
-    }
+    for (auto& sl : s->slots) slot_free(sl.get());
     for (auto& kv : s->reqs) delete kv.second;
     if (s->arena) (void)hipFree(s->arena);
     if (s->key_mem) (void)hipFree(s->key_mem);

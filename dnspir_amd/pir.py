@@ -15,8 +15,8 @@ import os
 
 import numpy as np
 
-from ._lib import (LANES, N, QUERY_KEYLESS, Arrival, DnspirError, RequestInfo, ServerConfig, ServerStats, Timing,
-                   TuneEntry, check, lib)
+from ._lib import (LANES, N, QUERY_KEYLESS, Arrival, CtxStats, DnspirError, RequestInfo, ServerConfig, ServerStats,
+                   Timing, TuneEntry, check, lib)
 
 _ptr = lambda a, t: a.ctypes.data_as(t)  # noqa: E731
 u8p = ctypes.POINTER(ctypes.c_uint8)
@@ -160,7 +160,8 @@ class Context:
 
     def set_exchange_polls(self, max_polls):
         """Bound of each partner wait of the two-CU secondary chain (0: give
-        up at once, which makes the call fail with DNSPIR_E_HIP)."""
+        up at once; the call then runs its secondary phase again on one CU
+        per chain)."""
         check(lib().dnspir_ctx_set_exchange_polls(self.h, int(max_polls)), "ctx_set_exchange_polls")
 
     def set_k1_mode(self, mode):
@@ -168,6 +169,14 @@ class Context:
         streams TE and IPGK1 with per-query weights in place of IPGK0, 0
         (default) picks 2 for fleets of 8 or more shards. Same bytes."""
         check(lib().dnspir_ctx_set_k1_mode(self.h, int(mode)), "ctx_set_k1_mode")
+
+    def stats(self):
+        """counters of this context's secondary phases: two-CU runs, runs on
+        one CU because the device's two-CU budget was in use, retries after a
+        partner wait timed out"""
+        st = CtxStats()
+        check(lib().dnspir_ctx_get_stats(self.h, ctypes.byref(st)), "ctx_get_stats")
+        return {k: getattr(st, k) for k, _ in CtxStats._fields_}
 
     def __del__(self):
         h = getattr(self, "h", None)
@@ -282,17 +291,26 @@ def autotune_batch(fleet, nq, ctx=None):
 
 
 def get_database_shape(num_entries):
+    """pir_wrapper.rs:219-231; DnspirError(DNSPIR_E_SHAPE) where the reference
+    asserts (0 entries, more than N^2)."""
     lp, ls = ctypes.c_int(), ctypes.c_int()
-    lib().dnspir_database_shape(num_entries, ctypes.byref(lp), ctypes.byref(ls))
+    check(lib().dnspir_database_shape(num_entries, ctypes.byref(lp), ctypes.byref(ls)), "get_database_shape")
     return lp.value, ls.value
 
 
 def num_databases(num_entries):
-    return lib().dnspir_num_databases(num_entries)
+    n = lib().dnspir_num_databases(num_entries)
+    if n == 0:
+        raise DnspirError(-4, "num_databases")
+    return n
 
 
 def choose_index_group_secondary(num_shards):
-    return lib().dnspir_choose_index_group_secondary(num_shards)
+    """pir_wrapper.rs:166-175; DnspirError(DNSPIR_E_SHAPE) above N/8 shards."""
+    rc = lib().dnspir_choose_index_group_secondary(num_shards)
+    if rc < 0:
+        raise DnspirError(rc, "choose_index_group_secondary")
+    return rc
 
 
 def query_bytes(lp, ls):

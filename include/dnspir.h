@@ -126,11 +126,22 @@ void* dnspir_ctx_stream(dnspir_ctx* ctx);
  * eagerly), else 0. */
 int dnspir_ctx_set_graphs(dnspir_ctx* ctx, int on);
 /* Few-query calls run each secondary chain on two CUs that exchange data
- * every step; each wait for the partner CU ends after `max_polls` polls
- * (default 2^21, about a second) and the call then fails with DNSPIR_E_HIP.
- * 0 makes every wait give up at once (to exercise that error path). No
- * counterpart in the reference. */
+ * every step (while the blocks of all such launches in flight on the device
+ * fit its CUs, one block per CU; otherwise on one CU per chain). Each wait
+ * for the partner CU ends after `max_polls` polls (default 2^21, about a
+ * second); the call then runs its secondary phase again on one CU per chain,
+ * which never waits, and still answers exactly. 0 makes every wait give up at
+ * once (to exercise that path). No counterpart in the reference. */
 int dnspir_ctx_set_exchange_polls(dnspir_ctx* ctx, int max_polls);
+/* counters of a context's secondary phases (NULL ctx = the default context) */
+typedef struct {
+    uint64_t pair_runs;      /* phases run on two CUs per chain */
+    uint64_t pair_refused;   /* phases run on one CU per chain because the device's
+                                two-CU budget was taken by other calls in flight */
+    uint64_t pair_retries;   /* two-CU phases whose partner wait timed out and that
+                                were run again on one CU per chain */
+} dnspir_ctx_stats;
+int dnspir_ctx_get_stats(dnspir_ctx* ctx, dnspir_ctx_stats* out);
 /* Inner product of single queries on shards with loop length N/2. mode 3:
  * stream all three shard arrays (TE, IPGK0, IPGK1), as main_pir_loop does
  * (base_pir.rs:1153-1208). mode 2: stream TE and IPGK1 only and fold IPGK0
@@ -194,11 +205,14 @@ typedef struct {
 } dnspir_tune_entry;
 int dnspir_autotune_batch(dnspir_ctx* ctx, const dnspir_shard* const* fleet, size_t len, int nq, dnspir_tune_entry* entries,
                           int max_entries, int* n_entries);
-/* get_database_shape (pir_wrapper.rs:219-231): index-group orders */
-void dnspir_database_shape(size_t num_entries, int* l_primary, int* l_secondary);
-/* num_databases (pir_wrapper.rs:238-241) */
+/* get_database_shape (pir_wrapper.rs:219-231): index-group orders.
+ * DNSPIR_E_SHAPE (outputs untouched) where the reference asserts: num_entries
+ * of 0 or above N^2 (pir_wrapper.rs:221-222). */
+int dnspir_database_shape(size_t num_entries, int* l_primary, int* l_secondary);
+/* num_databases (pir_wrapper.rs:238-241); 0 where get_database_shape refuses */
 size_t dnspir_num_databases(size_t num_entries);
-/* choose_index_group_secondary (pir_wrapper.rs:166-175) */
+/* choose_index_group_secondary (pir_wrapper.rs:166-175): the group order, or
+ * DNSPIR_E_SHAPE for more than N/8 shards (pir_wrapper.rs:168) */
 int dnspir_choose_index_group_secondary(size_t num_shards);
 size_t dnspir_query_bytes(int l_primary, int l_secondary);
 size_t dnspir_reply_bytes(void);
@@ -250,6 +264,9 @@ typedef struct {
     uint64_t admission_waits; /* times waiting requests had to wait for scratch */
     size_t scratch_in_use_peak;
     int inflight_peak;        /* most batches on the device at once */
+    uint64_t pair_refused;    /* batches whose secondary chains took one CU each because
+                                 the device's two-CU budget was in use (dnspir_ctx_stats) */
+    uint64_t pair_retries;    /* batches whose two-CU chains timed out and ran again on one CU */
 } dnspir_server_stats;
 
 /* one request of an open-loop load (dnspir_server_replay) */

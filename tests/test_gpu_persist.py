@@ -29,28 +29,42 @@ def _rand_q(seed, n=1):
     return np.random.default_rng(seed).integers(0, orc.Q, (n, N), dtype=np.uint64)
 
 
-@pytest.mark.parametrize("l", [1024, 64])
+@pytest.mark.parametrize("l", [1024, 2048, 64])
 def test_save_load_round_trip(pir, tmp_path, l):
+    """The resumed shard answers as an oracle shard preprocessed from the same
+    entries: get_a, perform_pir and a reply in which it is the queried shard,
+    alone and beside the shard it was saved from."""
     e = np.random.default_rng(40 + l).integers(0, 1025, (8 * l, N)).astype(np.uint16)
     g = pir.PIRDatabase(l)
     g.set_db(e)
+    _say(f"[persist] oracle shard l={l}")
+    o = orc.Shard(11, l)
+    o.set_db(e)
     path = tmp_path / f"s{l}.shard"
     _say(f"[persist] save l={l}")
     g.save(str(path))
     assert path.stat().st_size == 64 + pir.required_bytes(l)
     assert pir.shard_file_info(str(path)) == (l, 0, pir.required_bytes(l))
+    del g  # the resumed shard must not depend on the original's device memory
     r = pir.PIRDatabase.load(str(path))
     _say(f"[persist] loaded l={l}")
-    assert np.array_equal(r.get_a(), g.get_a())
+    assert np.array_equal(r.get_a(), o.get_a())
     qb, gk = _rand_q(1)[0], _rand_q(2, 2)
-    assert np.array_equal(r.perform_pir(qb, None, gk), g.perform_pir(qb, None, gk))
-    ls = orc.choose_index_group_secondary(2)
-    rng = pir.Rng(bytes([8] * 32))
-    q, sk = pir.prepare_query(rng, 8 * l - 1, 1, l, ls)
-    want, _ = pir.process_queries([g, g], None, q, None)
-    got, _ = pir.process_queries([g, r], None, q, None)
-    assert got == want
-    assert np.array_equal(pir.process_reply(got[0], sk), e[8 * l - 1])
+    qc = _rand_q(3)[0] if l == N else None
+    assert np.array_equal(r.perform_pir(qb, qc, gk), o.perform_pir(qb, qc, gk))
+    for fleet_len in (1, 2):
+        ls = orc.choose_index_group_secondary(fleet_len)
+        rng = pir.Rng(bytes([8, fleet_len] * 16))
+        q, sk = pir.prepare_query(rng, 8 * l - 1, fleet_len - 1, l, ls)
+        other = pir.PIRDatabase(l)
+        other.set_db(e[::-1].copy())
+        ofl = [orc.Shard(11, l)] if fleet_len == 2 else []
+        for x in ofl:
+            x.set_db(e[::-1].copy())
+        got, _ = pir.process_queries(([other] if fleet_len == 2 else []) + [r], None, q, None)
+        want, _, _ = orc.process_queries(ofl + [o], None, q, None)
+        assert got == want, fleet_len
+        assert np.array_equal(pir.process_reply(got[0], sk), e[8 * l - 1])
 
 
 def test_damaged_files_are_rejected(pir, tmp_path):

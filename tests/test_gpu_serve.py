@@ -197,3 +197,44 @@ def test_batching_window_keeps_replies_exact(pir, dbs):
     _say(f"[serve] window: {st['batches']} batches for 48 requests")
     assert st["completed"] == 48 and st["batches"] < 48
     srv.close()
+
+
+def test_concurrent_two_cu_batches_share_the_device(pir):
+    """Four slots each take a batch of 42 queries (126 secondary chains: the
+    largest two-CU chain launch, 256 blocks) of four databases at once. Only
+    as many two-CU chain blocks as the GPU has CUs may be in flight, so the
+    batches that find the budget taken run their chains on one CU each; no
+    request fails and every reply equals the query answered alone."""
+    fleet, ents = [], []
+    for k in range(4):
+        e = np.random.default_rng(4200 + k).integers(0, 1024, (8 * 1024, N)).astype(np.uint16)
+        s = pir.PIRDatabase(1024)
+        s.set_db(e)
+        fleet.append(s)
+        ents.append(e)
+    ls = pir.choose_index_group_secondary(4)
+    fleets = [[fleet[(k + d) % 4] for k in range(4)] for d in range(4)]  # four databases over the same shards
+    srv = pir.Server(slots=4, max_batch=42, scratch_budget=6 << 30, key_cache_entries=0)
+    ids = [srv.add_database(f) for f in fleets]
+    assert all(srv.max_batch(i) == 42 for i in ids)
+    rng = pir.Rng(bytes([42] * 32))
+    jobs = []
+    for j in range(4 * 42):
+        d = j % 4
+        r = np.random.default_rng(j)
+        pi, si = int(r.integers(0, 8 * 1024)), int(r.integers(0, 4))
+        q, sk = pir.prepare_query(rng, pi, si, 1024, ls)
+        jobs.append((d, pi, si, q, sk))
+    _say("[serve] 168 pair-sized requests: submitting")
+    tickets = [srv.submit(ids[d], 9000 + i, q) for i, (d, pi, si, q, sk) in enumerate(jobs)]
+    for i, ((d, pi, si, q, sk), t) in enumerate(zip(jobs, tickets)):
+        rep, info = srv.wait(t)
+        assert info.status == 0
+        want = ents[(si + d) % 4][pi]
+        assert np.array_equal(pir.process_reply(rep, sk), want), i
+        if i % 7 == 0:
+            assert rep == _alone(pir, (fleets[d],), q), i
+    st = srv.stats()
+    _say(f"[serve] stats {st}")
+    assert st["failed"] == 0 and st["completed"] == 168
+    srv.close()

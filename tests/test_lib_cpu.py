@@ -152,3 +152,50 @@ def test_shard_file_header(tmp_path):
     with pytest.raises(_lib.DnspirError) as ei:
         pir.shard_file_info(str(tmp_path / "missing.shard"))
     assert ei.value.code == -13
+
+
+@pytest.mark.parametrize("n", [0, 2048 * 2048 + 1, 1 << 40])
+def test_shape_rule_refuses_what_the_reference_asserts(n):
+    """get_database_shape asserts 1 <= num_entries <= N^2 (pir_wrapper.rs:221-222):
+    the ABI returns DNSPIR_E_SHAPE and leaves the outputs untouched."""
+    import ctypes
+    lp, ls = ctypes.c_int(-77), ctypes.c_int(-77)
+    assert _lib.lib().dnspir_database_shape(n, ctypes.byref(lp), ctypes.byref(ls)) == -4
+    assert (lp.value, ls.value) == (-77, -77)
+    assert _lib.lib().dnspir_num_databases(n) == 0
+    with pytest.raises(_lib.DnspirError) as ei:
+        pir.get_database_shape(n)
+    assert ei.value.code == -4
+    with pytest.raises(_lib.DnspirError) as ei:
+        pir.num_databases(n)
+    assert ei.value.code == -4
+
+
+def test_shape_rule_edges():
+    """the largest accepted database (N^2 entries = 256 full shards) and the
+    secondary group's bound of N/8 shards (pir_wrapper.rs:168); zero shards
+    give the smallest group, as abs_log2_ceil(0).unwrap_or(0) does there"""
+    assert pir.get_database_shape(2048 * 2048) == (2048, 2048) == orc.database_shape(2048 * 2048)
+    assert pir.num_databases(2048 * 2048) == 256
+    assert pir.choose_index_group_secondary(256) == 2048
+    assert pir.choose_index_group_secondary(0) == 8 == orc.choose_index_group_secondary(0)
+    assert pir.choose_index_group_secondary(129) == 2048
+    assert pir.choose_index_group_secondary(128) == 1024
+    assert _lib.lib().dnspir_choose_index_group_secondary(257) == -4
+    with pytest.raises(_lib.DnspirError) as ei:
+        pir.choose_index_group_secondary(1 << 33)
+    assert ei.value.code == -4
+
+
+def test_every_error_code_has_a_text():
+    """dnspir_strerror and the Python table name every DNSPIR_E_* of the header"""
+    text = open(_lib.HEADER).read()
+    codes = {int(v): k for k, v in re.findall(r"#define (DNSPIR_E_[A-Z_]+) \((-\d+)\)", text)}
+    assert len(codes) >= 14
+    for code, name in codes.items():
+        s = _lib.lib().dnspir_strerror(code).decode()
+        assert s and s != "unknown error", (code, name)
+        assert _lib.ERRORS[code] == name
+    assert _lib.lib().dnspir_strerror(0).decode() == "ok"
+    assert _lib.lib().dnspir_strerror(-999).decode() == "unknown error"
+    assert set(_lib.ERRORS) == set(codes)

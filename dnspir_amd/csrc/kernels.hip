@@ -2140,20 +2140,20 @@ __device__ __forceinline__ void k1w_body(const uint32_t* const* __restrict__ sla
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
-                u32x4 b[4];
+                // one weight digit vector at a time (4 registers live, not 16)
 #pragma unroll
-                for (int v = 0; v < 4; ++v) b[v] = wl[(q * nk + ((kk + rot) & (nk - 1))) * 32 + v * 8 + lb];
+                for (int v = 0; v < 4; ++v) {
+                    const u32x4 b = wl[(q * nk + ((kk + rot) & (nk - 1))) * 32 + v * 8 + lb];
 #pragma unroll
-                for (int u = 0; u < 4; ++u)
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) {
+                    for (int u = 0; u < 4; ++u) {
                         int t = acc[q][u + v];
-                        t = __builtin_amdgcn_sdot4(a[u].x, (int)b[v].x, t, false);
-                        t = __builtin_amdgcn_sdot4(a[u].y, (int)b[v].y, t, false);
-                        t = __builtin_amdgcn_sdot4(a[u].z, (int)b[v].z, t, false);
-                        t = __builtin_amdgcn_sdot4(a[u].w, (int)b[v].w, t, false);
+                        t = __builtin_amdgcn_sdot4(a[u].x, (int)b.x, t, false);
+                        t = __builtin_amdgcn_sdot4(a[u].y, (int)b.y, t, false);
+                        t = __builtin_amdgcn_sdot4(a[u].z, (int)b.z, t, false);
+                        t = __builtin_amdgcn_sdot4(a[u].w, (int)b.w, t, false);
                         acc[q][u + v] = t;
                     }
+                }
             }
         }
     }

@@ -29,6 +29,7 @@ ERRORS = {
     -12: "DNSPIR_E_CLOSED",
     -13: "DNSPIR_E_IO",
     -14: "DNSPIR_E_FORMAT",
+    -15: "DNSPIR_E_SCRATCH",
 }
 E_NOKEY = -11
 QUERY_KEYLESS = 1
@@ -113,6 +114,7 @@ SIGNATURES = {
     "dnspir_double_pir": (ci, [vp, vp, sz, vp, sz, u64p, u64p, u64p, u64p, ctypes.POINTER(Timing)]),
     "dnspir_process_queries": (ci, [vp, vp, sz, vp, sz, u8p, sz, u8p, sz, u8p, u8p, ctypes.POINTER(Timing)]),
     "dnspir_process_queries_device": (ci, [vp, vp, sz, vp, sz, vp, sz, vp, sz, vp, vp, ctypes.POINTER(Timing)]),
+    "dnspir_process_queries_noalloc": (ci, [vp, sz, vp, sz, vp, sz, vp, sz, vp, vp, ctypes.POINTER(Timing)]),
     "dnspir_process_batch": (ci, [vp, vp, sz, u8p, sz, ci, u8p, ctypes.POINTER(Timing)]),
     "dnspir_process_batch_device": (ci, [vp, vp, sz, vp, sz, ci, vp, ctypes.POINTER(Timing)]),
     "dnspir_database_shape": (ci, [sz, ip, ip]),

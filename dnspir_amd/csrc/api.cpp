@@ -25,6 +25,12 @@ namespace {
 thread_local char g_last_error[256] = "";
 }  // namespace
 
+// Fixed device scratch of dnspir_process_queries_noalloc, one copy per GPU,
+// held until the process ends. In host code `scratch` only names the symbol:
+// its device address comes from hipGetSymbolAddress at dnspir_init
+// (Global::NoAlloc::base), and only that address is ever passed on.
+static __device__ __align__(256) unsigned char scratch[DNSPIR_NOALLOC_SCRATCH_BYTES];
+
 int dne::hip_fail(hipError_t e, int line) {
     snprintf(g_last_error, sizeof(g_last_error), "api.cpp:%d: %s", line, hipGetErrorString(e));
     return DNSPIR_E_HIP;
@@ -51,6 +57,15 @@ struct Global {
     // blocks of two-CU chain launches in flight (all contexts) and the CU count
     std::mutex pair_mu;
     int pair_used = 0, pair_cap = 0;
+    // per GPU: the device address of `scratch` and the context whose buffers
+    // are carved from it (dnspir_process_queries_noalloc)
+    struct NoAlloc {
+        char* base = nullptr;
+        dnspir_ctx* ctx = nullptr;
+        Arena arena;
+    };
+    static constexpr int MAX_DEVICES = 64;
+    NoAlloc na[MAX_DEVICES];
 };
 Global g;
 
@@ -179,6 +194,45 @@ int dne::ctx_presize(dnspir_ctx* c, int max_nq) {
     if ((rc = c->idx.ensure((size_t)3 * max_nq * (sizeof(long long) + 2 * sizeof(int)) + 64))) return rc;
     if (c->idx.n != cap) c->idx_uploaded = nullptr;  // the lane lists are uploaded again
     return c->keyp.ensure((size_t)2 * max_nq * sizeof(void*));
+}
+
+// once per GPU, with that GPU current: the device address of `scratch` and
+// the context of the no-allocation entry point
+static int noalloc_setup(int device) {
+    if (device >= Global::MAX_DEVICES) return 0;  // that entry point then reports DNSPIR_E_NOINIT
+    Global::NoAlloc& na = g.na[device];
+    if (na.base) return 0;
+    void* p = nullptr;
+    HIP_OK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p || ((uintptr_t)p & 255)) return DNSPIR_E_ALIGN;
+    dnspir_ctx* c = new (std::nothrow) dnspir_ctx;
+    if (!c) return DNSPIR_E_OOM;
+    const int rc = ctx_init(c, nullptr);
+    if (rc) {
+        delete c;
+        return rc;
+    }
+    // one operation of each kind the entry point issues (kernel launch,
+    // copies from pageable and to pinned host memory, event records) now, on
+    // its stream, so that whatever the runtime sets up on first use of a
+    // stream happens here and not inside a call
+    {
+        uint32_t host_words[64] = {0};
+        HIP_OK(hipEventRecord(c->ev[0], c->stream));
+        launch_fill_u32(static_cast<uint32_t*>(p), 64, 0, c->stream);
+        HIP_OK(hipMemcpyAsync(p, host_words, sizeof(host_words), hipMemcpyHostToDevice, c->stream));
+        HIP_OK(hipMemcpyAsync(c->h_err, p, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(hipEventRecord(c->ev[1], c->stream));
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipStreamSynchronize(c->stream));
+        *c->h_err = 0;
+    }
+    na.arena.base = static_cast<char*>(p);
+    na.arena.cap = DNSPIR_NOALLOC_SCRATCH_BYTES;
+    c->bind_all(&na.arena);  // from now on every buffer of c lives in scratch
+    na.ctx = c;
+    na.base = static_cast<char*>(p);
+    return 0;
 }
 
 int dne::ensure_init() {
@@ -569,6 +623,7 @@ const char* dnspir_strerror(int code) {
         case DNSPIR_E_CLOSED: return "the server is shutting down";
         case DNSPIR_E_IO: return "a shard file could not be opened, written or closed";
         case DNSPIR_E_FORMAT: return "not a shard file of this build's layout, or corrupt or truncated";
+        case DNSPIR_E_SCRATCH: return "the call needs more device scratch than the no-allocation entry point holds";
         default: return "unknown error";
     }
 }
@@ -615,7 +670,8 @@ int dnspir_init(int device) {
     }
     HIP_OK(hipStreamCreateWithFlags(&g.pre_stream, hipStreamNonBlocking));
     g.default_ctx = new dnspir_ctx;
-    return ctx_init(g.default_ctx, nullptr);
+    if ((rc = ctx_init(g.default_ctx, nullptr))) return rc;
+    return noalloc_setup(device);
 }
 
 size_t dnspir_required_bytes(int group_order) {
@@ -1110,6 +1166,97 @@ int dnspir_process_queries_device(dnspir_ctx* c, const dnspir_shard* const* f0, 
     if (!c) c = default_ctx();
     std::lock_guard<std::mutex> lk(c->mu);
     return process_device_impl(c, f0, len0, f1, len1, q0, ql0, q1, ql1, r0, r1, tm, now_us());
+}
+
+}  // extern "C"
+
+namespace {
+
+// Carves, in the no-allocation context's arena, every buffer that
+// process_device_impl takes for these fleets: first what lives through the
+// whole call, then the primary phase's buffers, then -- over the same bytes,
+// which are dead once the primary epilogue has written the secondary entries
+// (one stream orders it) -- the secondary phase's. The call's own ensure()s
+// are then no-ops. DNSPIR_E_SCRATCH if it does not fit.
+int noalloc_reserve(dnspir_ctx* c, Arena* ar, int l_p, const size_t lens[2], const bool present[2], bool joint) {
+    const int l_s = dnspir_choose_index_group_secondary(lens[0]);
+    if (l_s < 0) return l_s;
+    const int L_s = loop_len(l_s), nst_s = nstreams(l_s), nst_p = nstreams(l_p);
+    const int nl = 3 * ((present[0] ? 1 : 0) + (present[1] ? 1 : 0));
+    bool two = false, three = false;  // some slot takes the two-array / three-array inner product
+    for (int sl = 0; sl < 2; ++sl)
+        if (present[sl]) (use_k1w(c, l_p, lens[sl], joint ? 2 : 1) ? two : three) = true;
+    const size_t whole[] = {
+        PTR_SLOTS * sizeof(void*),                                 // ptrs
+        (size_t)nl * (sizeof(long long) + 2 * sizeof(int)) + 64,   // idx
+        (chain_pair_xbuf_words(nst_s * nl) + 64) * 4,              // xch
+        (size_t)LANES * 6 * N * 8,                                 // qcoef
+        (size_t)2 * 6 * 2 * N * 4,                                 // qs
+        (size_t)2 * 4 * 2 * N * 4,                                 // qp
+        (size_t)l_s * LANES * N * 2,                               // se
+    };
+    DevBuf* wb[] = {&c->ptrs, &c->idx, &c->xch, &c->qcoef, &c->qs, &c->qp, &c->se};
+    int rc;
+    for (size_t i = 0; i < sizeof(whole) / sizeof(whole[0]); ++i)
+        if ((rc = wb[i]->ensure(whole[i]))) return DNSPIR_E_SCRATCH;
+    const size_t mark = ar->used;
+    if (loop_len(l_p) == N / 2) {
+        const int nq = joint ? 2 : 1;
+        if ((rc = c->outb.ensure((size_t)nq * lens[0] * 2 * N * LANES * 4))) return DNSPIR_E_SCRATCH;
+        if (two && ((rc = c->wscan.ensure(nq * k1w_scan_bytes())) || (rc = c->wtile.ensure(nq * k1w_tile_bytes(nst_p)))))
+            return DNSPIR_E_SCRATCH;
+        if (three && (rc = c->ql.ensure(qlimb_bytes(1, nst_p)))) return DNSPIR_E_SCRATCH;
+    } else if ((rc = c->acc.ensure((size_t)lens[0] * 2 * N * LANES * 8))) {
+        return DNSPIR_E_SCRATCH;
+    }
+    const size_t primary_end = ar->used;
+    ar->used = mark;
+    const size_t sec[] = {
+        (size_t)nl * l_s * N * 4,                  // M
+        (size_t)nst_s * nl * L_s * 2 * N * 4,      // te
+        (size_t)nst_s * nl * 2 * N * 4,            // aout
+        (size_t)nst_s * nl * 2 * N * 4,            // cout
+        (size_t)nl * N * 8,                        // acoef
+        (size_t)nl * N * 8,                        // bcoef
+        (size_t)nl * N * 4,                        // a17o
+        (size_t)nl * N * 4,                        // b12o
+    };
+    DevBuf* sb[] = {&c->M, &c->te, &c->aout, &c->cout, &c->acoef, &c->bcoef, &c->a17o, &c->b12o};
+    for (size_t i = 0; i < sizeof(sec) / sizeof(sec[0]); ++i)
+        if ((rc = sb[i]->ensure(sec[i]))) return DNSPIR_E_SCRATCH;
+    ar->used = std::max(ar->used, primary_end);
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dnspir_process_queries_noalloc(const dnspir_shard* const* f0, size_t len0, const dnspir_shard* const* f1, size_t len1,
+                                   const uint8_t* q0, size_t ql0, const uint8_t* q1, size_t ql1, uint8_t* r0, uint8_t* r1,
+                                   dnspir_timing* tm) {
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (g.device >= Global::MAX_DEVICES || !g.na[g.device].ctx) return DNSPIR_E_NOINIT;
+    Global::NoAlloc& na = g.na[g.device];
+    dnspir_ctx* c = na.ctx;
+    if (!f0 || len0 == 0 || !f0[0] || len0 < len1) return DNSPIR_E_SHAPE;
+    const int l_p = f0[0]->l;
+    const bool present[2] = {q0 != nullptr, q1 != nullptr};
+    const dnspir_shard* const* fleets[2] = {f0, f1};
+    const size_t lens[2] = {len0, len1};
+    for (int sl = 0; sl < 2; ++sl)
+        if (present[sl] && (rc = check_fleet(fleets[sl], lens[sl], l_p))) return rc;
+    std::lock_guard<std::mutex> lk(c->mu);
+    const bool joint = present[0] && present[1] && len0 == len1 && std::equal(f0, f0 + len0, f1) && use_k1w(c, l_p, len0, 2);
+    const double t0 = now_us();
+    // the carving starts over at every call; the array keeps the last call's
+    // contents, and nothing here relies on them
+    na.arena.used = 0;
+    c->bind_all(&na.arena);
+    if ((rc = noalloc_reserve(c, &na.arena, l_p, lens, present, joint))) return rc;
+    rc = process_device_impl(c, f0, len0, f1, len1, q0, ql0, q1, ql1, r0, r1, tm, t0);
+    return rc == DNSPIR_E_OOM ? DNSPIR_E_SCRATCH : rc;
 }
 
 int dnspir_process_queries(dnspir_ctx* c, const dnspir_shard* const* f0, size_t len0, const dnspir_shard* const* f1,

@@ -188,6 +188,19 @@ struct dnspir_ctx {
             b->arena = a;
         }
     }
+    // the same for the small persistent buffers too (a context whose every
+    // buffer lives in a caller's region; never called on one that used hipMalloc)
+    void bind_all(dne::Arena* a) {
+        bind_arena(a);
+        for (auto* b : {&ptrs, &idx, &keyp, &xch}) {
+            b->p = nullptr;
+            b->n = 0;
+            b->arena = a;
+        }
+        cached[0].clear();
+        cached[1].clear();
+        idx_uploaded = nullptr;
+    }
 };
 
 namespace dne {

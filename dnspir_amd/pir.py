@@ -247,6 +247,24 @@ def process_queries_device(fleet0, fleet1, query0_dev, query1_dev, reply0_dev, r
     return tm
 
 
+def process_queries_noalloc(fleet0, fleet1, query0_dev, query1_dev, reply0_dev, reply1_dev):
+    """process_queries_device with all device scratch in the library's fixed
+    per-GPU array (dnspir_process_queries_noalloc): allocates nothing."""
+    import torch
+    init()
+    f0, l0 = _fleet(fleet0)
+    f1, l1 = _fleet(fleet1)
+    tm = Timing()
+    torch.cuda.current_stream().synchronize()
+    dp = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None  # noqa: E731
+    check(lib().dnspir_process_queries_noalloc(f0, l0, f1, l1,
+                                               dp(query0_dev), query0_dev.numel() if query0_dev is not None else 0,
+                                               dp(query1_dev), query1_dev.numel() if query1_dev is not None else 0,
+                                               dp(reply0_dev), dp(reply1_dev), ctypes.byref(tm)),
+          "process_queries_noalloc")
+    return tm
+
+
 def process_batch(fleet, queries, ctx=None):
     """B independent queries against one fleet in one pass (host bytes);
     returns (list of reply bytes, Timing)."""

@@ -37,6 +37,7 @@ extern "C" {
 #define DNSPIR_E_CLOSED (-12)     /* the server is shutting down */
 #define DNSPIR_E_IO (-13)         /* a shard file could not be opened, written or closed */
 #define DNSPIR_E_FORMAT (-14)     /* not a shard file of this build's layout, or corrupt / truncated */
+#define DNSPIR_E_SCRATCH (-15)    /* the call needs more than the fixed scratch of dnspir_process_queries_noalloc */
 
 typedef struct dnspir_shard dnspir_shard;
 typedef struct dnspir_ctx dnspir_ctx;
@@ -177,6 +178,23 @@ int dnspir_process_queries_device(dnspir_ctx* ctx, const dnspir_shard* const* fl
                                   const dnspir_shard* const* fleet1, size_t len1, const uint8_t* query0_dev,
                                   size_t qlen0, const uint8_t* query1_dev, size_t qlen1, uint8_t* reply0_dev,
                                   uint8_t* reply1_dev, dnspir_timing* timing);
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): dnspir_process_queries_device with every scratch buffer
+ * carved from one fixed device array of DNSPIR_NOALLOC_SCRATCH_BYTES per GPU,
+ * set up by dnspir_init and held until the process ends. The call allocates
+ * and frees nothing (no hipMalloc / hipFree) and runs on a stream the library
+ * owns for these calls. It answers one query (query1 / reply1 NULL) against
+ * any fleet the reference allows (up to N^2 entries), or both slots when their
+ * scratch fits; larger calls fail with DNSPIR_E_SCRATCH before any device work.
+ * Every such call on a GPU uses that GPU's one array: calls are serialised by a
+ * lock, and overlapping them from several threads only waits; device work of
+ * other streams must not read the array. Replies equal
+ * dnspir_process_queries_device's byte for byte. */
+#define DNSPIR_NOALLOC_SCRATCH_BYTES ((size_t)224 << 20)
+int dnspir_process_queries_noalloc(const dnspir_shard* const* fleet0, size_t len0, const dnspir_shard* const* fleet1,
+                                   size_t len1, const uint8_t* query0_dev, size_t qlen0, const uint8_t* query1_dev,
+                                   size_t qlen1, uint8_t* reply0_dev, uint8_t* reply1_dev, dnspir_timing* timing);
 /* Batched serving: nq independent queries (each answered exactly as
  * process_query, pir_wrapper.rs:328-337) against one fleet of half- or
  * full-size shards, sharing one pass over the shards (an int8-MFMA contraction

@@ -30,6 +30,7 @@ ERRORS = {
     -13: "DNSPIR_E_IO",
     -14: "DNSPIR_E_FORMAT",
     -15: "DNSPIR_E_SCRATCH",
+    -16: "DNSPIR_E_FRAME",
 }
 E_NOKEY = -11
 QUERY_KEYLESS = 1
@@ -70,6 +71,17 @@ class CtxStats(ctypes.Structure):
     _fields_ = [("pair_runs", ctypes.c_uint64), ("pair_refused", ctypes.c_uint64), ("pair_retries", ctypes.c_uint64)]
 
 
+class FrontendConfig(ctypes.Structure):
+    _fields_ = [("server", ctypes.c_void_p), ("db", ctypes.c_int), ("num_entries", ctypes.c_size_t),
+                ("zone_name", ctypes.c_char_p), ("hash_salt", ctypes.c_uint64), ("dump", ctypes.c_void_p),
+                ("dump_len", ctypes.c_size_t)]
+
+
+class FrontendStats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_uint64) for k in ("connections", "frames_in", "frames_out", "bytes_in", "bytes_out",
+                                                "queries", "infos", "dumps", "unavailable", "errors")]
+
+
 class TuneEntry(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 48), ("column_tiles", ctypes.c_int), ("ms", ctypes.c_double),
                 ("exact", ctypes.c_int), ("chosen", ctypes.c_int)]
@@ -102,6 +114,13 @@ SIGNATURES = {
     "dnspir_shard_set_db": (ci, [vp, u16p, sz]),
     "dnspir_shard_set_db_device": (ci, [vp, vp, sz]),
     "dnspir_shard_get_a": (ci, [vp, u64p]),
+    "dnspir_shard_set_db_buckets": (ci, [vp, u8p, sz]),
+    "dnspir_pack_bytes_into_coeffs": (ci, [u8p, sz, u16p]),
+    "dnspir_frontend_create": (ci, [ctypes.POINTER(FrontendConfig), ctypes.POINTER(vp)]),
+    "dnspir_frontend_handle_fd": (ci, [vp, ci]),
+    "dnspir_frontend_listen": (ci, [vp, ctypes.c_char_p, ci, ctypes.POINTER(ci)]),
+    "dnspir_frontend_get_stats": (ci, [vp, ctypes.POINTER(FrontendStats)]),
+    "dnspir_frontend_destroy": (ci, [vp]),
     "dnspir_shard_perform_pir": (ci, [vp, u64p, u64p, u64p, u64p]),
     "dnspir_shard_perform_pir_split": (ci, [vp, u64p, u64p, u64p, u64p]),
     "dnspir_ctx_create": (ci, [vp, ctypes.POINTER(vp)]),

@@ -52,7 +52,7 @@ struct Global {
     DevTables* tables = nullptr;
     uint64_t layout = 0;  // fingerprint of the device shard layout (persisted shard files)
     hipStream_t pre_stream = nullptr;
-    DevBuf pre_entries, pre_M, pre_te, pre_ip, pre_aout, pre_idx;
+    DevBuf pre_entries, pre_M, pre_te, pre_ip, pre_aout, pre_idx, pre_buckets;
     dnspir_ctx* default_ctx = nullptr;
     // blocks of two-CU chain launches in flight (all contexts) and the CU count
     std::mutex pair_mu;
@@ -624,6 +624,7 @@ const char* dnspir_strerror(int code) {
         case DNSPIR_E_IO: return "a shard file could not be opened, written or closed";
         case DNSPIR_E_FORMAT: return "not a shard file of this build's layout, or corrupt or truncated";
         case DNSPIR_E_SCRATCH: return "the call needs more device scratch than the no-allocation entry point holds";
+        case DNSPIR_E_FRAME: return "frame too large, empty request or unknown command";
         default: return "unknown error";
     }
 }
@@ -850,15 +851,23 @@ int dnspir_shard_load(const char* path, void* dev_mem, size_t bytes, dnspir_shar
     return 0;
 }
 
-static int set_db_impl(dnspir_shard* sh, const uint16_t* entries, size_t n_entries, bool on_device) {
-    if (!sh || !entries) return DNSPIR_E_ARG;
+// where set_db's entries come from
+enum EntrySource { HOST_COEFFS, DEVICE_COEFFS, HOST_BUCKETS };
+
+static int set_db_impl(dnspir_shard* sh, const void* src, size_t n, EntrySource kind) {
+    if (!sh || !src) return DNSPIR_E_ARG;
     int rc = ensure_init();
     if (rc) return rc;
-    if (n_entries != (size_t)LANES * sh->l) return DNSPIR_E_COUNT;
+    const size_t n_entries = (size_t)LANES * sh->l;
+    // base_pir.rs:693 asks for exactly 8*l entries; buckets may fill a
+    // shard only partly (the last shard of a database, server.rs:664-739)
+    if (kind == HOST_BUCKETS ? n > n_entries : n != n_entries) return DNSPIR_E_COUNT;
     std::lock_guard<std::mutex> lk(g.mu);
     const int l = sh->l, L = sh->L, nst = sh->nst;
     const hipStream_t s = g.pre_stream;
+    const bool on_device = kind == DEVICE_COEFFS;
     if (!on_device && (rc = g.pre_entries.ensure(n_entries * N * 2))) return rc;
+    if (kind == HOST_BUCKETS && (rc = g.pre_buckets.ensure(std::max<size_t>(n, 1) * BUCKET_BYTES))) return rc;
     if ((rc = g.pre_M.ensure((size_t)LANES * l * N * 4))) return rc;
     if ((rc = g.pre_te.ensure((size_t)nst * LANES * L * 2 * N * 4))) return rc;
     if ((rc = g.pre_ip.ensure((size_t)nst * LANES * L * 6 * N * 4))) return rc;
@@ -871,9 +880,13 @@ static int set_db_impl(dnspir_shard* sh, const uint16_t* entries, size_t n_entri
         h_nc[i] = l;
     }
     HIP_OK(hipMemcpyAsync(g.pre_idx.p, h_idx, sizeof(h_idx), hipMemcpyHostToDevice, s));
-    const uint16_t* ent = entries;
-    if (!on_device) {
-        HIP_OK(hipMemcpyAsync(g.pre_entries.p, entries, n_entries * N * 2, hipMemcpyHostToDevice, s));
+    const uint16_t* ent = static_cast<const uint16_t*>(src);
+    if (kind == HOST_COEFFS) {
+        HIP_OK(hipMemcpyAsync(g.pre_entries.p, src, n_entries * N * 2, hipMemcpyHostToDevice, s));
+        ent = g.pre_entries.as<uint16_t>();
+    } else if (kind == HOST_BUCKETS) {
+        if (n) HIP_OK(hipMemcpyAsync(g.pre_buckets.p, src, n * BUCKET_BYTES, hipMemcpyHostToDevice, s));
+        launch_pack_buckets(g.pre_buckets.as<uint8_t>(), n, n_entries, g.pre_entries.as<uint16_t>(), s);
         ent = g.pre_entries.as<uint16_t>();
     }
     launch_k5(ent, g.pre_idx.as<long long>(), (long long)LANES * N, g.pre_M.as<uint32_t>(), g.pre_te.as<uint32_t>(), l,
@@ -889,11 +902,28 @@ static int set_db_impl(dnspir_shard* sh, const uint16_t* entries, size_t n_entri
 }
 
 int dnspir_shard_set_db(dnspir_shard* sh, const uint16_t* entries, size_t n_entries) {
-    return set_db_impl(sh, entries, n_entries, false);
+    return set_db_impl(sh, entries, n_entries, HOST_COEFFS);
 }
 
 int dnspir_shard_set_db_device(dnspir_shard* sh, const uint16_t* entries_dev, size_t n_entries) {
-    return set_db_impl(sh, entries_dev, n_entries, true);
+    return set_db_impl(sh, entries_dev, n_entries, DEVICE_COEFFS);
+}
+
+int dnspir_shard_set_db_buckets(dnspir_shard* sh, const uint8_t* buckets, size_t n_buckets) {
+    static const uint8_t none = 0;
+    return set_db_impl(sh, n_buckets ? (const void*)buckets : (const void*)&none, n_buckets, HOST_BUCKETS);
+}
+
+int dnspir_pack_bytes_into_coeffs(const uint8_t* buckets, size_t n, uint16_t* coeffs) {
+    if ((!buckets || !coeffs) && n) return DNSPIR_E_ARG;
+    for (size_t e = 0; e < n; ++e)
+        for (int gi = 0; gi < N / 4; ++gi) {
+            const uint8_t* p = buckets + e * BUCKET_BYTES + gi * 5;
+            uint64_t v = 0;
+            for (int b = 0; b < 5; ++b) v |= (uint64_t)p[b] << (8 * b);
+            for (int k = 0; k < 4; ++k) coeffs[e * N + gi * 4 + k] = (uint16_t)((v >> (10 * k)) & 1023);
+        }
+    return 0;
 }
 
 int dnspir_shard_get_a(const dnspir_shard* sh, uint64_t* out) {

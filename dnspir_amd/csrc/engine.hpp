@@ -21,6 +21,9 @@ using dn::N;
 // wire widths (pir_wrapper.rs:57-72)
 constexpr int A_BITS = 17, B_BITS = 12, GK_BITS = 50, QRY_BITS = 34;
 constexpr size_t REPLY_BYTES = 3 * (N * A_BITS / 8 + N * B_BITS / 8);
+// bytes of one database entry as the DNS layer stores it: N ten-bit
+// coefficients, 4 per 5 bytes (dns/bucket.rs:54-73)
+constexpr size_t BUCKET_BYTES = N / 4 * 5;
 // the two Galois keys at the head of every query (pir_wrapper.rs:370-372)
 constexpr size_t KEY_BYTES = 2 * (N / 8) * GK_BITS;
 // a device-cached key: NTT rows of gk0, gk1 in both moduli, in slot order
@@ -234,6 +237,9 @@ int ctx_presize(dnspir_ctx* c, int max_nq);
 // context's graphs on): the device work runs as a captured HIP graph.
 int batch_impl(dnspir_ctx* c, const dnspir_shard* const* fleet, size_t len, const uint8_t* qdev, size_t qlen, int nq,
                uint8_t* rdev, dnspir_timing* tm, double t0, const BatchKeys* keys, bool sync = true, bool graph = false);
+
+// primary group order and fleet length of a serving loop's database
+int server_db_shape(dnspir_server* s, int db, int* l_p, size_t* len);
 
 // device scratch bytes batch_impl takes from an arena for nq queries
 // against a fleet of `len` shards of order l_p, plus the query and reply

@@ -2273,6 +2273,32 @@ __global__ void diff_kernel(const uint32_t* a, const uint32_t* b, size_t n, unsi
         }
 }
 
+// Database entries from 2560-byte buckets (pack_bytes_into_coeffs,
+// dns/bucket.rs:289-303): one thread per 5-byte group, read as a
+// little-endian 40-bit integer v, writes the four ten-bit coefficients
+// v >> 0, >> 10, >> 20, >> 30. Entries from n_valid to n_total are zero
+// (the padding of a database's last shard).
+__global__ void pack_buckets_kernel(const uint8_t* __restrict__ b, size_t n_valid, size_t n_total, uint16_t* __restrict__ c) {
+    const size_t groups = n_total * (N / 4);
+    for (size_t gi = blockIdx.x * (size_t)blockDim.x + threadIdx.x; gi < groups; gi += (size_t)gridDim.x * blockDim.x) {
+        uint64_t v = 0;
+        if (gi < n_valid * (N / 4)) {
+            const uint8_t* p = b + gi * 5;
+            v = (uint64_t)p[0] | ((uint64_t)p[1] << 8) | ((uint64_t)p[2] << 16) | ((uint64_t)p[3] << 24) | ((uint64_t)p[4] << 32);
+        }
+        ushort4 o;
+        o.x = (uint16_t)(v & 1023);
+        o.y = (uint16_t)((v >> 10) & 1023);
+        o.z = (uint16_t)((v >> 20) & 1023);
+        o.w = (uint16_t)((v >> 30) & 1023);
+        reinterpret_cast<ushort4*>(c)[gi] = o;
+    }
+}
+
+void launch_pack_buckets(const uint8_t* buckets, size_t n_valid, size_t n_total, uint16_t* coeffs, hipStream_t s) {
+    hipLaunchKernelGGL(pack_buckets_kernel, dim3(1024), dim3(256), 0, s, buckets, n_valid, n_total, coeffs);
+}
+
 void launch_fill_hash(uint32_t* p, size_t n, uint32_t seed, hipStream_t s) {
     hipLaunchKernelGGL(fill_hash_kernel, dim3(2048), dim3(256), 0, s, p, n, seed);
 }

@@ -58,6 +58,9 @@ int k1b_get_choice(int ct, int nst);
 void launch_k1b_variant(int i, const uint32_t* const* slabs, int nshards, const uint32_t* ql, uint32_t* outb, int nst, int nq,
                         hipStream_t s);
 void launch_fill_hash(uint32_t* p, size_t n, uint32_t seed, hipStream_t s);
+// n_total entries [n_total][N] u16 from n_valid buckets of 2560 bytes (the
+// rest zero): pack_bytes_into_coeffs, dns/bucket.rs:289-303
+void launch_pack_buckets(const uint8_t* buckets, size_t n_valid, size_t n_total, uint16_t* coeffs, hipStream_t s);
 // *flag |= 1 where a and b differ
 void launch_diff(const uint32_t* a, const uint32_t* b, size_t n, unsigned* flag, hipStream_t s);
 void launch_chain_out(const uint32_t* a_out, const uint32_t* c_out, int nst, int nlanes, uint64_t* a_coef, uint64_t* b_coef,

@@ -648,6 +648,19 @@ int dnspir_server_replay(dnspir_server* s, const dnspir_arrival* arr, size_t n, 
     return first;
 }
 
+}  // extern "C"
+
+int dne::server_db_shape(dnspir_server* s, int db, int* l_p, size_t* len) {
+    if (!s) return DNSPIR_E_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (db < 0 || db >= (int)s->dbs.size()) return DNSPIR_E_ARG;
+    *l_p = s->dbs[db].l_p;
+    *len = s->dbs[db].fleet.size();
+    return 0;
+}
+
+extern "C" {
+
 int dnspir_server_get_stats(dnspir_server* s, dnspir_server_stats* out) {
     if (!s || !out) return DNSPIR_E_ARG;
     std::lock_guard<std::mutex> lk(s->mu);

@@ -15,8 +15,10 @@ import os
 
 import numpy as np
 
-from ._lib import (LANES, N, QUERY_KEYLESS, Arrival, CtxStats, DnspirError, RequestInfo, ServerConfig, ServerStats,
-                   Timing, TuneEntry, check, lib)
+from ._lib import (LANES, N, QUERY_KEYLESS, Arrival, CtxStats, DnspirError, FrontendConfig, FrontendStats, RequestInfo,
+                   ServerConfig, ServerStats, Timing, TuneEntry, check, lib)
+
+BUCKET_BYTES = 2560
 
 _ptr = lambda a, t: a.ctypes.data_as(t)  # noqa: E731
 u8p = ctypes.POINTER(ctypes.c_uint8)
@@ -113,6 +115,13 @@ class PIRDatabase:
         # the library works on its own streams: finish the producer's work first
         torch.cuda.current_stream().synchronize()
         check(lib().dnspir_shard_set_db_device(self.h, ctypes.c_void_p(e.data_ptr()), e.shape[0]), "set_db_device")
+
+    def set_db_buckets(self, buckets):
+        """set_db from up to 8*l DNS buckets of 2560 bytes ([n][2560] uint8),
+        packed into coefficients on the device (dns/bucket.rs:289-303); the
+        remaining entries of the shard are zero."""
+        b = np.ascontiguousarray(buckets, dtype=np.uint8).reshape(-1, BUCKET_BYTES)
+        check(lib().dnspir_shard_set_db_buckets(self.h, _ptr(b, u8p), b.shape[0]), "set_db_buckets")
 
     def get_a(self):
         out = np.zeros((LANES, N), np.uint64)
@@ -466,6 +475,56 @@ class Server:
         h = getattr(self, "h", None)
         if h is not None and h.value:
             lib().dnspir_server_destroy(h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # interpreter shutdown: the library is already gone
+            pass
+
+
+def pack_bytes_into_coeffs(buckets):
+    """dns/bucket.rs:289-303 on the host: [n][2560] bytes -> [n][2048] uint16"""
+    b = np.ascontiguousarray(buckets, dtype=np.uint8).reshape(-1, BUCKET_BYTES)
+    out = np.zeros((b.shape[0], N), np.uint16)
+    check(lib().dnspir_pack_bytes_into_coeffs(_ptr(b, u8p), b.shape[0], _ptr(out, u16p)), "pack_bytes_into_coeffs")
+    return out
+
+
+class Frontend:
+    """The reference server's TCP protocol (dns/protocol.rs, server.rs:747-807)
+    in front of a serving loop database: INFO / QUERY / DUMP frames."""
+
+    def __init__(self, server=None, db=0, num_entries=0, zone_name="", hash_salt=0, dump=None):
+        self._dump = None if dump is None else ctypes.create_string_buffer(bytes(dump), len(dump))
+        self._zone = zone_name.encode()
+        cfg = FrontendConfig(server.h if server is not None else None, db, num_entries, self._zone, hash_salt,
+                             ctypes.cast(self._dump, ctypes.c_void_p) if self._dump is not None else None,
+                             len(dump) if dump is not None else 0)
+        h = ctypes.c_void_p()
+        check(lib().dnspir_frontend_create(ctypes.byref(cfg), ctypes.byref(h)), "dnspir_frontend_create")
+        self.h = h
+        self.server = server  # must outlive the front end
+
+    def handle_fd(self, fd):
+        """serves one connection on fd until the peer closes; returns the status"""
+        return lib().dnspir_frontend_handle_fd(self.h, fd)
+
+    def listen(self, host="127.0.0.1", port=0):
+        bound = ctypes.c_int()
+        check(lib().dnspir_frontend_listen(self.h, host.encode(), port, ctypes.byref(bound)), "frontend_listen")
+        return bound.value
+
+    def stats(self):
+        st = FrontendStats()
+        check(lib().dnspir_frontend_get_stats(self.h, ctypes.byref(st)), "frontend_get_stats")
+        return {k: getattr(st, k) for k, _ in FrontendStats._fields_}
+
+    def close(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value:
+            lib().dnspir_frontend_destroy(h)
             self.h = None
 
     def __del__(self):

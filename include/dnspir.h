@@ -38,6 +38,8 @@ extern "C" {
 #define DNSPIR_E_IO (-13)         /* a shard file could not be opened, written or closed */
 #define DNSPIR_E_FORMAT (-14)     /* not a shard file of this build's layout, or corrupt / truncated */
 #define DNSPIR_E_SCRATCH (-15)    /* the call needs more than the fixed scratch of dnspir_process_queries_noalloc */
+#define DNSPIR_E_FRAME (-16)      /* front end: frame over 64 MiB, empty request or unknown command,
+                                     protocol.rs:111-113, server.rs:756-761, 798-803 */
 
 typedef struct dnspir_shard dnspir_shard;
 typedef struct dnspir_ctx dnspir_ctx;
@@ -103,6 +105,17 @@ int dnspir_shard_set_db_device(dnspir_shard* shard, const uint16_t* entries_dev,
 int dnspir_shard_save(const dnspir_shard* shard, const char* path);
 int dnspir_shard_file_info(const char* path, int* group_order, int* used_seeds, size_t* bytes);
 int dnspir_shard_load(const char* path, void* dev_mem, size_t bytes, dnspir_shard** out);
+/* set_db from the DNS layer's entries: up to 8 * group_order buckets of
+ * DNSPIR_BUCKET_BYTES bytes (host memory, back to back), each packed into N
+ * ten-bit coefficients on the device as pack_bytes_into_coeffs does
+ * (dns/bucket.rs:289-303); the shard's remaining entries are zero, as the
+ * last shard of a database is padded (pir_wrapper.rs:235-237,
+ * server.rs:664-739). DNSPIR_E_COUNT for more than 8 * group_order buckets. */
+#define DNSPIR_BUCKET_BYTES 2560
+int dnspir_shard_set_db_buckets(dnspir_shard* shard, const uint8_t* buckets, size_t n_buckets);
+/* pack_bytes_into_coeffs (dns/bucket.rs:289-303) on the host: n buckets ->
+ * coeffs [n][N] */
+int dnspir_pack_bytes_into_coeffs(const uint8_t* buckets, size_t n, uint16_t* coeffs);
 /* PIRDatabase::get_a (base_pir.rs:897-903): out [8][N] host */
 int dnspir_shard_get_a(const dnspir_shard* shard, uint64_t* out);
 /* PIRDatabase::perform_pir (base_pir.rs:945-952): qb [N], qb_conj [N] (NULL
@@ -320,6 +333,61 @@ int dnspir_server_get_stats(dnspir_server* s, dnspir_server_stats* out);
 /* stops the workers after their current batches; queued requests end with
  * DNSPIR_E_CLOSED. Call after every dnspir_server_wait has returned. */
 int dnspir_server_destroy(dnspir_server* s);
+
+/* ---- DNS-over-PIR front end: the reference server's TCP protocol ----------
+ * (crates/dns-over-pir/src/dns/protocol.rs, server.rs:152-173, 747-807).
+ * Frames are a 4-byte big-endian length and at most 64 MiB of payload; a
+ * request is [cmd][body]. CMD_INFO answers encode_info's bytes (flags,
+ * num_entries u32, zone name length u32 + bytes, dump size u32, salt u64, all
+ * big endian); CMD_QUERY answers the reply bytes of the query in its body
+ * (the serving loop's dnspir_server_submit / _wait in place of process_query,
+ * so the queries of all connections are batched together); CMD_DUMP answers
+ * the dump bytes; a capability the front end lacks answers the single byte
+ * DNSPIR_RESP_UNAVAILABLE. A connection ends cleanly when the peer closes
+ * (also inside a frame, as read_exact's UnexpectedEof does there); an
+ * oversized frame, an empty request or an unknown command end it with
+ * DNSPIR_E_FRAME, a query the serving loop refuses with that loop's error.
+ * The zone parser, bucket builder and resolver of the reference's DNS layer
+ * are not part of this library. */
+#define DNSPIR_CMD_INFO 1
+#define DNSPIR_CMD_QUERY 2
+#define DNSPIR_CMD_DUMP 3
+#define DNSPIR_FLAG_PIR_AVAILABLE 1
+#define DNSPIR_FLAG_DUMP_AVAILABLE 2
+#define DNSPIR_RESP_UNAVAILABLE 0xFF
+#define DNSPIR_FRAME_MAX ((size_t)64 << 20)
+
+typedef struct dnspir_frontend dnspir_frontend;
+typedef struct {
+    dnspir_server* server;   /* answers CMD_QUERY; NULL = no PIR (RESP_UNAVAILABLE) */
+    int db;                  /* its database (dnspir_server_add_database) */
+    size_t num_entries;      /* entries advertised by INFO; get_database_shape of it must
+                                give that database's group order and shard count */
+    const char* zone_name;   /* INFO zone name ("" = the root zone) */
+    uint64_t hash_salt;      /* INFO hash salt */
+    const uint8_t* dump;     /* bytes answered to CMD_DUMP (the gzipped zone), NULL = none */
+    size_t dump_len;
+} dnspir_frontend_config;
+
+typedef struct {
+    uint64_t connections, frames_in, frames_out, bytes_in, bytes_out;  /* bytes include length prefixes */
+    uint64_t queries, infos, dumps;
+    uint64_t unavailable;    /* RESP_UNAVAILABLE answers */
+    uint64_t errors;         /* listener connections that ended with an error */
+} dnspir_frontend_stats;
+
+int dnspir_frontend_create(const dnspir_frontend_config* cfg, dnspir_frontend** out);
+/* handle (server.rs:747-807) on one connected stream socket (or pipe pair
+ * read and written through one fd) in the calling thread: 0 when the peer
+ * closed, else the error that ended it. The fd is not closed. */
+int dnspir_frontend_handle_fd(dnspir_frontend* fe, int fd);
+/* run (server.rs:152-173): listens on host:port (IPv4, "0.0.0.0" = any; port 0
+ * = a free one, written to *bound_port) and handles each connection on a
+ * thread of its own until dnspir_frontend_destroy */
+int dnspir_frontend_listen(dnspir_frontend* fe, const char* host, int port, int* bound_port);
+int dnspir_frontend_get_stats(dnspir_frontend* fe, dnspir_frontend_stats* out);
+/* stops the listener, ends open connections and waits for their threads */
+int dnspir_frontend_destroy(dnspir_frontend* fe);
 
 /* ---- client side (host only): prepare_query / process_reply --------------- */
 /* StdRng stand-in: ChaCha12 in rand_chacha 0.3 word order */

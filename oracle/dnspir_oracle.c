@@ -1271,3 +1271,24 @@ int orc_process_reply(const uint8_t* reply, size_t rlen, const uint8_t sk_seed[3
     free(sk); free(tmp); free(a); free(b); free(parts);
     return rc;
 }
+
+/* ---- bucket <-> coefficient packing (crates/dns-over-pir/src/dns/bucket.rs) ----
+ * 2560-byte bucket -> N = 2048 ten-bit coefficients: each group of 5 bytes
+ * read as a little-endian 40-bit integer v gives the 4 coefficients
+ * v >> 0, >> 10, >> 20, >> 30 (masked to 10 bits), bucket.rs:289-303. */
+void orc_pack_bytes_into_coeffs(const uint8_t* bucket, uint16_t* coeffs) {
+    for (int g = 0; g < 2048 / 4; ++g) {
+        uint64_t v = 0;
+        for (int b = 4; b >= 0; --b) v = (v << 8) | bucket[g * 5 + b];
+        for (int k = 0; k < 4; ++k) coeffs[g * 4 + k] = (uint16_t)((v >> (10 * k)) & 1023);
+    }
+}
+
+/* the inverse (bucket.rs:271-283): low 10 bits of each coefficient */
+void orc_unpack_coeffs_into_bytes(const uint16_t* coeffs, uint8_t* bucket) {
+    for (int g = 0; g < 2048 / 4; ++g) {
+        uint64_t v = 0;
+        for (int k = 0; k < 4; ++k) v |= (uint64_t)(coeffs[g * 4 + k] & 1023) << (10 * k);
+        for (int b = 0; b < 5; ++b) bucket[g * 5 + b] = (uint8_t)(v >> (8 * b));
+    }
+}

@@ -101,6 +101,11 @@ int orc_prepare_query(orc_rng* rng, size_t primary_idx, size_t secondary_idx, in
                       uint8_t* query, size_t* qlen, uint8_t sk_seed[32]);
 int orc_process_reply(const uint8_t* reply, size_t rlen, const uint8_t sk_seed[32], uint16_t* out);
 
+/* ---- bucket packing (dns/bucket.rs:271-303), N = 2048 only --------------- */
+/* 2560 bucket bytes -> 2048 ten-bit coefficients, and back */
+void orc_pack_bytes_into_coeffs(const uint8_t* bucket, uint16_t* coeffs);
+void orc_unpack_coeffs_into_bytes(const uint16_t* coeffs, uint8_t* bucket);
+
 /* threads used by the oracle's OpenMP loops (0 = runtime default) */
 void orc_set_threads(int n);
 

@@ -73,6 +73,8 @@ def lib():
             "orc_choose_index_group_secondary": (ctypes.c_int, [ctypes.c_int, sz]),
             "orc_condense": (sz, [ctypes.c_int, u64p, sz, u8p]),
             "orc_uncondense": (ctypes.c_int, [ctypes.c_int, sz, u8p, sz, ctypes.POINTER(sz), u64p]),
+            "orc_pack_bytes_into_coeffs": (None, [u8p, u16p]),
+            "orc_unpack_coeffs_into_bytes": (None, [u16p, u8p]),
             "orc_database_shape": (None, [sz, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
             "orc_num_databases": (sz, [sz]),
             "orc_query_bytes": (sz, [ctypes.c_int, ctypes.c_int]),
@@ -364,4 +366,22 @@ def process_reply(reply, sk_seed):
     rc = lib().orc_process_reply(ptr(r, u8p), len(reply), ptr(s, u8p), ptr(out, u16p))
     if rc:
         raise ValueError(f"process_reply failed: {rc}")
+    return out
+
+
+def pack_bytes_into_coeffs(buckets):
+    """dns/bucket.rs:289-303: [n][2560] bucket bytes -> [n][2048] coefficients"""
+    b = np.ascontiguousarray(buckets, np.uint8).reshape(-1, 2560)
+    out = np.zeros((b.shape[0], 2048), np.uint16)
+    for i in range(b.shape[0]):
+        lib().orc_pack_bytes_into_coeffs(ptr(b[i], u8p), ptr(out[i], u16p))
+    return out
+
+
+def unpack_coeffs_into_bytes(coeffs):
+    """dns/bucket.rs:271-283: [n][2048] coefficients -> [n][2560] bytes"""
+    c = np.ascontiguousarray(coeffs, np.uint16).reshape(-1, 2048)
+    out = np.zeros((c.shape[0], 2560), np.uint8)
+    for i in range(c.shape[0]):
+        lib().orc_unpack_coeffs_into_bytes(ptr(c[i], u16p), ptr(out[i], u8p))
     return out

@@ -62,35 +62,68 @@ def read_traffic(workload_tag):
     return float(m.group(1)) if m else None
 
 
+PUBLISHED_ONE_THREAD_MBS = 15341  # main_pir_loop, one thread, Xeon Gold 5318S (ARTIFACT-APPENDIX.md:151)
+
+
 def cpu_baseline(lp, nshards, ls, query_block, threads):
-    """Oracle ("port") on a bounded sample: one full shard preprocessed, the
-    primary phase over 2 fleet entries and the full-size secondary phase
-    (l_s as in the workload) timed; scaled to ms/query for `nshards`."""
+    """The reference's primary loop timed on this host's cores (kind "port"):
+    main_pir_loop in its own form (oracle/cpu_baseline.c: lazy u64 sums
+    reduced every 341 rows, AVX-512 / AVX2, OpenMP over the (shard, modulus,
+    stream) loops as rayon does) on a bounded sample -- two distinct oracle
+    shards of order lp, one thread on one shard, then all threads on a fleet of
+    16 entries cycling them -- scaled to `nshards` shards; plus one secondary
+    phase at l_s from the oracle (the fully reducing checker). The port's
+    reply for the sample query is compared with the oracle's perform_pir."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as orc
     orc.set_threads(threads)
-    e = np.random.default_rng(7).integers(0, 1024, (8 * lp, 2048)).astype(np.uint16)
     t0 = time.perf_counter()
-    sh = orc.Shard(11, lp)
-    sh.set_db(e)
+    shards = []
+    for k in range(2):
+        e = np.random.default_rng(7 + k).integers(0, 1024, (8 * lp, 2048)).astype(np.uint16)
+        sh = orc.Shard(11, lp)
+        sh.set_db(e)
+        shards.append(sh)
     t_pre = time.perf_counter() - t0
-    outs, pu, su = orc.double_pir([sh, sh], None, query_block, None, ls)
-    per_shard_ms = pu / 1e3 / 2
-    ms = nshards * per_shard_ms + su / 1e3
-    # the same on one thread (one fleet entry)
-    orc.set_threads(1)
-    _, pu1, su1 = orc.double_pir([sh], None, query_block, None, ls)
-    orc.set_threads(threads)
-    ms1 = nshards * pu1 / 1e3 + su1 / 1e3
+    conj = lp == 2048
+    lanes = lambda v: np.ascontiguousarray(np.broadcast_to(v, (8,) + v.shape))  # noqa: E731  perform_pir's broadcast
+    qb, qbc, gkb = lanes(query_block[2]), (lanes(query_block[3]) if conj else None), lanes(query_block[0:2])
+    parts = [orc.RefK1(sh, qb, qbc, gkb) for sh in shards]
+    sums = orc.RefK1.run(parts[:1], threads=threads)
+    matches = bool(np.array_equal(parts[0].finish(sums[0]),
+                                  shards[0].perform_pir(query_block[2], query_block[3] if conj else None,
+                                                        query_block[0:2])))
+    shard_bytes = 6 * 2048 * lp * 32  # MEM_LOAD_DATA per shard (base_pir.rs:1242)
+    one = min(_timed(lambda: orc.RefK1.run(parts[:1], threads=1, keep=False)) for _ in range(3))
+    fleet = [parts[k % 2] for k in range(16)]
+    many = min(_timed(lambda: orc.RefK1.run(fleet, threads=threads, keep=False)) for _ in range(3)) / len(fleet)
+    # the secondary phase (one query at l_s) from the oracle
+    _, pu, su = orc.double_pir([shards[0]], None, query_block, None, ls)
+    ms = nshards * many * 1e3 + su / 1e3
+    ms1 = nshards * one * 1e3
     return {
         "value": round(ms, 1), "unit": "ms/query", "cores": threads, "kind": "port",
-        "sample": (f"oracle (C, OpenMP) on this host: primary phase over 2 full shards "
-                   f"({per_shard_ms:.1f} ms/shard, x{nshards}) + one secondary phase at l_s={ls} "
-                   f"({su / 1e3:.1f} ms); shard preprocessing {t_pre:.1f} s not counted"),
-        "one_thread": {"value": round(ms1, 1), "unit": "ms/query", "cores": 1,
-                       "sample": (f"primary phase over 1 full shard ({pu1 / 1e3:.1f} ms, x{nshards}) + one "
-                                  f"secondary phase at l_s={ls} ({su1 / 1e3:.1f} ms)")},
+        "isa": orc.cpu_isa(),
+        "matches_oracle": matches,
+        "one_thread_gbs": round(shard_bytes / one / 1e9, 2),
+        "published_one_thread_gbs": PUBLISHED_ONE_THREAD_MBS / 1e3,
+        "primary_ms_per_shard": round(many * 1e3, 2),
+        "secondary_ms": round(su / 1e3, 1),
+        "sample": (f"primary: main_pir_loop in the reference's form ({orc.cpu_isa()}, u64 sums reduced every "
+                   f"{orc.cpu_reduce_every()} rows, {threads} OpenMP threads) over 16 fleet entries cycling 2 "
+                   f"distinct shards of l={lp}: {many * 1e3:.2f} ms per shard, x{nshards}; secondary: one phase at "
+                   f"l_s={ls} from the oracle ({su / 1e3:.1f} ms); preprocessing the 2 shards ({t_pre:.1f} s) "
+                   f"not counted"),
+        "one_thread": {"value": round(ms1, 1), "unit": "ms/query (primary phase)", "cores": 1,
+                       "sample": (f"one shard on one thread: {one * 1e3:.1f} ms = {shard_bytes / one / 1e9:.2f} GB/s "
+                                  f"(the reference: {PUBLISHED_ONE_THREAD_MBS} MB/s), x{nshards}")},
     }
+
+
+def _timed(fn):
+    t = time.perf_counter()
+    fn()
+    return time.perf_counter() - t
 
 
 # bench_wrapped_pir's literal (primary index, shard) pairs, pir_wrapper.rs:562

@@ -863,6 +863,67 @@ int orc_shard_perform_pir_split(const orc_shard* s, const uint64_t* qb, const ui
     return 0;
 }
 
+/* The operands of main_pir_loop (base_pir.rs:1085-1247) in the reference's
+ * memory layout, for the CPU-baseline port (cpu_baseline.c): each slot holds
+ * its 8 lanes side by side (CompressedZnx8El, 32 B). arrays: [r][st][IPGK0,
+ * IPGK1, TE][row][slot][lane] u32; vecs: [r][st][gk0, gk1, query][slot][lane]
+ * u32 (the conjugate query for st = 1) in the NTT domain; gal: the Galois
+ * lookup table (base_pir.rs:974-978). Query k goes to lane k, as in
+ * perform_pir_split. */
+int orc_shard_k1_operands(const orc_shard* s, const uint64_t* qb, const uint64_t* qbc, const uint64_t* gkb,
+                          uint32_t* arrays, uint32_t* vecs, int* gal) {
+    const int N = s->N, L = s->L, nst = s->conj ? 2 : 1;
+    if (s->conj && !qbc) return -2;
+    const int amap[3] = {ARR_IP0, ARR_IP1, ARR_TE};
+    for (int r = 0; r < 2; ++r)
+        for (int st = 0; st < nst; ++st)
+            for (int a = 0; a < 3; ++a) {
+                uint32_t* dst = arrays + ((size_t)(r * nst + st) * 3 + a) * L * N * LANES;
+#pragma omp parallel for
+                for (int row = 0; row < L; ++row)
+                    for (int lane = 0; lane < LANES; ++lane) {
+                        const uint32_t* src = row_ptr(s, amap[a], r, st, row, lane);
+                        for (int j = 0; j < N; ++j) dst[((size_t)row * N + j) * LANES + lane] = src[j];
+                    }
+            }
+    uint64_t* t = (uint64_t*)malloc(sizeof(uint64_t) * N);
+    for (int r = 0; r < 2; ++r)
+        for (int st = 0; st < nst; ++st)
+            for (int v = 0; v < 3; ++v)
+                for (int lane = 0; lane < LANES; ++lane) {
+                    const uint64_t* x = v < 2 ? gkb + ((size_t)lane * 2 + v) * N
+                                              : (st ? qbc : qb) + (size_t)lane * N;
+                    to_ntt(&s->R, r, x, t);
+                    uint32_t* dst = vecs + ((size_t)(r * nst + st) * 3 + v) * N * LANES;
+                    for (int j = 0; j < N; ++j) dst[(size_t)j * LANES + lane] = (uint32_t)t[j];
+                }
+    free(t);
+    galois_table(N, gal);
+    return 0;
+}
+
+/* the end of perform_pir_split (base_pir.rs:1072-1077) after main_pir_loop:
+ * sums [r][slot][lane] (both streams added, reduced mod p_r) are gathered by
+ * the Galois table and taken out of the NTT domain: out [8][N] mod q */
+int orc_shard_k1_finish(const orc_shard* s, const uint64_t* sums, uint64_t* out) {
+    const int N = s->N;
+    int* gt = (int*)malloc(sizeof(int) * N);
+    galois_table(N, gt);
+    uint64_t* x1 = (uint64_t*)malloc(sizeof(uint64_t) * N);
+    uint64_t* x2 = (uint64_t*)malloc(sizeof(uint64_t) * N);
+    for (int lane = 0; lane < LANES; ++lane) {
+        for (int j = 0; j < N; ++j) {
+            x1[j] = sums[((size_t)0 * N + gt[j]) * LANES + lane] % s->R.M[0].p;
+            x2[j] = sums[((size_t)1 * N + gt[j]) * LANES + lane] % s->R.M[1].p;
+        }
+        from_simd(&s->R, x1, x2, out + (size_t)lane * N);
+    }
+    free(gt);
+    free(x1);
+    free(x2);
+    return 0;
+}
+
 /* base_pir.rs:945-952: one query broadcast to all lanes */
 int orc_shard_perform_pir(const orc_shard* s, const uint64_t* qb, const uint64_t* qbc, const uint64_t* gkb, uint64_t* out) {
     int N = s->N;

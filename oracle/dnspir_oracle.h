@@ -101,6 +101,22 @@ int orc_prepare_query(orc_rng* rng, size_t primary_idx, size_t secondary_idx, in
                       uint8_t* query, size_t* qlen, uint8_t sk_seed[32]);
 int orc_process_reply(const uint8_t* reply, size_t rlen, const uint8_t sk_seed[32], uint16_t* out);
 
+/* ---- CPU baseline in the reference's loop form (cpu_baseline.c) --------- */
+/* main_pir_loop's operands in the reference layout: arrays [r][st][IPGK0,
+ * IPGK1, TE][L][N][8] u32 (2 * nst * 3 * L * N * 8 words), vecs [r][st][gk0,
+ * gk1, query][N][8] u32, gal [N]; query k on lane k (perform_pir_split) */
+int orc_shard_k1_operands(const orc_shard* s, const uint64_t* qb /*[8][N]*/, const uint64_t* qbc,
+                          const uint64_t* gkb /*[8][2][N]*/, uint32_t* arrays, uint32_t* vecs, int* gal);
+/* sums [r][N][8] mod p_r -> Galois gather + iNTT + CRT -> out [8][N] mod q */
+int orc_shard_k1_finish(const orc_shard* s, const uint64_t* sums, uint64_t* out);
+/* n shards' main_pir_loops (lazy u64 sums reduced every orc_cpu_reduce_every()
+ * rows, AVX-512 / AVX2 / scalar) on `threads` threads; sums [n][2][N][8] or NULL */
+int orc_cpu_k1(int n, const uint32_t* const* arrays, int N, int L, int nst, const uint32_t* vecs, const int* gal,
+               int threads, int window /* 0 = the reference's */, uint64_t* sums);
+const char* orc_cpu_isa(void);
+void orc_cpu_force_isa(int isa); /* 512 / 256 / 0 (scalar), -1 = by the CPU */
+int orc_cpu_reduce_every(void);
+
 /* ---- bucket packing (dns/bucket.rs:271-303), N = 2048 only --------------- */
 /* 2560 bucket bytes -> 2048 ten-bit coefficients, and back */
 void orc_pack_bytes_into_coeffs(const uint8_t* bucket, uint16_t* coeffs);

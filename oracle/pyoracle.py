@@ -73,6 +73,13 @@ def lib():
             "orc_choose_index_group_secondary": (ctypes.c_int, [ctypes.c_int, sz]),
             "orc_condense": (sz, [ctypes.c_int, u64p, sz, u8p]),
             "orc_uncondense": (ctypes.c_int, [ctypes.c_int, sz, u8p, sz, ctypes.POINTER(sz), u64p]),
+            "orc_shard_k1_operands": (ctypes.c_int, [vp, u64p, u64p, u64p, u32p, u32p, ctypes.POINTER(ctypes.c_int)]),
+            "orc_shard_k1_finish": (ctypes.c_int, [vp, u64p, u64p]),
+            "orc_cpu_k1": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(u32p), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          u32p, ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int, u64p]),
+            "orc_cpu_isa": (ctypes.c_char_p, []),
+            "orc_cpu_reduce_every": (ctypes.c_int, []),
+            "orc_cpu_force_isa": (None, [ctypes.c_int]),
             "orc_pack_bytes_into_coeffs": (None, [u8p, u16p]),
             "orc_unpack_coeffs_into_bytes": (None, [u16p, u8p]),
             "orc_database_shape": (None, [sz, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
@@ -385,3 +392,58 @@ def unpack_coeffs_into_bytes(coeffs):
     for i in range(c.shape[0]):
         lib().orc_unpack_coeffs_into_bytes(ptr(c[i], u16p), ptr(out[i], u8p))
     return out
+
+
+class RefK1:
+    """main_pir_loop in the reference's own form (oracle/cpu_baseline.c), for
+    the CPU baseline: a shard's arrays and a query's vectors exported once in
+    the reference layout, then the timed loop."""
+
+    def __init__(self, shard, qb, qbc, gkb):
+        """qb, qbc: [8][N] (qbc None without a conjugate), gkb: [8][2][N]; query k on lane k"""
+        self.shard = shard
+        N, L, nst = shard.N, shard.L, (2 if shard.conj else 1)
+        self.dims = (N, L, nst)
+        self.arrays = np.zeros(2 * nst * 3 * L * N * LANES, np.uint32)
+        self.vecs = np.zeros(2 * nst * 3 * N * LANES, np.uint32)
+        self.gal = np.zeros(N, np.int32)
+        qb = np.ascontiguousarray(qb, np.uint64)
+        gkb = np.ascontiguousarray(gkb, np.uint64)
+        qc = ptr(np.ascontiguousarray(qbc, np.uint64), u64p) if qbc is not None else None
+        rc = lib().orc_shard_k1_operands(shard.h, ptr(qb, u64p), qc, ptr(gkb, u64p), ptr(self.arrays, u32p),
+                                         ptr(self.vecs, u32p), self.gal.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+        if rc:
+            raise ValueError(f"k1_operands failed: {rc}")
+
+    @staticmethod
+    def run(parts, threads=1, window=0, keep=True):
+        """the loops of every shard in `parts` (RefK1 objects of one query) ->
+        sums [n][2][N][8] (None if keep is False)"""
+        N, L, nst = parts[0].dims
+        arr = (u32p * len(parts))(*[ptr(p.arrays, u32p) for p in parts])
+        sums = np.zeros((len(parts), 2, N, LANES), np.uint64) if keep else None
+        rc = lib().orc_cpu_k1(len(parts), arr, N, L, nst, ptr(parts[0].vecs, u32p),
+                              parts[0].gal.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), threads, window,
+                              ptr(sums, u64p) if keep else None)
+        if rc:
+            raise MemoryError("orc_cpu_k1")
+        return sums
+
+    def finish(self, sums):
+        """sums [2][N][8] -> perform_pir_split's output [8][N] mod q"""
+        s = np.ascontiguousarray(sums, np.uint64)
+        out = np.zeros((LANES, self.dims[0]), np.uint64)
+        lib().orc_shard_k1_finish(self.shard.h, ptr(s, u64p), ptr(out, u64p))
+        return out
+
+
+def cpu_isa():
+    return lib().orc_cpu_isa().decode()
+
+
+def cpu_reduce_every():
+    return lib().orc_cpu_reduce_every()
+
+
+def cpu_force_isa(isa):
+    lib().orc_cpu_force_isa(isa)

@@ -65,7 +65,7 @@ def read_traffic(workload_tag):
 PUBLISHED_ONE_THREAD_MBS = 15341  # main_pir_loop, one thread, Xeon Gold 5318S (ARTIFACT-APPENDIX.md:151)
 
 
-def cpu_baseline(lp, nshards, ls, query_block, threads):
+def cpu_baseline(lp, nshards, ls, query_block, threads, slots=1):
     """The reference's primary loop timed on this host's cores (kind "port"):
     main_pir_loop in its own form (oracle/cpu_baseline.c: lazy u64 sums
     reduced every 341 rows, AVX-512 / AVX2, OpenMP over the (shard, modulus,
@@ -99,7 +99,7 @@ def cpu_baseline(lp, nshards, ls, query_block, threads):
     many = min(_timed(lambda: orc.RefK1.run(fleet, threads=threads, keep=False)) for _ in range(3)) / len(fleet)
     # the secondary phase (one query at l_s) from the oracle
     _, pu, su = orc.double_pir([shards[0]], None, query_block, None, ls)
-    ms = nshards * many * 1e3 + su / 1e3
+    ms = nshards * many * 1e3 + slots * su / 1e3  # one secondary phase per batch slot
     ms1 = nshards * one * 1e3
     return {
         "value": round(ms, 1), "unit": "ms/query", "cores": threads, "kind": "port",
@@ -406,10 +406,10 @@ def serving(pir, dbs, rates, seconds, n_clients, args):
     share /= share.sum()
     sample = {"n": 0, "ok": True}
 
-    def new_server(window):
+    def new_server(window, budget_gib=None):
         srv = pir.Server(slots=args.serve_slots, max_batch=args.serve_max_batch,
-                         scratch_budget=args.serve_budget_gib << 30, key_cache_entries=args.serve_key_cache,
-                         batch_window=window)
+                         scratch_budget=(budget_gib or args.serve_budget_gib) << 30,
+                         key_cache_entries=args.serve_key_cache, batch_window=window)
         ids = [srv.add_database(d[1]) for d in dbs]
         # warm the key cache, the slots and the pass-time estimates
         warm = [(ids[k % len(ids)], c, 1000.0 * k, clients[c][2][k % len(ids)][2], clients[c][0]) for k, c in
@@ -477,6 +477,51 @@ def serving(pir, dbs, rates, seconds, n_clients, args):
                    "result": run_rate(srv0, ids0, max(rates), np.random.default_rng(2024 + 1))}
         log(f"[bench] serving {max(rates)}/s (no window): {compare['result']}")
         srv0.close()
+
+    def saturate(window):
+        """raise the offered rate by 1.5x from the sweep's top until the
+        server falls behind (achieved < 0.95 x offered): its capacity is the
+        rate it then achieves; p99 is quoted at the last rate it kept up with"""
+        srv_s, ids_s = new_server(window)
+        r_rng = np.random.default_rng(4048 + int(window * 10))
+        rate, kept, steps = max(rates) if rates else 500, None, []
+        for _ in range(args.serve_saturation_steps):
+            rate = int(rate * 1.5)
+            res = run_rate(srv_s, ids_s, rate, r_rng)
+            steps.append({"offered_qps": res["offered_qps"], "achieved_qps": res["achieved_qps"],
+                          "p50_ms": res["p50_ms"], "p99_ms": res["p99_ms"], "mean_batch": res["mean_batch"],
+                          "failed": res["failed"]})
+            log(f"[bench] saturation (window {window}) {rate}/s: {steps[-1]}")
+            if res["achieved_qps"] < 0.95 * res["offered_qps"]:
+                srv_s.close()
+                return {"capacity_qps": res["achieved_qps"], "p99_ms_at_last_kept_rate": kept["p99_ms"] if kept else None,
+                        "last_kept_rate_qps": kept["offered_qps"] if kept else None, "steps": steps}
+            kept = res
+        srv_s.close()
+        return {"capacity_qps": None, "note": f"kept up with every rate up to {rate}/s", "steps": steps}
+
+    saturation = None
+    if args.serve_saturation_steps > 0:
+        saturation = {f"window_{args.serve_window}": saturate(args.serve_window)}
+        if args.serve_window > 0:
+            saturation["window_0"] = saturate(0.0)
+
+    tight = None
+    if args.serve_tight_budget_gib > 0 and rates:
+        # admission by HBM budget: the top sweep rate with a budget that holds
+        # only a few large batches at once
+        srv_t, ids_t = new_server(args.serve_window, args.serve_tight_budget_gib)
+        st_t0 = srv_t.stats()
+        res = run_rate(srv_t, ids_t, max(rates), np.random.default_rng(777))
+        st_t1 = srv_t.stats()
+        tight = {"scratch_budget_gib": args.serve_tight_budget_gib, "rate": max(rates),
+                 "max_batch_per_db": [srv_t.max_batch(i) for i in ids_t],
+                 "admission_waits": int(st_t1["admission_waits"] - st_t0["admission_waits"]),
+                 "scratch_peak_gib": round(st_t1["scratch_in_use_peak"] / 2**30, 2),
+                 "achieved_qps": res["achieved_qps"], "p50_ms": res["p50_ms"], "p99_ms": res["p99_ms"],
+                 "failed": res["failed"], "mean_batch": res["mean_batch"]}
+        log(f"[bench] serving with a {args.serve_tight_budget_gib} GiB budget: {tight}")
+        srv_t.close()
     return {
         "workload": ("Poisson arrivals (seeded) against " +
                      ", ".join(f"{name} ({len(sh)} shards of l={sh[0].group_order}, {res} resident)"
@@ -490,6 +535,8 @@ def serving(pir, dbs, rates, seconds, n_clients, args):
                    "key_evictions": st["key_evictions"]},
         "rates": results,
         "top_rate_without_window": compare,
+        "saturation": saturation,
+        "tight_budget": tight,
         "decrypt_sample": sample,
     }
 
@@ -499,7 +546,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=16)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--entries", type=int, default=1 << 22)
+    ap.add_argument("--entries", type=int, default=1 << 22,
+                    help="database entries, up to 2 * 2048^2 (above 2048^2 the second batch slot takes the rest, "
+                         "as bench_wrapped_pir does, pir_wrapper.rs:501-505, 577-581)")
+    ap.add_argument("--force-full-dbs", action="store_true",
+                    help="full-size primary shards even for small databases (pir_wrapper.rs:486-511)")
     ap.add_argument("--resident", type=int, default=256)
     ap.add_argument("--k1-mode", type=int, default=0, choices=[0, 2, 3],
                     help="single-query inner product: 0 auto, 2 two streams, 3 three streams")
@@ -512,6 +563,10 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serve-rates", default="50,200,500,1000")
+    ap.add_argument("--serve-tight-budget-gib", type=int, default=8,
+                    help="an extra run at the top sweep rate with this scratch budget (admission by HBM budget)")
+    ap.add_argument("--serve-saturation-steps", type=int, default=5,
+                    help="rates above the sweep (x1.5 each) until the server falls behind; 0 = skip")
     ap.add_argument("--serve-seconds", type=float, default=6.0)
     ap.add_argument("--serve-clients", type=int, default=1000)
     ap.add_argument("--serve-small-entries", type=int, default=32768)
@@ -544,11 +599,22 @@ def main():
         return
 
     pir.init(0)
-    lp, ls = pir.get_database_shape(args.entries)
-    nshards = pir.num_databases(args.entries)
+    N = 2048
+    if not 1 <= args.entries <= 2 * N * N:
+        print(json.dumps({"error": f"--entries must lie in 1 .. {2 * N * N}"}))
+        sys.exit(2)
+    # bench_wrapped_pir's shape (pir_wrapper.rs:501-513): that of the first,
+    # fully loaded fleet; --force-full-dbs keeps full-size shards throughout
+    fleet_entries = min(args.entries, N * N)
+    if args.force_full_dbs:
+        lp = N
+        ls = pir.choose_index_group_secondary(-(-fleet_entries // (8 * N)))
+    else:
+        lp, ls = pir.get_database_shape(fleet_entries)
+    nshards = -(-args.entries // (8 * lp))
     resident = min(args.resident, nshards)
     log(f"[bench] {args.entries} entries: {nshards} shards of l={lp} ({8 * lp} entries), "
-        f"{resident} resident, secondary l={ls}")
+        f"{resident} resident, secondary l={ls}" + (" (full size forced)" if args.force_full_dbs else ""))
 
     # resident shards: seeded uniform 2560-byte entries packed into coefficients
     t0 = time.perf_counter()
@@ -556,6 +622,10 @@ def main():
     t_pre = time.perf_counter() - t0
     log(f"[bench] preprocessed {resident} shards in {t_pre:.1f} s")
     fleet = [shards[k % resident] for k in range(nshards)]
+    # one fleet per batch slot, N/8 shards each (pir_wrapper.rs:577-581)
+    per_slot = N // 8
+    fleets = [fleet[i:i + per_slot] for i in range(0, nshards, per_slot)]
+    assert len(fleets) <= 2
 
     # seeded queries, staged in HBM
     idx_rng = np.random.default_rng(0)
@@ -563,17 +633,24 @@ def main():
     qs = []
     for _ in range(args.queries):
         pi = int(idx_rng.integers(0, 8 * lp))
-        si = int(idx_rng.integers(0, nshards))
+        si = int(idx_rng.integers(0, len(fleets[0])))
         q, sk = pir.prepare_query(rng, pi, si, lp, ls)
         qs.append((pi, si, sk, torch.from_numpy(np.frombuffer(q, np.uint8).copy()).cuda()))
     rb = pir.reply_bytes()
     replies = [torch.zeros(rb, dtype=torch.uint8, device="cuda") for _ in qs]
+    reply1 = torch.zeros(rb, dtype=torch.uint8, device="cuda") if len(fleets) == 2 else None
     ctx = pir.Context()
     ctx.set_k1_mode(args.k1_mode)
 
+    def one_query(qd, rep):
+        # the same query in both slots when the database spans two fleets
+        two = len(fleets) == 2
+        return pir.process_queries_device(fleets[0], fleets[1] if two else None, qd, qd if two else None, rep,
+                                          reply1 if two else None, ctx=ctx)
+
     for w in range(args.warmup):
         pi, si, sk, qd = qs[w % len(qs)]
-        pir.process_queries_device(fleet, None, qd, None, replies[w % len(qs)], None, ctx=ctx)
+        one_query(qd, replies[w % len(qs)])
     torch.cuda.synchronize()
     log(f"[bench] warmup done, timing {args.steps} queries")
 
@@ -583,7 +660,7 @@ def main():
     for k in range(args.steps):
         pi, si, sk, qd = qs[k % len(qs)]
         t = time.perf_counter()
-        tm = pir.process_queries_device(fleet, None, qd, None, replies[k % len(qs)], None, ctx=ctx)
+        tm = one_query(qd, replies[k % len(qs)])
         lat.append((time.perf_counter() - t) * 1e3)
         prim.append(tm.primary_us / 1e3)
         sec.append(tm.secondary_us / 1e3)
@@ -631,11 +708,11 @@ def main():
         sshards, sentries = make_shards(pir, slp, sn, 9000)
         dbs = [(f"{args.serve_small_entries} entries", sshards, sentries, sn, 0.2),
                (f"{args.batch_entries} entries", bshards, bentries, len(bshards), 0.3),
-               (f"{args.entries} entries", fleet, entries, resident, 0.5)]
+               (f"{fleet_entries} entries", fleets[0], entries, min(resident, len(fleets[0])), 0.5)]
         serve = serving(pir, dbs, [int(x) for x in args.serve_rates.split(",")], args.serve_seconds,
                         args.serve_clients, args)
         del sshards, sentries
-    del bshards, bentries, shards, fleet, entries
+    del bshards, bentries, shards, fleet, fleets, entries
     torch.cuda.empty_cache()
 
     cpu = None
@@ -650,7 +727,7 @@ def main():
         if ls != 2048:
             block[5] = 0
         log(f"[bench] CPU baseline on {threads} threads")
-        cpu = cpu_baseline(lp, nshards, ls, block, threads)
+        cpu = cpu_baseline(lp, nshards, ls, block, threads, slots=len(fleets))
         cpu["small_configurations"] = cpu_cases(small_keep, threads)
 
     out = {
@@ -673,7 +750,8 @@ def main():
                             else f"{resident} resident reused cyclically") + f", secondary l={ls}, "
                          f"{args.queries} seeded queries"),
             "entries": args.entries, "shards": nshards, "resident_shards": resident,
-            "group_order_primary": lp, "group_order_secondary": ls,
+            "group_order_primary": lp, "group_order_secondary": ls, "force_full_dbs": args.force_full_dbs,
+            "fleets": len(fleets),
         },
         "primary_ms": round(float(np.mean(prim)), 3),
         "secondary_ms": round(float(np.mean(sec)), 3),

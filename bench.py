@@ -478,13 +478,14 @@ def serving(pir, dbs, rates, seconds, n_clients, args):
         log(f"[bench] serving {max(rates)}/s (no window): {compare['result']}")
         srv0.close()
 
-    def saturate(window):
+    def saturate(window, kept):
         """raise the offered rate by 1.5x from the sweep's top until the
         server falls behind (achieved < 0.95 x offered): its capacity is the
-        rate it then achieves; p99 is quoted at the last rate it kept up with"""
+        rate it then achieves; p99 is quoted at the last rate it kept up with
+        (`kept`: the sweep's top rate with this window)"""
         srv_s, ids_s = new_server(window)
         r_rng = np.random.default_rng(4048 + int(window * 10))
-        rate, kept, steps = max(rates) if rates else 500, None, []
+        rate, steps = max(rates) if rates else 500, []
         for _ in range(args.serve_saturation_steps):
             rate = int(rate * 1.5)
             res = run_rate(srv_s, ids_s, rate, r_rng)
@@ -502,9 +503,10 @@ def serving(pir, dbs, rates, seconds, n_clients, args):
 
     saturation = None
     if args.serve_saturation_steps > 0:
-        saturation = {f"window_{args.serve_window}": saturate(args.serve_window)}
+        top = results[str(max(rates))] if rates else None
+        saturation = {f"window_{args.serve_window}": saturate(args.serve_window, top)}
         if args.serve_window > 0:
-            saturation["window_0"] = saturate(0.0)
+            saturation["window_0"] = saturate(0.0, compare["result"] if compare else None)
 
     tight = None
     if args.serve_tight_budget_gib > 0 and rates:
@@ -626,6 +628,7 @@ def main():
     per_slot = N // 8
     fleets = [fleet[i:i + per_slot] for i in range(0, nshards, per_slot)]
     assert len(fleets) <= 2
+    nslots = len(fleets)
 
     # seeded queries, staged in HBM
     idx_rng = np.random.default_rng(0)
@@ -727,7 +730,7 @@ def main():
         if ls != 2048:
             block[5] = 0
         log(f"[bench] CPU baseline on {threads} threads")
-        cpu = cpu_baseline(lp, nshards, ls, block, threads, slots=len(fleets))
+        cpu = cpu_baseline(lp, nshards, ls, block, threads, slots=nslots)
         cpu["small_configurations"] = cpu_cases(small_keep, threads)
 
     out = {
@@ -751,7 +754,7 @@ def main():
                          f"{args.queries} seeded queries"),
             "entries": args.entries, "shards": nshards, "resident_shards": resident,
             "group_order_primary": lp, "group_order_secondary": ls, "force_full_dbs": args.force_full_dbs,
-            "fleets": len(fleets),
+            "fleets": nslots,
         },
         "primary_ms": round(float(np.mean(prim)), 3),
         "secondary_ms": round(float(np.mean(sec)), 3),

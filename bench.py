@@ -683,7 +683,12 @@ def main():
     if not ok:
         log("[bench] DECRYPTION CHECK FAILED")
 
-    small, small_keep = small_cases(pir, shards[0], entries[0])
+    if lp == N:
+        small, small_keep = small_cases(pir, shards[0], entries[0])
+    else:  # the one-full-shard case needs a shard of order N
+        fs, fe = make_shards(pir, N, 1, 7777)
+        small, small_keep = small_cases(pir, fs[0], fe[0])
+        del fs, fe
     log(f"[bench] small configurations: {small}")
 
     k1_ms = float(np.mean(k1))

@@ -1317,6 +1317,7 @@ static const K1bVariant k_k1b_variants[] = {
     {"no LDS, prefetch 3/3", 1, k1b_direct_kernel<1, 3, 3, 6>},
     {"no LDS, prefetch 2/2", 2, k1b_direct_kernel<2, 2, 2, 2>},
     {"no LDS, prefetch 3/2", 2, k1b_direct_kernel<2, 3, 2, 6>},
+    {"no LDS, prefetch 4/3", 2, k1b_direct_kernel<2, 4, 3, 12>},
 };
 constexpr int K1B_NVAR = sizeof(k_k1b_variants) / sizeof(k_k1b_variants[0]);
 // chosen variant per (column tiles, streams); -1 = default (register ring, prefetch 4)

@@ -68,7 +68,8 @@ class ServerStats(ctypes.Structure):
 
 
 class CtxStats(ctypes.Structure):
-    _fields_ = [("pair_runs", ctypes.c_uint64), ("pair_refused", ctypes.c_uint64), ("pair_retries", ctypes.c_uint64)]
+    _fields_ = [("pair_runs", ctypes.c_uint64), ("pair_refused", ctypes.c_uint64), ("pair_retries", ctypes.c_uint64),
+                ("quad_runs", ctypes.c_uint64)]
 
 
 class FrontendConfig(ctypes.Structure):
@@ -128,6 +129,7 @@ SIGNATURES = {
     "dnspir_ctx_stream": (vp, [vp]),
     "dnspir_ctx_set_graphs": (ci, [vp, ci]),
     "dnspir_ctx_set_exchange_polls": (ci, [vp, ci]),
+    "dnspir_ctx_set_chain_cus": (ci, [vp, ci]),
     "dnspir_ctx_set_k1_mode": (ci, [vp, ci]),
     "dnspir_ctx_get_stats": (ci, [vp, ctypes.POINTER(CtxStats)]),
     "dnspir_double_pir": (ci, [vp, vp, sz, vp, sz, u64p, u64p, u64p, u64p, ctypes.POINTER(Timing)]),

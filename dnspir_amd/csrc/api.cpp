@@ -153,6 +153,10 @@ int build_tables(DevTables* T) {
         fi = fi * inv5 % M2N;
     }
     for (int s = 0; s < N; ++s) T->sigma5[s] = T->exp_slot[(5 * T->slot_exp[s]) % M2N];
+    // the four-CU chain keeps each half of the slots on its own blocks: sigma_5
+    // must map each half to itself (the halves are exponents 1 and 3 mod 4)
+    for (int s = 0; s < N; ++s)
+        if ((T->sigma5[s] < HALFN) != (s < HALFN) || (T->slot_exp[s] & 3) != (s < HALFN ? 1 : 3)) return DNSPIR_E_HIP;
     for (int gp = 0; gp < N; ++gp) {
         int e = gp < HALFN ? T->pow5[gp] : (M2N - T->pow5[gp - HALFN]) % M2N;
         T->gpos_slot[gp] = T->exp_slot[e];
@@ -359,17 +363,16 @@ static int copy_pair_flag(dnspir_ctx* c) {
     return 0;
 }
 
-// ---- the device's budget of two-CU chain blocks -------------------------------
-// Each block of chain_pair_kernel spins on its partner block, so a wait can
-// end only once the partner is resident. One such block takes 85 KB of LDS,
-// so at most one fits a CU (160 KB). If the pair blocks in flight on the
-// device never outnumber its CUs, every waiting block's partner finds a CU as
-// the blocks of other kernels (which never wait) retire. g.pair_used counts
-// the pair blocks reserved by every context of the process; a call that
-// would exceed the CU count runs its chains on one CU each instead.
+// ---- the device's budget of multi-CU chain blocks -----------------------------
+// Each block of chain_pair_kernel / chain_quad_kernel spins on its partner
+// blocks, so a wait can end only once the partners are resident. One such
+// block takes more than 80 KB of LDS, so at most one fits a CU (160 KB). If
+// the multi-CU chain blocks in flight on the device never outnumber its CUs,
+// every waiting block's partners find a CU as the blocks of other kernels
+// (which never wait) retire. g.pair_used counts the blocks reserved by every
+// context of the process; a call that would exceed the CU count runs its
+// chains on fewer CUs each (four -> two -> one).
 namespace {
-
-int pair_blocks(int nchains) { return 16 * ((nchains + 7) / 8); }  // launch_chain_pair's grid
 
 void pair_release(dnspir_ctx* c) {
     if (!c->pair_held) return;
@@ -385,19 +388,28 @@ void pair_settle(dnspir_ctx* c) {
     pair_release(c);
 }
 
-// reserves the blocks of a two-CU chain launch for nchains chains
-bool pair_acquire(dnspir_ctx* c, int nchains) {
+// CUs per chain for nchains chains on this context: 4 up to QUAD_MAX_CHAINS
+// chains, 2 up to PAIR_MAX_CHAINS, else 1, capped by the context's setting;
+// reserves the blocks of a multi-CU launch (the result falls to fewer CUs
+// when the device's budget has no room)
+int chain_acquire(dnspir_ctx* c, int nchains) {
     pair_settle(c);
-    const int b = pair_blocks(nchains);
+    int want = nchains <= QUAD_MAX_CHAINS ? 4 : nchains <= PAIR_MAX_CHAINS ? 2 : 1;
+    if (c->chain_cus) want = std::min(want, c->chain_cus);
+    if (want == 1) return 1;
     std::lock_guard<std::mutex> lk(g.pair_mu);
-    if (g.pair_used + b > g.pair_cap) {
-        ++c->stats.pair_refused;
-        return false;
+    for (int cus = want; cus > 1; cus /= 2) {
+        const int b = chain_multi_blocks(nchains, cus == 4);
+        if (g.pair_used + b <= g.pair_cap) {
+            g.pair_used += b;
+            c->pair_held = b;
+            ++c->stats.pair_runs;
+            if (cus == 4) ++c->stats.quad_runs;
+            return cus;
+        }
     }
-    g.pair_used += b;
-    c->pair_held = b;
-    ++c->stats.pair_runs;
-    return true;
+    ++c->stats.pair_refused;
+    return 1;
 }
 
 // records a reply encoding that follows the secondary phase (repeated by a
@@ -427,13 +439,13 @@ void note_secondary(dnspir_ctx* c, const uint16_t* entries, long long col_stride
 
 // Secondary phase for `nlanes` lanes whose entries are described by the lane
 // lists; the reply (a, b per lane) lands in c->acoef/bcoef/a17o/b12o.
-// pair: run the chains on two CUs each (chain_pair_kernel, lower latency for a
-// few chains; the caller holds their pair_acquire reservation, queues
-// copy_pair_flag and calls check_pair after its sync). The arguments are kept
-// in c->last_sec so that check_pair can run the phase again on one CU per
-// chain.
+// cus: CUs per chain. 2 or 4: chain_pair_kernel / chain_quad_kernel, lower
+// latency for a few chains; the caller holds their chain_acquire reservation,
+// queues copy_pair_flag and calls check_pair after its sync. The arguments
+// are kept in c->last_sec so that check_pair can run the phase again on one
+// CU per chain.
 int run_secondary(dnspir_ctx* c, const uint16_t* entries, long long col_stride, const long long* d_off, const int* d_ncols,
-                  const int* d_slot, int nlanes, int l_s, bool pair = false) {
+                  const int* d_slot, int nlanes, int l_s, int cus = 1) {
     const int L_s = loop_len(l_s), nst_s = nstreams(l_s);
     const hipStream_t s = c->stream;
     int rc;
@@ -447,13 +459,13 @@ int run_secondary(dnspir_ctx* c, const uint16_t* entries, long long col_stride, 
     if ((rc = c->b12o.ensure((size_t)nlanes * N * 4))) return rc;
     note_secondary(c, entries, col_stride, d_off, d_ncols, d_slot, nlanes, l_s);
     launch_k5(entries, d_off, col_stride, c->M.as<uint32_t>(), c->te.as<uint32_t>(), l_s, d_ncols, nlanes, g.tables, s);
-    if (pair) {
+    if (cus > 1) {
         // xch: the timeout flag in word 0, the exchange words from word 64
         const size_t words = chain_pair_xbuf_words(nst_s * nlanes);
         if ((rc = c->xch.ensure((words + 64) * 4))) return rc;
         launch_chain_pair(c->te.as<uint32_t>(), c->aout.as<uint32_t>(), c->cout.as<uint32_t>(), c->qs.as<uint32_t>(), d_slot,
                           L_s, nst_s, nlanes, c->xch.as<uint32_t>() + 64, c->xch.as<unsigned>(), c->exchange_polls,
-                          g.tables, s);
+                          g.tables, s, cus == 4);
     } else {
         launch_chain_sec(c->te.as<uint32_t>(), c->aout.as<uint32_t>(), c->cout.as<uint32_t>(), c->qs.as<uint32_t>(), d_slot,
                          L_s, nst_s, nlanes, g.tables, s);
@@ -478,7 +490,7 @@ int dne::check_pair(dnspir_ctx* c, bool* retried) {
     if (!timed_out) return 0;
     ++c->stats.pair_retries;
     const dnspir_ctx::SecondaryCall sc = c->last_sec;
-    int rc = run_secondary(c, sc.entries, sc.col_stride, sc.d_off, sc.d_ncols, sc.d_slot, sc.nlanes, sc.l_s, false);
+    int rc = run_secondary(c, sc.entries, sc.col_stride, sc.d_off, sc.d_ncols, sc.d_slot, sc.nlanes, sc.l_s, 1);
     if (rc) return rc;
     for (int i = 0; i < sc.n_enc; ++i)
         launch_encode_reply(c->a17o.as<uint32_t>(), c->b12o.as<uint32_t>(), sc.enc_lane0[i], sc.enc_nq[i], sc.enc_reply[i],
@@ -556,11 +568,11 @@ int run_double(dnspir_ctx* c, const dnspir_shard* const* fleets[2], const size_t
         ref_streamed += (uint64_t)len * ref_slab_bytes(l_p);
     }
     HIP_OK(hipEventRecord(c->ev[1], s));
-    const bool pair = pair_acquire(c, nstreams(l_s) * (int)off.size());
+    const int cus = chain_acquire(c, nstreams(l_s) * (int)off.size());
     if ((rc = run_secondary(c, c->se.as<uint16_t>(), (long long)LANES * N, d_off, d_ncols, d_slot, (int)off.size(), l_s,
-                            pair)))
+                            cus)))
         return rc;
-    if (pair && (rc = copy_pair_flag(c))) return rc;
+    if (cus > 1 && (rc = copy_pair_flag(c))) return rc;
     HIP_OK(hipEventRecord(c->ev[2], s));
     if (tm) {
         tm->bytes_streamed = streamed;
@@ -1040,6 +1052,14 @@ int dnspir_ctx_set_exchange_polls(dnspir_ctx* c, int max_polls) {
     return 0;
 }
 
+int dnspir_ctx_set_chain_cus(dnspir_ctx* c, int cus) {
+    if (!c || (cus != 0 && cus != 1 && cus != 2 && cus != 4)) return DNSPIR_E_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->drop_graphs();  // the kernel is baked into captured launches
+    c->chain_cus = cus;
+    return 0;
+}
+
 int dnspir_ctx_get_stats(dnspir_ctx* c, dnspir_ctx_stats* out) {
     if (!out) return DNSPIR_E_ARG;
     if (!c) {
@@ -1405,7 +1425,8 @@ int dne::batch_impl(dnspir_ctx* c, const dnspir_shard* const* fleet, size_t len,
     if ((rc = upload_lanes(c, loff, ncols, slot, &d_off, &d_ncols, &d_slot))) return rc;
     // the pair form only while the device's pair budget has room (taken last,
     // after every early return above)
-    const bool pair = nchains <= PAIR_MAX_CHAINS && pair_acquire(c, nchains);
+    const int cus = chain_acquire(c, nchains);
+    const bool pair = cus > 1;
     const bool use_keys = keys && (keys->src || keys->dst);
     if (use_keys) {
         // cached keys replace the decoded ones; fresh ones are stored
@@ -1448,7 +1469,7 @@ int dne::batch_impl(dnspir_ctx* c, const dnspir_shard* const* fleet, size_t len,
     };
     auto enqueue_secondary = [&]() -> int {
         int r2;
-        if ((r2 = run_secondary(c, c->se.as<uint16_t>(), 3LL * N, d_off, d_ncols, d_slot, 3 * nq, l_s, pair))) return r2;
+        if ((r2 = run_secondary(c, c->se.as<uint16_t>(), 3LL * N, d_off, d_ncols, d_slot, 3 * nq, l_s, cus))) return r2;
         launch_encode_reply(c->a17o.as<uint32_t>(), c->b12o.as<uint32_t>(), 0, nq, rdev, s);
         note_encode(c, 0, nq, rdev);
         return 0;
@@ -1461,7 +1482,7 @@ int dne::batch_impl(dnspir_ctx* c, const dnspir_shard* const* fleet, size_t len,
                                       (uintptr_t)use_keys, (uintptr_t)c->keyp.p, (uintptr_t)c->ptrs.p, (uintptr_t)c->idx.p,
                                       (uintptr_t)k1b_get_choice(1, nstreams(l_p)), (uintptr_t)k1b_get_choice(2, nstreams(l_p)),
                                       (uintptr_t)c->xch.p, (uintptr_t)c->h_err, (uintptr_t)use_k1w(c, l_p, len, nq),
-                                      (uintptr_t)pair, (uintptr_t)c->exchange_polls};
+                                      (uintptr_t)cus, (uintptr_t)c->exchange_polls};
         for (auto* b : c->scratch_bufs) key.push_back((uintptr_t)b->p);
         if (!c->graph_exec[0] || key != c->graph_key) {
             c->drop_graphs();

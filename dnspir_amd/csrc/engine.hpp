@@ -136,10 +136,11 @@ struct dnspir_ctx {
     // two-stream inner product of single queries: the C table and the weight tiles
     dne::DevBuf wscan, wtile;
     int k1_mode = 0;  // 0: two streams from K1W_MIN_SHARDS shards on, 2: always, 3: never
+    int chain_cus = 0;  // most CUs per secondary chain (0: by chain count; 1, 2 or 4)
     unsigned* h_err = nullptr;  // pinned copy of the chain timeout flag
     int exchange_polls = 1 << 21;  // bound of each exchange wait of the two-CU chain
     bool pair_pending = false;  // h_err is to be checked after the next sync
-    int pair_held = 0;          // two-CU chain blocks this context has reserved on the device
+    int pair_held = 0;          // multi-CU chain blocks this context has reserved on the device
     // the last secondary phase's arguments and the reply encodings that
     // followed it (repeated with one CU per chain when a partner wait timed out)
     struct SecondaryCall {
@@ -210,8 +211,9 @@ namespace dne {
 
 int ctx_init(dnspir_ctx* c, hipStream_t s);
 // chains at most this many run on two CUs each (chain_pair_kernel): up to one
-// block per CU of the 256
+// block per CU of the 256; at most QUAD_MAX_CHAINS on four (chain_quad_kernel)
 constexpr int PAIR_MAX_CHAINS = 128;
+constexpr int QUAD_MAX_CHAINS = 16;
 // fleets of at least this many limb-block shards take the two-stream inner
 // product for single queries (below, preparing the weights costs about what
 // the third stream would)

@@ -1839,18 +1839,260 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
     }
 }
 
+// The secondary chain on four CUs per chain (the lowest latency for a few
+// chains): block (chain, r, h) keeps a and c mod p_r on the slots of half h.
+// sigma_5 maps each half of the slots to itself (slot s holds f(psi^e), the
+// halves are e = 1 and e = 3 mod 4 -- the two factors X^(N/2) -+ psi^(N/2) of
+// the first NTT stage -- and 5e = e mod 4), and below its first stage the
+// negacyclic transform splits into two independent half transforms. So each
+// step a block gathers sigma_5(a) on its half and runs the inverse transform
+// down to stage 1 (1024 points), all four blocks of the chain swap those 1024
+// words, and every block finishes the last inverse stage of both moduli for
+// all 2048 coefficients, takes their CRT and top gadget digit D1 (the work
+// the two-CU chain also repeats in both blocks), applies the first forward
+// stage for its half and transforms it mod p_r (stages 1..10, 1024 points),
+// then updates a and c on its half. The arithmetic is chain_pair_kernel's,
+// stage by stage, so the outputs are identical; each block runs half the
+// transform work of that kernel and the same one exchange per step. The
+// blocks of a chain are b, b + 8, b + 16, b + 24 (one XCD under round-robin
+// dispatch); tags, bounded waits and the timeout flag work as in
+// chain_pair_kernel. LDS above 80 KB keeps one block per CU.
+constexpr int QUAD_NT = 512;
+__device__ __forceinline__ void c_update_half(const uint32_t* __restrict__ c_in, uint32_t* __restrict__ c_out,
+                                              const uint32_t* d1t, int off, const uint32_t sga[2], const uint32_t te[2],
+                                              const uint32_t cb0[2], const uint32_t cb1[2], const uint32_t cqb[2],
+                                              const int sig[2], uint32_t p, int K, int tid) {
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int sl = tid + QUAD_NT * k;
+        const uint32_t d1 = d1t[lx<true>(off + sl)];  // [0, 4p)
+        const uint64_t cv = (uint64_t)c_in[sig[k]] + (uint64_t)sga[k] * cb0[k] + (uint64_t)d1 * cb1[k] + (uint64_t)te[k] * cqb[k];
+        c_out[sl] = red64(cv, p, K);
+    }
+}
+
+// D1 mod p of the coefficient whose residues are x1 mod p1, x2 mod p2 (both
+// in [0, 2p), N times the coefficient): base_pir.rs:544-570
+__device__ __forceinline__ uint32_t digit1_mod(uint32_t x1, uint32_t x2, uint32_t p) {
+    const uint32_t e1 = mul_shoup(x1, NINV_Q2INV_P1, NINV_Q2INV_P1_SH, P1);
+    const uint32_t e2 = mul_shoup(x2, NINV_Q1INV_P2, NINV_Q1INV_P2_SH, P2);
+    uint64_t v = (uint64_t)e1 * P2 + (uint64_t)e2 * P1;
+    v = v >= Q ? v - Q : v;
+    const int64_t x = center_q(v);
+    // |x| < q/2 < 2^50: the second balanced digit is hi itself (the third is 0)
+    const int32_t d1 = (int32_t)((x + (1ll << (LOG2_B - 1))) >> LOG2_B);  // |d1| <= 2^24
+    const uint32_t t = (uint32_t)(d1 + (int32_t)p);
+    return t >= p ? t - p : t;
+}
+
+__global__ __launch_bounds__(QUAD_NT) void chain_quad_kernel(const uint32_t* __restrict__ te_tmp, uint32_t* __restrict__ a_out,
+                                                             uint32_t* __restrict__ c_out, const uint32_t* __restrict__ qs,
+                                                             const int* __restrict__ lane_slot, int L, int nlanes, int nchains,
+                                                             uint32_t* xbuf, unsigned* err, int spin_limit,
+                                                             const DevTables* __restrict__ T) {
+    __shared__ __attribute__((aligned(16))) uint2 tw2[2][N];  // [fwd/inv][k] = (twiddle, Shoup companion), modulus r
+    __shared__ uint32_t sa[HALFN];     // a on this half's slots
+    __shared__ uint32_t sc[2][HALFN];  // c, [buffer][slot]
+    __shared__ uint32_t xa[NPAD];      // inverse transform of this half (elements of the half only)
+    __shared__ uint32_t xd[4][NPAD];   // forward transform of D1 on this half in xd[0]; xd[1..3] only keep LDS > 80 KB
+    const int b = blockIdx.x;
+    const int chain = (b >> 5) * 8 + (b & 7), role = (b >> 3) & 3;
+    const int r = role & 1, h = role >> 1;
+    if (chain >= nchains) return;
+    const int st = chain / nlanes, li = chain % nlanes;
+    const int tid = threadIdx.x;
+    const uint32_t p = mod_p(r);
+    const int K = mod_k(r);
+    const int off = h * HALFN;  // first slot (and transform element) of the half
+    for (int k = tid; k < N; k += QUAD_NT) {
+        tw2[0][k] = make_uint2(T->psi[r][k], T->psiq[r][k]);
+        tw2[1][k] = make_uint2(T->ipsi[r][k], T->ipsiq[r][k]);
+    }
+    for (int k = tid; k < HALFN; k += QUAD_NT) {
+        sa[k] = 0;
+        sc[0][k] = 0;
+    }
+    // stage 0: the last inverse stage of both moduli, the first forward stage mod p_r
+    const uint32_t wi[2] = {T->ipsi[0][1], T->ipsi[1][1]}, wiq[2] = {T->ipsiq[0][1], T->ipsiq[1][1]};
+    const uint32_t wf = T->psi[r][1], wfq = T->psiq[r][1];
+    // per-thread constants of the update phase: local slots tid, tid + 512
+    uint32_t cg0[2], cg1[2], cqa[2], cb0[2], cb1[2], cqb[2];
+    int sig_upd[2];
+    const uint32_t* qa = T->qa[2 + st][0];
+    const uint32_t* qv = qs + (size_t)lane_slot[li] * 6 * 2 * N;  // [v][r][slot]
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int s = off + tid + QUAD_NT * k;
+        cg0[k] = T->gka[0][r][s];
+        cg1[k] = T->gka[1][r][s];
+        cqa[k] = qa[r * N + s];
+        cb0[k] = qv[(0 * 2 + r) * N + s];
+        cb1[k] = qv[(1 * 2 + r) * N + s];
+        cqb[k] = qv[((4 + st) * 2 + r) * N + s];
+        sig_upd[k] = T->sigma5[s] - off;  // within the half
+        const uint64_t t26 = (1ull << LOG2_B) % p;
+        cg1[k] = (uint32_t)(((uint64_t)cg1[k] + p - t26 * cg0[k] % p) % p);
+        cb1[k] = (uint32_t)(((uint64_t)cb1[k] + p - t26 * cb0[k] % p) % p);
+    }
+    // the transform passes: threads 0..255, radix-4 group g = 256 h + tid of the
+    // full-length passes (the groups of half h)
+    const int gq = off / 4 + (tid & 255);
+    int sigA[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) sigA[c] = T->sigma5[4 * gq + c] - off;
+    lds_barrier();
+
+    uint32_t* xo = xbuf + (size_t)chain * 4 * N;  // [parity][role][HALFN]
+    const uint32_t* te_base = te_tmp + ((size_t)(st * nlanes + li) * L) * 2 * N;
+    int cur = 0;
+    uint32_t te[2] = {0, 0};
+    uint32_t sga[2] = {0, 0};
+#pragma unroll 1
+    for (int i = 0; i < L; ++i) {
+        uint32_t tn[2];
+        {
+            const uint32_t* te_row = te_base + (size_t)i * 2 * N + (size_t)r * N + off;
+#pragma unroll
+            for (int k = 0; k < 2; ++k) tn[k] = te_row[tid + QUAD_NT * k];
+        }
+        // ---- A: sigma_5 gather and inverse stages 10..1 on this half ----
+        if (tid < 256) {
+            uint32_t v0 = sa[sigA[0]], v1 = sa[sigA[1]], v2 = sa[sigA[2]], v3 = sa[sigA[3]];
+            const uint4 wb = *reinterpret_cast<const uint4*>(&tw2[1][1024 + 2 * gq]);
+            bf_gs(v0, v1, wb.x, wb.y, p);
+            bf_gs(v2, v3, wb.z, wb.w, p);
+            const uint2 wa = tw2[1][512 + gq];
+            bf_gs(v0, v2, wa.x, wa.y, p);
+            bf_gs(v1, v3, wa.x, wa.y, p);
+            const int b4 = lx<true>(4 * gq);
+            xa[b4] = v0; xa[b4 + 1] = v1; xa[b4 + 2] = v2; xa[b4 + 3] = v3;
+            wave_lds_sync();
+            inv_pass4gi<true>(xa, tw2[1], p, 7, gq);
+            wave_lds_sync();
+            inv_pass4gi<true>(xa, tw2[1], p, 5, gq);
+            wave_lds_sync();
+            inv_pass4gi<true>(xa, tw2[1], p, 3, gq);
+        }
+        lds_barrier();
+        if (tid < 256) inv_pass4gi<true>(xa, tw2[1], p, 1, gq);
+        lds_barrier();
+        // ---- exchange: the four halves of stage 1's output ----
+        const uint32_t tag = (uint32_t)(i % 31 + 1) << 27;
+        uint32_t* base = xo + (size_t)(i & 1) * 4 * HALFN;
+        uint32_t own[2];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int j = tid + QUAD_NT * k;
+            own[k] = xa[lx<true>(off + j)];  // [0, 2p)
+            __hip_atomic_store(base + role * HALFN + j, tag | own[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (i > 0) {
+            c_update_half(sc[cur], sc[cur ^ 1], xd[0], off, sga, te, cb0, cb1, cqb, sig_upd, p, K, tid);
+            cur ^= 1;
+        }
+        uint64_t pa[2];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            te[k] = tn[k];
+            sga[k] = sa[sig_upd[k]];
+            pa[k] = (uint64_t)sga[k] * cg0[k] + (uint64_t)te[k] * cqa[k];
+        }
+        // the other three roles' words for pairs j = tid, tid + 512, in one round trip
+        uint32_t w[2][4];
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                w[k][q] = q == role ? (tag | own[k])
+                                    : __hip_atomic_load(base + q * HALFN + tid + QUAD_NT * k, __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (q == role) continue;
+                uint32_t u = w[k][q];
+                if (spin_limit <= 0) {  // no waiting at all (exercises the timeout path)
+                    atomicOr(err, 1u);
+                    u = tag;
+                }
+                int n = 0;
+                while ((u & 0xF8000000u) != tag) {
+                    if ((++n & 255) == 0 &&
+                        (n >= spin_limit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
+                        atomicOr(err, 1u);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    u = __hip_atomic_load(base + q * HALFN + tid + QUAD_NT * k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                w[k][q] = u;
+            }
+        // ---- B: stage 0 inverse (both moduli), CRT, D1, stage 0 forward (this half) ----
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int j = tid + QUAD_NT * k;
+            // roles: 0 = (p1, half 0), 1 = (p2, half 0), 2 = (p1, half 1), 3 = (p2, half 1)
+            uint32_t x1lo = w[k][0] & 0x07FFFFFFu, x1hi = w[k][2] & 0x07FFFFFFu;
+            uint32_t x2lo = w[k][1] & 0x07FFFFFFu, x2hi = w[k][3] & 0x07FFFFFFu;
+            bf_gs(x1lo, x1hi, wi[0], wiq[0], P1);
+            bf_gs(x2lo, x2hi, wi[1], wiq[1], P2);
+            const uint32_t dlo = digit1_mod(x1lo, x2lo, p), dhi = digit1_mod(x1hi, x2hi, p);
+            const uint32_t t = shoup_lazy(dhi, wf, wfq, p);
+            xd[0][lx<true>(off + j)] = h == 0 ? dlo + t : dlo - t + 2 * p;  // [0, 4p)
+        }
+        lds_barrier();
+        // ---- C: forward stages 1..10 on this half ----
+        if (tid < 256) fwd_pass4gi<true>(xd[0], tw2[0], p, 1, gq);
+        lds_barrier();
+        if (tid < 256) {
+            fwd_pass4gi<true>(xd[0], tw2[0], p, 3, gq);
+            wave_lds_sync();
+            fwd_pass4gi<true>(xd[0], tw2[0], p, 5, gq);
+            wave_lds_sync();
+            fwd_pass4gi<true>(xd[0], tw2[0], p, 7, gq);
+            wave_lds_sync();
+            fwd_pass4gi<true>(xd[0], tw2[0], p, 9, gq);
+        }
+        lds_barrier();
+        // ---- D: update a on this half (c follows in the next step's exchange) ----
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int sl = tid + QUAD_NT * k;
+            const uint32_t d1 = xd[0][lx<true>(off + sl)];  // NTT(D1), [0, 4p)
+            sa[sl] = red64(pa[k] + (uint64_t)d1 * cg1[k], p, K);
+        }
+        lds_barrier();
+    }
+    c_update_half(sc[cur], sc[cur ^ 1], xd[0], off, sga, te, cb0, cb1, cqb, sig_upd, p, K, tid);
+    cur ^= 1;
+    lds_barrier();
+    uint32_t* ao = a_out + ((size_t)chain * 2 + r) * N + off;
+    uint32_t* co = c_out + ((size_t)chain * 2 + r) * N + off;
+    for (int sl = tid; sl < HALFN; sl += QUAD_NT) {
+        ao[sl] = sa[sl];
+        co[sl] = sc[cur][sl];
+    }
+}
+
 // exchange words per chain of chain_pair_kernel
 size_t chain_pair_xbuf_words(int nchains) { return (size_t)nchains * 4 * N; }
 
 void launch_chain_pair(const uint32_t* te_tmp, uint32_t* a_out, uint32_t* c_out, const uint32_t* qs, const int* lane_slot,
                        int L, int nst, int nlanes, uint32_t* xbuf, unsigned* err, int spin_limit, const DevTables* T,
-                       hipStream_t s) {
+                       hipStream_t s, bool quad) {
     const int nchains = nst * nlanes;
     launch_fill_u32(xbuf, chain_pair_xbuf_words(nchains), 0, s);
     launch_fill_u32(err, 1, 0, s);
-    hipLaunchKernelGGL(chain_pair_kernel, dim3(16 * ((nchains + 7) / 8)), dim3(1024), 0, s, te_tmp, a_out, c_out, qs,
-                       lane_slot, L, nlanes, nchains, xbuf, err, spin_limit, T);
+    if (quad)
+        hipLaunchKernelGGL(chain_quad_kernel, dim3(chain_multi_blocks(nchains, true)), dim3(QUAD_NT), 0, s, te_tmp, a_out, c_out,
+                           qs, lane_slot, L, nlanes, nchains, xbuf, err, spin_limit, T);
+    else
+        hipLaunchKernelGGL(chain_pair_kernel, dim3(chain_multi_blocks(nchains, false)), dim3(1024), 0, s, te_tmp, a_out, c_out,
+                           qs, lane_slot, L, nlanes, nchains, xbuf, err, spin_limit, T);
 }
+
+int chain_multi_blocks(int nchains, bool quad) { return (quad ? 32 : 16) * ((nchains + 7) / 8); }
 
 // K1 for one or two queries on the limb-block layout: the same exact limb
 // arithmetic as k1b_kernel, on the vector ALU (v_dot4_i32_i8 over 4 digit

@@ -168,10 +168,15 @@ class Context:
         return rc == 1
 
     def set_exchange_polls(self, max_polls):
-        """Bound of each partner wait of the two-CU secondary chain (0: give
+        """Bound of each partner wait of the multi-CU secondary chain (0: give
         up at once; the call then runs its secondary phase again on one CU
         per chain)."""
         check(lib().dnspir_ctx_set_exchange_polls(self.h, int(max_polls)), "ctx_set_exchange_polls")
+
+    def set_chain_cus(self, cus):
+        """Most CUs per secondary chain: 0 (default) four up to 16 chains, two
+        up to 128, else one; or 1, 2, 4. Same bytes."""
+        check(lib().dnspir_ctx_set_chain_cus(self.h, int(cus)), "ctx_set_chain_cus")
 
     def set_k1_mode(self, mode):
         """Single-query inner product: 3 streams all three shard arrays, 2
@@ -180,9 +185,9 @@ class Context:
         check(lib().dnspir_ctx_set_k1_mode(self.h, int(mode)), "ctx_set_k1_mode")
 
     def stats(self):
-        """counters of this context's secondary phases: two-CU runs, runs on
-        one CU because the device's two-CU budget was in use, retries after a
-        partner wait timed out"""
+        """counters of this context's secondary phases: multi-CU runs (and
+        the four-CU ones among them), runs on one CU because the device's
+        multi-CU budget was in use, retries after a partner wait timed out"""
         st = CtxStats()
         check(lib().dnspir_ctx_get_stats(self.h, ctypes.byref(st)), "ctx_get_stats")
         return {k: getattr(st, k) for k, _ in CtxStats._fields_}

@@ -139,21 +139,28 @@ void* dnspir_ctx_stream(dnspir_ctx* ctx);
  * Returns 1 if capture failed once on this context (it then launches
  * eagerly), else 0. */
 int dnspir_ctx_set_graphs(dnspir_ctx* ctx, int on);
-/* Few-query calls run each secondary chain on two CUs that exchange data
- * every step (while the blocks of all such launches in flight on the device
- * fit its CUs, one block per CU; otherwise on one CU per chain). Each wait
- * for the partner CU ends after `max_polls` polls (default 2^21, about a
- * second); the call then runs its secondary phase again on one CU per chain,
- * which never waits, and still answers exactly. 0 makes every wait give up at
- * once (to exercise that path). No counterpart in the reference. */
+/* Few-query calls run each secondary chain on four CUs (up to 16 chains:
+ * one per modulus and half of the slots) or two CUs (up to 128 chains: one
+ * per modulus) that exchange data every step, while the blocks of all such
+ * launches in flight on the device fit its CUs, one block per CU; otherwise
+ * on fewer CUs per chain. Each wait for a partner CU ends after `max_polls`
+ * polls (default 2^21, about a second); the call then runs its secondary
+ * phase again on one CU per chain, which never waits, and still answers
+ * exactly. 0 makes every wait give up at once (to exercise that path). No
+ * counterpart in the reference. */
 int dnspir_ctx_set_exchange_polls(dnspir_ctx* ctx, int max_polls);
+/* The most CUs per secondary chain on this context: 0 (default) by the
+ * chain count as above, 1, 2 or 4. Every setting gives the same bytes. No
+ * counterpart in the reference. */
+int dnspir_ctx_set_chain_cus(dnspir_ctx* ctx, int cus);
 /* counters of a context's secondary phases (NULL ctx = the default context) */
 typedef struct {
-    uint64_t pair_runs;      /* phases run on two CUs per chain */
+    uint64_t pair_runs;      /* phases run on two or four CUs per chain */
     uint64_t pair_refused;   /* phases run on one CU per chain because the device's
-                                two-CU budget was taken by other calls in flight */
-    uint64_t pair_retries;   /* two-CU phases whose partner wait timed out and that
+                                multi-CU budget was taken by other calls in flight */
+    uint64_t pair_retries;   /* multi-CU phases whose partner wait timed out and that
                                 were run again on one CU per chain */
+    uint64_t quad_runs;      /* the phases of pair_runs run on four CUs per chain */
 } dnspir_ctx_stats;
 int dnspir_ctx_get_stats(dnspir_ctx* ctx, dnspir_ctx_stats* out);
 /* Inner product of single queries on shards with loop length N/2. mode 3:

@@ -225,24 +225,29 @@ def test_error_paths(pir):
     assert ei.value.code == -1
 
 
-def test_chain_exchange_timeout_runs_again_on_one_cu(pir):
-    """The two-CU secondary chain bounds every wait for its partner block: with
-    the bound forced to zero from the host every wait gives up at once (every
-    kernel still completes), the call runs its secondary phase again with one
-    CU per chain and still gives the oracle's bytes; the context counts the
-    retry. With the bound restored the same context answers on two CUs."""
+@pytest.mark.parametrize("cus", [0, 2])
+def test_chain_exchange_timeout_runs_again_on_one_cu(pir, cus):
+    """The multi-CU secondary chain (four CUs per chain by default for a few
+    chains, two when the context caps it) bounds every wait for its partner
+    blocks: with the bound forced to zero from the host every wait gives up at
+    once (every kernel still completes), the call runs its secondary phase
+    again with one CU per chain and still gives the oracle's bytes; the
+    context counts the retry. With the bound restored the same context
+    answers on several CUs."""
     g, o, e = _shard(pir, 1024, 1, 0)
     rng = pir.Rng(bytes([9] * 32))
     q, sk = pir.prepare_query(rng, 123, 0, 1024, 8)
     q2, sk2 = pir.prepare_query(rng, 4095, 0, 1024, 8)
     oreps, _, _ = orc.process_queries([o], None, q, None)
     ctx = pir.Context()
+    ctx.set_chain_cus(cus)
     ctx.set_exchange_polls(0)
-    _say("[parity] chain exchange bound 0: expect a retry on one CU")
+    _say(f"[parity] chain exchange bound 0, chain CUs {cus}: expect a retry on one CU")
     reps, _ = pir.process_queries([g], None, q, None, ctx=ctx)
     assert reps[0] == oreps[0]
     st = ctx.stats()
     assert st["pair_retries"] == 1 and st["pair_runs"] == 1, st
+    assert st["quad_runs"] == (1 if cus == 0 else 0), st
     # a two-query batch (graph captured with the two-CU chain) takes the same retry
     breps, _ = pir.process_batch([g], [q, q2], ctx=ctx)
     assert breps[0] == oreps[0]
@@ -254,6 +259,52 @@ def test_chain_exchange_timeout_runs_again_on_one_cu(pir):
     assert np.array_equal(pir.process_reply(reps[0], sk), e[123])
     st = ctx.stats()
     assert st["pair_retries"] == 2 and st["pair_runs"] == 3, st
+    assert st["quad_runs"] == (3 if cus == 0 else 0), st
+
+
+@pytest.mark.parametrize("nsh,two", [(1, False), (3, True), (17, False), (130, False), (130, True)])
+def test_secondary_chain_on_one_two_four_cus(pir, nsh, two):
+    """The secondary chain on one CU per chain, on two (one per modulus) and
+    on four (one per modulus and half of the slots, which sigma_5 keeps
+    apart) gives the oracle's ring elements at secondary lengths 8, 32, 256
+    and the conjugated 2048 (3 to 12 chains), alone and with a second slot."""
+    g, o, _ = _shard(pir, 1024, 1, 0)
+    g2, o2, _ = _shard(pir, 1024, 2, 0)
+    fg = ([g, g2] * nsh)[:nsh]
+    fo = ([o, o2] * nsh)[:nsh]
+    ls = orc.choose_index_group_secondary(nsh)
+    q0 = _block(80 + nsh, False, ls == N)
+    q1 = _block(90 + nsh, False, ls == N) if two else None
+    f1g, f1o = (fg[:2], fo[:2]) if two else (None, None)
+    oo, _, _ = orc.double_pir(fo, f1o, q0, q1, ls)
+    for cus in (1, 2, 4):
+        ctx = pir.Context()
+        ctx.set_chain_cus(cus)
+        _say(f"[parity] secondary chain on {cus} CU(s), {nsh} shards (l_s={ls}), two slots {two}")
+        go, _ = pir.perform_batched_double_pir(fg, f1g, q0, q1, ctx=ctx)
+        assert np.array_equal(go[0], oo[0]), cus
+        if two:
+            assert np.array_equal(go[1], oo[1]), cus
+        st = ctx.stats()
+        assert st["pair_runs"] == (0 if cus == 1 else 1) and st["quad_runs"] == (1 if cus == 4 else 0), st
+        assert st["pair_retries"] == 0, st
+
+
+def test_batch_on_four_cu_chains_matches_alone(pir):
+    """A batch of five queries (15 chains: four-CU blocks for 16, the last
+    chain's blocks idle) answers every query as process_queries does alone."""
+    g, o, e = _shard(pir, 1024, 1, 0)
+    rng = pir.Rng(bytes([15] * 32))
+    idx = [0, 1, 4097, 8191, 777]
+    qs = [pir.prepare_query(rng, k, 0, 1024, 8) for k in idx]
+    ctx = pir.Context()
+    _say("[parity] batch of 5 on four-CU chains")
+    reps, _ = pir.process_batch([g], [q for q, _ in qs], ctx=ctx)
+    assert ctx.stats()["quad_runs"] == 1
+    for k, (q, sk), rep in zip(idx, qs, reps):
+        oreps, _, _ = orc.process_queries([o], None, q, None)
+        assert rep == oreps[0], k
+        assert np.array_equal(pir.process_reply(rep, sk), e[k])
 
 
 def test_fleet_pointer_list_follows_new_shards(pir):

@@ -1762,31 +1762,29 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
             sga[k] = sa[sig_upd[k]];
             pa[k] = (uint64_t)sga[k] * cg0[k] + (uint64_t)te[k] * cqa[k];
         }
-        // both words in one round trip; only a word not yet tagged is polled again
+        // both words in one round trip; every poll reloads the words not yet tagged together
         uint32_t u2[2];
 #pragma unroll
         for (int k = 0; k < 2; ++k) u2[k] = __hip_atomic_load(theirs + tid + 1024 * k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-#pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            const int j = tid + 1024 * k;
-            uint32_t u = u2[k];
-            if (spin_limit <= 0) {  // no waiting at all (exercises the timeout path)
-                atomicOr(err, 1u);
-                u = tag;
-            }
-            int n = 0;
-            while ((u & 0xF8000000u) != tag) {
-                // give up at the limit, or soon after any wait has (later steps then pass quickly)
-                if ((++n & 255) == 0 &&
-                    (n >= spin_limit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
-                    atomicOr(err, 1u);
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-                u = __hip_atomic_load(theirs + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-            oth[k] = u & 0x07FFFFFFu;
+        if (spin_limit <= 0) {  // no waiting at all (exercises the timeout path)
+            atomicOr(err, 1u);
+            u2[0] = u2[1] = tag;
         }
+        for (int n = 0; (u2[0] & 0xF8000000u) != tag || (u2[1] & 0xF8000000u) != tag;) {
+            // give up at the limit, or soon after any wait has (later steps then pass quickly)
+            if ((++n & 255) == 0 &&
+                (n >= spin_limit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
+                atomicOr(err, 1u);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+            for (int k = 0; k < 2; ++k)
+                if ((u2[k] & 0xF8000000u) != tag)
+                    u2[k] = __hip_atomic_load(theirs + tid + 1024 * k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+#pragma unroll
+        for (int k = 0; k < 2; ++k) oth[k] = u2[k] & 0x07FFFFFFu;
         // ---- B: CRT, centered lift, digits (base_pir.rs:544-570): D1 mod p_r ----
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
@@ -1997,7 +1995,8 @@ __global__ __launch_bounds__(QUAD_NT) void chain_quad_kernel(const uint32_t* __r
             sga[k] = sa[sig_upd[k]];
             pa[k] = (uint64_t)sga[k] * cg0[k] + (uint64_t)te[k] * cqa[k];
         }
-        // the other three roles' words for pairs j = tid, tid + 512, in one round trip
+        // the other three roles' words for pairs j = tid, tid + 512: every poll
+        // reloads all words not yet tagged in one round trip
         uint32_t w[2][4];
 #pragma unroll
         for (int k = 0; k < 2; ++k)
@@ -2006,28 +2005,35 @@ __global__ __launch_bounds__(QUAD_NT) void chain_quad_kernel(const uint32_t* __r
                 w[k][q] = q == role ? (tag | own[k])
                                     : __hip_atomic_load(base + q * HALFN + tid + QUAD_NT * k, __ATOMIC_RELAXED,
                                                         __HIP_MEMORY_SCOPE_AGENT);
+        if (spin_limit <= 0) {  // no waiting at all (exercises the timeout path)
+            atomicOr(err, 1u);
 #pragma unroll
-        for (int k = 0; k < 2; ++k)
+            for (int k = 0; k < 2; ++k)
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                if (q == role) continue;
-                uint32_t u = w[k][q];
-                if (spin_limit <= 0) {  // no waiting at all (exercises the timeout path)
-                    atomicOr(err, 1u);
-                    u = tag;
-                }
-                int n = 0;
-                while ((u & 0xF8000000u) != tag) {
-                    if ((++n & 255) == 0 &&
-                        (n >= spin_limit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
-                        atomicOr(err, 1u);
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                    u = __hip_atomic_load(base + q * HALFN + tid + QUAD_NT * k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                }
-                w[k][q] = u;
+                for (int q = 0; q < 4; ++q) w[k][q] = tag;
+        }
+        for (int n = 0;;) {
+            bool ready = true;
+#pragma unroll
+            for (int k = 0; k < 2; ++k)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) ready = ready && (w[k][q] & 0xF8000000u) == tag;
+            if (ready) break;
+            // give up at the limit, or soon after any wait has (later steps then pass quickly)
+            if ((++n & 255) == 0 &&
+                (n >= spin_limit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
+                atomicOr(err, 1u);
+                break;
             }
+            __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+            for (int k = 0; k < 2; ++k)
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if ((w[k][q] & 0xF8000000u) != tag)
+                        w[k][q] = __hip_atomic_load(base + q * HALFN + tid + QUAD_NT * k, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+        }
         // ---- B: stage 0 inverse (both moduli), CRT, D1, stage 0 forward (this half) ----
 #pragma unroll
         for (int k = 0; k < 2; ++k) {

@@ -1913,28 +1913,37 @@ __global__ __launch_bounds__(QUAD_NT) void chain_quad_kernel(const uint32_t* __r
     // stage 0: the last inverse stage of both moduli, the first forward stage mod p_r
     const uint32_t wi[2] = {T->ipsi[0][1], T->ipsi[1][1]}, wiq[2] = {T->ipsiq[0][1], T->ipsiq[1][1]};
     const uint32_t wf = T->psi[r][1], wfq = T->psiq[r][1];
-    // per-thread constants of the update phase: local slots tid, tid + 512
-    uint32_t cg0[2], cg1[2], cqa[2], cb0[2], cb1[2], cqb[2];
+    // per-thread constants of the c-update: local slots tid, tid + 512
+    uint32_t cb0[2], cb1[2], cqb[2];
     int sig_upd[2];
+    const uint64_t t26 = (1ull << LOG2_B) % p;
     const uint32_t* qa = T->qa[2 + st][0];
     const uint32_t* qv = qs + (size_t)lane_slot[li] * 6 * 2 * N;  // [v][r][slot]
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
         const int s = off + tid + QUAD_NT * k;
-        cg0[k] = T->gka[0][r][s];
-        cg1[k] = T->gka[1][r][s];
-        cqa[k] = qa[r * N + s];
         cb0[k] = qv[(0 * 2 + r) * N + s];
         cb1[k] = qv[(1 * 2 + r) * N + s];
         cqb[k] = qv[((4 + st) * 2 + r) * N + s];
         sig_upd[k] = T->sigma5[s] - off;  // within the half
-        const uint64_t t26 = (1ull << LOG2_B) % p;
-        cg1[k] = (uint32_t)(((uint64_t)cg1[k] + p - t26 * cg0[k] % p) % p);
         cb1[k] = (uint32_t)(((uint64_t)cb1[k] + p - t26 * cb0[k] % p) % p);
     }
     // the transform passes: threads 0..255, radix-4 group g = 256 h + tid of the
     // full-length passes (the groups of half h)
     const int gq = off / 4 + (tid & 255);
+    // the a-update runs inside the last forward pass: thread tid < 256 updates
+    // local slots 4 tid .. 4 tid + 3, the elements of its last radix-4 group
+    const int sd0 = 4 * (tid & 255);
+    uint32_t cg0[4], cg1[4], cqa[4];
+    int sig_d[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const int s = off + sd0 + c;
+        cg0[c] = T->gka[0][r][s];
+        cg1[c] = (uint32_t)(((uint64_t)T->gka[1][r][s] + p - t26 * cg0[c] % p) % p);
+        cqa[c] = qa[r * N + s];
+        sig_d[c] = T->sigma5[s] - off;
+    }
     int sigA[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) sigA[c] = T->sigma5[4 * gq + c] - off;
@@ -1948,10 +1957,12 @@ __global__ __launch_bounds__(QUAD_NT) void chain_quad_kernel(const uint32_t* __r
 #pragma unroll 1
     for (int i = 0; i < L; ++i) {
         uint32_t tn[2];
+        uint4 ted;  // TE at the a-update slots
         {
             const uint32_t* te_row = te_base + (size_t)i * 2 * N + (size_t)r * N + off;
 #pragma unroll
             for (int k = 0; k < 2; ++k) tn[k] = te_row[tid + QUAD_NT * k];
+            ted = *reinterpret_cast<const uint4*>(te_row + sd0);
         }
         // ---- A: sigma_5 gather and inverse stages 10..1 on this half ----
         if (tid < 256) {
@@ -1972,28 +1983,47 @@ __global__ __launch_bounds__(QUAD_NT) void chain_quad_kernel(const uint32_t* __r
             inv_pass4gi<true>(xa, tw2[1], p, 3, gq);
         }
         lds_barrier();
-        if (tid < 256) inv_pass4gi<true>(xa, tw2[1], p, 1, gq);
-        lds_barrier();
         // ---- exchange: the four halves of stage 1's output ----
+        // The last inverse pass (stages 2, 1; group g = 256 h + tid holds local
+        // elements tid + 256 c) sends its words as soon as they are computed,
+        // each tagged with the step, and keeps them in xa for this block's own use.
         const uint32_t tag = (uint32_t)(i % 31 + 1) << 27;
         uint32_t* base = xo + (size_t)(i & 1) * 4 * HALFN;
+        if (tid < 256) {
+            uint32_t v[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) v[c] = xa[lx<true>(off + tid + 256 * c)];
+            const uint4 wb = *reinterpret_cast<const uint4*>(&tw2[1][4 + 2 * h]);
+            bf_gs(v[0], v[1], wb.x, wb.y, p);
+            bf_gs(v[2], v[3], wb.z, wb.w, p);
+            const uint2 wa = tw2[1][2 + h];
+            bf_gs(v[0], v[2], wa.x, wa.y, p);
+            bf_gs(v[1], v[3], wa.x, wa.y, p);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                __hip_atomic_store(base + role * HALFN + tid + 256 * c, tag | v[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                xa[lx<true>(off + tid + 256 * c)] = v[c];  // [0, 2p)
+            }
+        }
+        lds_barrier();
         uint32_t own[2];
 #pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            const int j = tid + QUAD_NT * k;
-            own[k] = xa[lx<true>(off + j)];  // [0, 2p)
-            __hip_atomic_store(base + role * HALFN + j, tag | own[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
+        for (int k = 0; k < 2; ++k) own[k] = xa[lx<true>(off + tid + QUAD_NT * k)];
         if (i > 0) {
             c_update_half(sc[cur], sc[cur ^ 1], xd[0], off, sga, te, cb0, cb1, cqb, sig_upd, p, K, tid);
             cur ^= 1;
         }
-        uint64_t pa[2];
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
             te[k] = tn[k];
             sga[k] = sa[sig_upd[k]];
-            pa[k] = (uint64_t)sga[k] * cg0[k] + (uint64_t)te[k] * cqa[k];
+        }
+        // the two products of the a-update that do not need the digits
+        uint64_t pa[4];
+        {
+            const uint32_t tv[4] = {ted.x, ted.y, ted.z, ted.w};
+#pragma unroll
+            for (int c = 0; c < 4; ++c) pa[c] = (uint64_t)sa[sig_d[c]] * cg0[c] + (uint64_t)tv[c] * cqa[c];
         }
         // the other three roles' words for pairs j = tid, tid + 512: every poll
         // reloads all words not yet tagged in one round trip
@@ -2058,15 +2088,21 @@ __global__ __launch_bounds__(QUAD_NT) void chain_quad_kernel(const uint32_t* __r
             wave_lds_sync();
             fwd_pass4gi<true>(xd[0], tw2[0], p, 7, gq);
             wave_lds_sync();
-            fwd_pass4gi<true>(xd[0], tw2[0], p, 9, gq);
-        }
-        lds_barrier();
-        // ---- D: update a on this half (c follows in the next step's exchange) ----
-#pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            const int sl = tid + QUAD_NT * k;
-            const uint32_t d1 = xd[0][lx<true>(off + sl)];  // NTT(D1), [0, 4p)
-            sa[sl] = red64(pa[k] + (uint64_t)d1 * cg1[k], p, K);
+            // ---- D: the last pass (stages 9, 10) and the a-update of its four
+            // slots (c follows in the next step's exchange) ----
+            const int b4 = lx<true>(4 * gq);
+            uint32_t v0 = xd[0][b4], v1 = xd[0][b4 + 1], v2 = xd[0][b4 + 2], v3 = xd[0][b4 + 3];
+            const uint2 wa = tw2[0][512 + gq];
+            bf_ct(v0, v2, wa.x, wa.y, p);
+            bf_ct(v1, v3, wa.x, wa.y, p);
+            const uint4 wb = *reinterpret_cast<const uint4*>(&tw2[0][1024 + 2 * gq]);
+            bf_ct(v0, v1, wb.x, wb.y, p);
+            bf_ct(v2, v3, wb.z, wb.w, p);
+            xd[0][b4] = v0; xd[0][b4 + 1] = v1; xd[0][b4 + 2] = v2; xd[0][b4 + 3] = v3;  // NTT(D1), [0, 4p)
+            sa[sd0] = red64(pa[0] + (uint64_t)v0 * cg1[0], p, K);
+            sa[sd0 + 1] = red64(pa[1] + (uint64_t)v1 * cg1[1], p, K);
+            sa[sd0 + 2] = red64(pa[2] + (uint64_t)v2 * cg1[2], p, K);
+            sa[sd0 + 3] = red64(pa[3] + (uint64_t)v3 * cg1[3], p, K);
         }
         lds_barrier();
     }

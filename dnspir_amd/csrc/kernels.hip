@@ -1869,11 +1869,9 @@ __device__ __forceinline__ void c_update_half(const uint32_t* __restrict__ c_in,
     }
 }
 
-// D1 mod p of the coefficient whose residues are x1 mod p1, x2 mod p2 (both
-// in [0, 2p), N times the coefficient): base_pir.rs:544-570
-__device__ __forceinline__ uint32_t digit1_mod(uint32_t x1, uint32_t x2, uint32_t p) {
-    const uint32_t e1 = mul_shoup(x1, NINV_Q2INV_P1, NINV_Q2INV_P1_SH, P1);
-    const uint32_t e2 = mul_shoup(x2, NINV_Q1INV_P2, NINV_Q1INV_P2_SH, P2);
+// D1 mod p of the coefficient c whose CRT terms are e1 = c (q/p1)^-1 mod p1
+// and e2 = c (q/p2)^-1 mod p2 (both reduced): base_pir.rs:544-570
+__device__ __forceinline__ uint32_t digit1_mod(uint32_t e1, uint32_t e2, uint32_t p) {
     uint64_t v = (uint64_t)e1 * P2 + (uint64_t)e2 * P1;
     v = v >= Q ? v - Q : v;
     const int64_t x = center_q(v);
@@ -1911,7 +1909,17 @@ __global__ __launch_bounds__(QUAD_NT) void chain_quad_kernel(const uint32_t* __r
         sc[0][k] = 0;
     }
     // stage 0: the last inverse stage of both moduli, the first forward stage mod p_r
-    const uint32_t wi[2] = {T->ipsi[0][1], T->ipsi[1][1]}, wiq[2] = {T->ipsiq[0][1], T->ipsiq[1][1]};
+    // The inverse stage's butterfly (x, y) -> (x + y, (x - y) psi^-1) and the
+    // CRT factors N^-1 (q/p_r)^-1 fold into one Shoup product per output:
+    // e_lo = (x + y) k_r, e_hi = (x - y) psi_r^-1 k_r (mod p_r).
+    const uint32_t kc[2] = {NINV_Q2INV_P1, NINV_Q1INV_P2}, kcq[2] = {NINV_Q2INV_P1_SH, NINV_Q1INV_P2_SH};
+    uint32_t wk[2], wkq[2];
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+        const uint32_t pm = mod_p(m);
+        wk[m] = (uint32_t)((uint64_t)T->ipsi[m][1] * kc[m] % pm);
+        wkq[m] = (uint32_t)(((uint64_t)wk[m] << 32) / pm);
+    }
     const uint32_t wf = T->psi[r][1], wfq = T->psiq[r][1];
     // per-thread constants of the c-update: local slots tid, tid + 512
     uint32_t cb0[2], cb1[2], cqb[2];
@@ -2069,11 +2077,13 @@ __global__ __launch_bounds__(QUAD_NT) void chain_quad_kernel(const uint32_t* __r
         for (int k = 0; k < 2; ++k) {
             const int j = tid + QUAD_NT * k;
             // roles: 0 = (p1, half 0), 1 = (p2, half 0), 2 = (p1, half 1), 3 = (p2, half 1)
-            uint32_t x1lo = w[k][0] & 0x07FFFFFFu, x1hi = w[k][2] & 0x07FFFFFFu;
-            uint32_t x2lo = w[k][1] & 0x07FFFFFFu, x2hi = w[k][3] & 0x07FFFFFFu;
-            bf_gs(x1lo, x1hi, wi[0], wiq[0], P1);
-            bf_gs(x2lo, x2hi, wi[1], wiq[1], P2);
-            const uint32_t dlo = digit1_mod(x1lo, x2lo, p), dhi = digit1_mod(x1hi, x2hi, p);
+            const uint32_t x1lo = w[k][0] & 0x07FFFFFFu, x1hi = w[k][2] & 0x07FFFFFFu;  // [0, 2p1)
+            const uint32_t x2lo = w[k][1] & 0x07FFFFFFu, x2hi = w[k][3] & 0x07FFFFFFu;  // [0, 2p2)
+            const uint32_t e1lo = mul_shoup(x1lo + x1hi, kc[0], kcq[0], P1);
+            const uint32_t e1hi = mul_shoup(x1lo - x1hi + 2 * P1, wk[0], wkq[0], P1);
+            const uint32_t e2lo = mul_shoup(x2lo + x2hi, kc[1], kcq[1], P2);
+            const uint32_t e2hi = mul_shoup(x2lo - x2hi + 2 * P2, wk[1], wkq[1], P2);
+            const uint32_t dlo = digit1_mod(e1lo, e2lo, p), dhi = digit1_mod(e1hi, e2hi, p);
             const uint32_t t = shoup_lazy(dhi, wf, wfq, p);
             xd[0][lx<true>(off + j)] = h == 0 ? dlo + t : dlo - t + 2 * p;  // [0, 4p)
         }

@@ -2008,10 +2008,13 @@ __global__ __launch_bounds__(QUAD_NT) void chain_quad_kernel(const uint32_t* __r
             bf_gs(v[0], v[2], wa.x, wa.y, p);
             bf_gs(v[1], v[3], wa.x, wa.y, p);
 #pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                __hip_atomic_store(base + role * HALFN + tid + 256 * c, tag | v[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                xa[lx<true>(off + tid + 256 * c)] = v[c];  // [0, 2p)
-            }
+            for (int c = 0; c < 4; ++c) xa[lx<true>(off + tid + 256 * c)] = v[c];  // [0, 2p)
+            // words j and j + 512 side by side: pairs (tid, tid + 512), (tid + 256, tid + 768)
+            uint64_t* mine = reinterpret_cast<uint64_t*>(base + role * HALFN);
+            __hip_atomic_store(mine + tid, (uint64_t)(tag | v[0]) | ((uint64_t)(tag | v[2]) << 32), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(mine + tid + 256, (uint64_t)(tag | v[1]) | ((uint64_t)(tag | v[3]) << 32), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
         }
         lds_barrier();
         uint32_t own[2];
@@ -2033,29 +2036,24 @@ __global__ __launch_bounds__(QUAD_NT) void chain_quad_kernel(const uint32_t* __r
 #pragma unroll
             for (int c = 0; c < 4; ++c) pa[c] = (uint64_t)sa[sig_d[c]] * cg0[c] + (uint64_t)tv[c] * cqa[c];
         }
-        // the other three roles' words for pairs j = tid, tid + 512: every poll
-        // reloads all words not yet tagged in one round trip
-        uint32_t w[2][4];
+        // the other three roles' words for pairs j = tid, tid + 512 (one 8-byte
+        // load per role): every poll reloads all pairs not yet tagged in one round trip
+        const uint64_t tag2 = (uint64_t)tag | ((uint64_t)tag << 32);
+        uint64_t w2[4];
 #pragma unroll
-        for (int k = 0; k < 2; ++k)
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                w[k][q] = q == role ? (tag | own[k])
-                                    : __hip_atomic_load(base + q * HALFN + tid + QUAD_NT * k, __ATOMIC_RELAXED,
-                                                        __HIP_MEMORY_SCOPE_AGENT);
+        for (int q = 0; q < 4; ++q)
+            w2[q] = q == role ? (tag2 | own[0] | ((uint64_t)own[1] << 32))
+                              : __hip_atomic_load(reinterpret_cast<const uint64_t*>(base + q * HALFN) + tid, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT);
         if (spin_limit <= 0) {  // no waiting at all (exercises the timeout path)
             atomicOr(err, 1u);
 #pragma unroll
-            for (int k = 0; k < 2; ++k)
-#pragma unroll
-                for (int q = 0; q < 4; ++q) w[k][q] = tag;
+            for (int q = 0; q < 4; ++q) w2[q] = tag2;
         }
         for (int n = 0;;) {
             bool ready = true;
 #pragma unroll
-            for (int k = 0; k < 2; ++k)
-#pragma unroll
-                for (int q = 0; q < 4; ++q) ready = ready && (w[k][q] & 0xF8000000u) == tag;
+            for (int q = 0; q < 4; ++q) ready = ready && (w2[q] & 0xF8000000F8000000ull) == tag2;
             if (ready) break;
             // give up at the limit, or soon after any wait has (later steps then pass quickly)
             if ((++n & 255) == 0 &&
@@ -2065,12 +2063,16 @@ __global__ __launch_bounds__(QUAD_NT) void chain_quad_kernel(const uint32_t* __r
             }
             __builtin_amdgcn_s_sleep(1);
 #pragma unroll
-            for (int k = 0; k < 2; ++k)
+            for (int q = 0; q < 4; ++q)
+                if ((w2[q] & 0xF8000000F8000000ull) != tag2)
+                    w2[q] = __hip_atomic_load(reinterpret_cast<const uint64_t*>(base + q * HALFN) + tid, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+        }
+        uint32_t w[2][4];
 #pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    if ((w[k][q] & 0xF8000000u) != tag)
-                        w[k][q] = __hip_atomic_load(base + q * HALFN + tid + QUAD_NT * k, __ATOMIC_RELAXED,
-                                                    __HIP_MEMORY_SCOPE_AGENT);
+        for (int q = 0; q < 4; ++q) {
+            w[0][q] = (uint32_t)w2[q];
+            w[1][q] = (uint32_t)(w2[q] >> 32);
         }
         // ---- B: stage 0 inverse (both moduli), CRT, D1, stage 0 forward (this half) ----
 #pragma unroll

@@ -464,8 +464,7 @@ int run_secondary(dnspir_ctx* c, const uint16_t* entries, long long col_stride, 
         const size_t words = chain_pair_xbuf_words(nst_s * nlanes);
         if ((rc = c->xch.ensure((words + 64) * 4))) return rc;
         launch_chain_pair(c->te.as<uint32_t>(), c->aout.as<uint32_t>(), c->cout.as<uint32_t>(), c->qs.as<uint32_t>(), d_slot,
-                          L_s, nst_s, nlanes, c->xch.as<uint32_t>() + 64, c->xch.as<unsigned>(), c->exchange_polls,
-                          g.tables, s, cus == 4);
+                          L_s, nst_s, nlanes, c->xch.as<uint32_t>(), c->exchange_polls, g.tables, s, cus == 4);
     } else {
         launch_chain_sec(c->te.as<uint32_t>(), c->aout.as<uint32_t>(), c->cout.as<uint32_t>(), c->qs.as<uint32_t>(), d_slot,
                          L_s, nst_s, nlanes, g.tables, s);

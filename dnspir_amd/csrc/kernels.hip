@@ -1339,39 +1339,43 @@ int k1b_get_choice(int ct, int nst) {
 
 // sum of the chains' a (and c) over the streams -> inverse NTT -> CRT ->
 // coefficients mod q and their 17-bit (a) / 12-bit (b) mod switch.
-// One block per lane index. a_out/c_out [st][li][r][slot].
-__global__ __launch_bounds__(1024) void chain_out_kernel(const uint32_t* __restrict__ a_out, const uint32_t* __restrict__ c_out,
-                                                        int nst, int nlanes, uint64_t* __restrict__ a_coef,
-                                                        uint64_t* __restrict__ b_coef, uint32_t* __restrict__ a17,
-                                                        uint32_t* __restrict__ b12, const DevTables* __restrict__ T) {
-    __shared__ uint32_t w[4][N];
-    const int li = blockIdx.x;
-    const int tid = threadIdx.x, nt = blockDim.x;
-    for (int s = tid; s < N; s += nt) {
+// One block per lane index and ring element (a, c). a_out/c_out [st][li][r][slot].
+__global__ __launch_bounds__(512) void chain_out_kernel(const uint32_t* __restrict__ a_out, const uint32_t* __restrict__ c_out,
+                                                       int nst, int nlanes, uint64_t* __restrict__ a_coef,
+                                                       uint64_t* __restrict__ b_coef, uint32_t* __restrict__ a17,
+                                                       uint32_t* __restrict__ b12, const DevTables* __restrict__ T) {
+    __shared__ uint32_t w[2][NPAD];
+    const int li = blockIdx.x, part = blockIdx.y;  // part 0: a, 1: c (the b of the reply)
+    const uint32_t* src = part == 0 ? a_out : c_out;
+    if (!src) return;
+    const int tid = threadIdx.x;
+    for (int s = tid; s < N; s += 512) {
+#pragma unroll
         for (int r = 0; r < 2; ++r) {
             const uint32_t p = mod_p(r);
-            uint32_t va = 0, vc = 0;
-            for (int st = 0; st < nst; ++st) {
-                va = add_mod(va, a_out[((size_t)(st * nlanes + li) * 2 + r) * N + s], p);
-                if (c_out) vc = add_mod(vc, c_out[((size_t)(st * nlanes + li) * 2 + r) * N + s], p);
-            }
-            w[r][s] = va;
-            w[2 + r][s] = vc;
+            uint32_t v = 0;
+            for (int st = 0; st < nst; ++st) v = add_mod(v, src[((size_t)(st * nlanes + li) * 2 + r) * N + s], p);
+            w[r][lx<true>(s)] = v;
         }
     }
     __syncthreads();
     {
-        const int q4 = tid / (nt / 4);
-        ntt_inv(w[q4], T, q4 & 1, tid - q4 * (nt / 4), nt / 4);
+        // one modulus per 256 threads; N^-1 is folded into the CRT below
+        const int r = tid >> 8;
+        ntt256_inv_unscaled<true>(w[r], T->ipsi[r], T->ipsiq[r], mod_p(r), tid & 255);
     }
-    for (int s = tid; s < N; s += nt) {
-        const uint64_t a = crt(w[0][s], w[1][s]);
-        if (a_coef) a_coef[(size_t)li * N + s] = a;
-        if (a17) a17[(size_t)li * N + s] = mod_switch_enc(a, 17);
-        if (c_out) {
-            const uint64_t b = crt(w[2][s], w[3][s]);
-            if (b_coef) b_coef[(size_t)li * N + s] = b;
-            if (b12) b12[(size_t)li * N + s] = mod_switch_enc(b, 12);
+    for (int s = tid; s < N; s += 512) {
+        const int x = lx<true>(s);
+        const uint32_t e1 = mul_shoup(w[0][x], NINV_Q2INV_P1, NINV_Q2INV_P1_SH, P1);
+        const uint32_t e2 = mul_shoup(w[1][x], NINV_Q1INV_P2, NINV_Q1INV_P2_SH, P2);
+        uint64_t v = (uint64_t)e1 * P2 + (uint64_t)e2 * P1;
+        v = v >= Q ? v - Q : v;
+        if (part == 0) {
+            if (a_coef) a_coef[(size_t)li * N + s] = v;
+            if (a17) a17[(size_t)li * N + s] = mod_switch_enc(v, 17);
+        } else {
+            if (b_coef) b_coef[(size_t)li * N + s] = v;
+            if (b12) b12[(size_t)li * N + s] = mod_switch_enc(v, 12);
         }
     }
 }
@@ -1738,15 +1742,14 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
         lds_barrier();
         // ---- exchange with the partner (other modulus) ----
         const uint32_t tag = (uint32_t)(i % 31 + 1) << 27;
-        uint32_t* mine = xo + ((size_t)(i & 1) * 2 + r) * N;
-        uint32_t* theirs = xo + ((size_t)(i & 1) * 2 + (r ^ 1)) * N;
+        // words j = tid and tid + 1024 side by side: one 8-byte store and load per thread
+        uint64_t* mine = reinterpret_cast<uint64_t*>(xo + ((size_t)(i & 1) * 2 + r) * N);
+        const uint64_t* theirs = reinterpret_cast<const uint64_t*>(xo + ((size_t)(i & 1) * 2 + (r ^ 1)) * N);
         uint32_t own[2], oth[2];
 #pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            const int j = tid + 1024 * k;
-            own[k] = xa[lx<true>(j)];  // [0, 2p)
-            __hip_atomic_store(mine + j, tag | own[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
+        for (int k = 0; k < 2; ++k) own[k] = xa[lx<true>(tid + 1024 * k)];  // [0, 2p)
+        __hip_atomic_store(mine + tid, (uint64_t)(tag | own[0]) | ((uint64_t)(tag | own[1]) << 32), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
         // while the partner's words travel: the c-update of the previous step
         // (its digit transforms are still in xd, its TE row in te)
         if (i > 0) {
@@ -1762,15 +1765,14 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
             sga[k] = sa[sig_upd[k]];
             pa[k] = (uint64_t)sga[k] * cg0[k] + (uint64_t)te[k] * cqa[k];
         }
-        // both words in one round trip; every poll reloads the words not yet tagged together
-        uint32_t u2[2];
-#pragma unroll
-        for (int k = 0; k < 2; ++k) u2[k] = __hip_atomic_load(theirs + tid + 1024 * k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // both words in one round trip, polled again until both carry this step's tag
+        const uint64_t tag2 = (uint64_t)tag | ((uint64_t)tag << 32);
+        uint64_t u2 = __hip_atomic_load(theirs + tid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (spin_limit <= 0) {  // no waiting at all (exercises the timeout path)
             atomicOr(err, 1u);
-            u2[0] = u2[1] = tag;
+            u2 = tag2;
         }
-        for (int n = 0; (u2[0] & 0xF8000000u) != tag || (u2[1] & 0xF8000000u) != tag;) {
+        for (int n = 0; (u2 & 0xF8000000F8000000ull) != tag2;) {
             // give up at the limit, or soon after any wait has (later steps then pass quickly)
             if ((++n & 255) == 0 &&
                 (n >= spin_limit || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
@@ -1778,13 +1780,10 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
                 break;
             }
             __builtin_amdgcn_s_sleep(1);
-#pragma unroll
-            for (int k = 0; k < 2; ++k)
-                if ((u2[k] & 0xF8000000u) != tag)
-                    u2[k] = __hip_atomic_load(theirs + tid + 1024 * k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            u2 = __hip_atomic_load(theirs + tid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-#pragma unroll
-        for (int k = 0; k < 2; ++k) oth[k] = u2[k] & 0x07FFFFFFu;
+        oth[0] = (uint32_t)u2 & 0x07FFFFFFu;
+        oth[1] = (uint32_t)(u2 >> 32) & 0x07FFFFFFu;
         // ---- B: CRT, centered lift, digits (base_pir.rs:544-570): D1 mod p_r ----
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
@@ -2133,11 +2132,12 @@ __global__ __launch_bounds__(QUAD_NT) void chain_quad_kernel(const uint32_t* __r
 size_t chain_pair_xbuf_words(int nchains) { return (size_t)nchains * 4 * N; }
 
 void launch_chain_pair(const uint32_t* te_tmp, uint32_t* a_out, uint32_t* c_out, const uint32_t* qs, const int* lane_slot,
-                       int L, int nst, int nlanes, uint32_t* xbuf, unsigned* err, int spin_limit, const DevTables* T,
-                       hipStream_t s, bool quad) {
+                       int L, int nst, int nlanes, uint32_t* xch, int spin_limit, const DevTables* T, hipStream_t s,
+                       bool quad) {
     const int nchains = nst * nlanes;
-    launch_fill_u32(xbuf, chain_pair_xbuf_words(nchains), 0, s);
-    launch_fill_u32(err, 1, 0, s);
+    launch_fill_u32(xch, 64 + chain_pair_xbuf_words(nchains), 0, s);
+    uint32_t* xbuf = xch + 64;
+    unsigned* err = reinterpret_cast<unsigned*>(xch);
     if (quad)
         hipLaunchKernelGGL(chain_quad_kernel, dim3(chain_multi_blocks(nchains, true)), dim3(QUAD_NT), 0, s, te_tmp, a_out, c_out,
                            qs, lane_slot, L, nlanes, nchains, xbuf, err, spin_limit, T);
@@ -2605,8 +2605,9 @@ void launch_diff(const uint32_t* a, const uint32_t* b, size_t n, unsigned* flag,
 
 void launch_chain_out(const uint32_t* a_out, const uint32_t* c_out, int nst, int nlanes, uint64_t* a_coef, uint64_t* b_coef,
                       uint32_t* a17, uint32_t* b12, const DevTables* T, hipStream_t s) {
-    // one block per lane: four inverse transforms of 256 threads each
-    hipLaunchKernelGGL(chain_out_kernel, dim3(nlanes), dim3(1024), 0, s, a_out, c_out, nst, nlanes, a_coef, b_coef, a17, b12, T);
+    // one block per lane and ring element (a, then c), one inverse transform
+    // per modulus of 256 threads each
+    hipLaunchKernelGGL(chain_out_kernel, dim3(nlanes, 2), dim3(512), 0, s, a_out, c_out, nst, nlanes, a_coef, b_coef, a17, b12, T);
 }
 
 void launch_qprep(const uint64_t* qcoef, uint32_t* qs, uint32_t* qp, int nq, const DevTables* T, hipStream_t s) {

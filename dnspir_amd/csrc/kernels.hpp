@@ -21,16 +21,17 @@ void launch_chain_pre(const uint32_t* te_tmp, uint32_t* ip_tmp, uint32_t* a_out,
 void launch_chain_sec(const uint32_t* te_tmp, uint32_t* a_out, uint32_t* c_out, const uint32_t* qs, const int* lane_slot,
                       int L, int nst, int nlanes, const DevTables* T, hipStream_t s);
 // the same on two CUs per chain (one per modulus) with a per-step exchange
-// through xbuf (chain_pair_xbuf_words words, cleared here); every wait ends
-// after spin_limit polls (<= 0: at once); *err != 0 after the launch means a
-// wait timed out (the outputs are then invalid)
+// through xch: word 0 the timeout flag, then 63 words unused, the exchange
+// words (chain_pair_xbuf_words) from word 64; all cleared here. Every wait
+// ends after spin_limit polls (<= 0: at once); xch[0] != 0 after the launch
+// means a wait timed out (the outputs are then invalid)
 // quad: four CUs per chain (modulus x half of the slots, chain_quad_kernel)
 // instead of two; the same xbuf size. chain_multi_blocks: the launch's grid
 size_t chain_pair_xbuf_words(int nchains);
 int chain_multi_blocks(int nchains, bool quad);
 void launch_chain_pair(const uint32_t* te_tmp, uint32_t* a_out, uint32_t* c_out, const uint32_t* qs, const int* lane_slot,
-                       int L, int nst, int nlanes, uint32_t* xbuf, unsigned* err, int spin_limit, const DevTables* T,
-                       hipStream_t s, bool quad = false);
+                       int L, int nst, int nlanes, uint32_t* xch, int spin_limit, const DevTables* T, hipStream_t s,
+                       bool quad = false);
 // rows_tmp [st][lane][row][arr][r][gpos] (written by launch_chain_pre) -> slab;
 // L = N/2 gives the limb-block layout read by launch_k1b, else the rotated one (launch_k1)
 void launch_relayout(const uint32_t* rows_tmp, uint32_t* slab, int L, int nst, hipStream_t s);

@@ -357,8 +357,9 @@ int run_primary(dnspir_ctx* c, const uint32_t* const* slabs, int len, int l_p, c
 
 // queues the copy of the two-CU chain's timeout flag to pinned host memory
 // (outside any graph capture: a copy to host memory cannot be captured)
+// the multi-CU chain's timeout flag reaches c->h_err through chain_out_kernel;
+// check_pair reads it after the call's sync
 static int copy_pair_flag(dnspir_ctx* c) {
-    HIP_OK(hipMemcpyAsync(c->h_err, c->xch.p, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
     c->pair_pending = true;
     return 0;
 }
@@ -458,19 +459,21 @@ int run_secondary(dnspir_ctx* c, const uint16_t* entries, long long col_stride, 
     if ((rc = c->a17o.ensure((size_t)nlanes * N * 4))) return rc;
     if ((rc = c->b12o.ensure((size_t)nlanes * N * 4))) return rc;
     note_secondary(c, entries, col_stride, d_off, d_ncols, d_slot, nlanes, l_s);
-    launch_k5(entries, d_off, col_stride, c->M.as<uint32_t>(), c->te.as<uint32_t>(), l_s, d_ncols, nlanes, g.tables, s);
+    // xch: the timeout flag in word 0, the exchange words from word 64, zeroed by K5
+    const size_t xwords = cus > 1 ? chain_pair_xbuf_words(nst_s * nlanes) + 64 : 0;
+    if (cus > 1 && (rc = c->xch.ensure(xwords * 4))) return rc;
+    launch_k5(entries, d_off, col_stride, c->M.as<uint32_t>(), c->te.as<uint32_t>(), l_s, d_ncols, nlanes, g.tables, s,
+              cus > 1 ? c->xch.as<uint32_t>() : nullptr, xwords);
     if (cus > 1) {
-        // xch: the timeout flag in word 0, the exchange words from word 64
-        const size_t words = chain_pair_xbuf_words(nst_s * nlanes);
-        if ((rc = c->xch.ensure((words + 64) * 4))) return rc;
         launch_chain_pair(c->te.as<uint32_t>(), c->aout.as<uint32_t>(), c->cout.as<uint32_t>(), c->qs.as<uint32_t>(), d_slot,
-                          L_s, nst_s, nlanes, c->xch.as<uint32_t>(), c->exchange_polls, g.tables, s, cus == 4);
+                          L_s, nst_s, nlanes, c->xch.as<uint32_t>(), c->exchange_polls, g.tables, s, cus == 4, true);
     } else {
         launch_chain_sec(c->te.as<uint32_t>(), c->aout.as<uint32_t>(), c->cout.as<uint32_t>(), c->qs.as<uint32_t>(), d_slot,
                          L_s, nst_s, nlanes, g.tables, s);
     }
     launch_chain_out(c->aout.as<uint32_t>(), c->cout.as<uint32_t>(), nst_s, nlanes, c->acoef.as<uint64_t>(),
-                     c->bcoef.as<uint64_t>(), c->a17o.as<uint32_t>(), c->b12o.as<uint32_t>(), g.tables, s);
+                     c->bcoef.as<uint64_t>(), c->a17o.as<uint32_t>(), c->b12o.as<uint32_t>(), g.tables, s,
+                     cus > 1 ? c->xch.as<unsigned>() : nullptr, cus > 1 ? c->h_err : nullptr);
     return 0;
 }
 

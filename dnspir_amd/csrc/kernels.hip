@@ -446,9 +446,17 @@ __device__ __forceinline__ void ntt256_inv_unscaled(uint32_t* buf, const uint32_
 // M [lane_idx][j][slot]
 __global__ __launch_bounds__(256) void k5_col_ntt_kernel(const uint16_t* __restrict__ entries, const long long* __restrict__ ent_off,
                                                          long long col_stride, uint32_t* __restrict__ M, int l,
-                                                         const int* __restrict__ ncols_nonzero, const DevTables* __restrict__ T) {
+                                                         const int* __restrict__ ncols_nonzero, const DevTables* __restrict__ T,
+                                                         uint32_t* __restrict__ clear, int clear_per_block, size_t clear_words) {
     __shared__ uint32_t a[NPAD];
     const int j = blockIdx.x, li = blockIdx.y;
+    // side task: zero a slice of `clear` (the chain's exchange words, which the
+    // chain launched after this phase needs cleared)
+    if (clear) {
+        const size_t c0 = ((size_t)li * gridDim.x + j) * clear_per_block;
+        for (int k = threadIdx.x; k < clear_per_block; k += blockDim.x)
+            if (c0 + k < clear_words) clear[c0 + k] = 0;
+    }
     uint32_t* out = M + ((size_t)li * l + j) * N;
     if (j >= ncols_nonzero[li]) {
         for (int s = threadIdx.x; s < N; s += blockDim.x) out[s] = 0;
@@ -1343,12 +1351,16 @@ int k1b_get_choice(int ct, int nst) {
 __global__ __launch_bounds__(512) void chain_out_kernel(const uint32_t* __restrict__ a_out, const uint32_t* __restrict__ c_out,
                                                        int nst, int nlanes, uint64_t* __restrict__ a_coef,
                                                        uint64_t* __restrict__ b_coef, uint32_t* __restrict__ a17,
-                                                       uint32_t* __restrict__ b12, const DevTables* __restrict__ T) {
+                                                       uint32_t* __restrict__ b12, const unsigned* __restrict__ flag_src,
+                                                       unsigned* flag_dst, const DevTables* __restrict__ T) {
     __shared__ uint32_t w[2][NPAD];
     const int li = blockIdx.x, part = blockIdx.y;  // part 0: a, 1: c (the b of the reply)
+    const int tid = threadIdx.x;
+    // the chains' timeout flag to pinned host memory (read after the call's sync)
+    if (flag_dst && li == 0 && part == 0 && tid == 0)
+        __hip_atomic_store(flag_dst, *flag_src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     const uint32_t* src = part == 0 ? a_out : c_out;
     if (!src) return;
-    const int tid = threadIdx.x;
     for (int s = tid; s < N; s += 512) {
 #pragma unroll
         for (int r = 0; r < 2; ++r) {
@@ -1588,7 +1600,7 @@ void launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s) {
 }
 
 void launch_k5(const uint16_t* entries, const long long* ent_off, long long col_stride, uint32_t* M, uint32_t* te_tmp, int l,
-               const int* ncols_nonzero, int nlanes, const DevTables* T, hipStream_t s) {
+               const int* ncols_nonzero, int nlanes, const DevTables* T, hipStream_t s, uint32_t* clear, size_t clear_words) {
     const int L = l < N ? l : N / 2;
     const int conj = l == N;
     const int kq = N / 2 / L;
@@ -1597,7 +1609,10 @@ void launch_k5(const uint16_t* entries, const long long* ent_off, long long col_
     int scale = 1;
     for (int x = 1; x < (int)T_PLAIN; ++x)
         if ((x * l) % (int)T_PLAIN == 1) { scale = x; break; }
-    hipLaunchKernelGGL(k5_col_ntt_kernel, dim3(l, nlanes), dim3(256), 0, s, entries, ent_off, col_stride, M, l, ncols_nonzero, T);
+    const int nblocks = l * nlanes;
+    const int clear_per_block = clear ? (int)((clear_words + nblocks - 1) / nblocks) : 0;
+    hipLaunchKernelGGL(k5_col_ntt_kernel, dim3(l, nlanes), dim3(256), 0, s, entries, ent_off, col_stride, M, l, ncols_nonzero, T,
+                       clear, clear_per_block, clear_words);
     // tiles of 64 KiB: TS slots x l rows, TS * 4 bytes contiguous per row
     if (l <= 512)
         hipLaunchKernelGGL(k5_rows_kernel<32>, dim3(N / 32, nlanes), dim3(K5R_NT), (size_t)l * 32 * 4, s, M, l, logl, kq, conj, T);
@@ -2133,9 +2148,9 @@ size_t chain_pair_xbuf_words(int nchains) { return (size_t)nchains * 4 * N; }
 
 void launch_chain_pair(const uint32_t* te_tmp, uint32_t* a_out, uint32_t* c_out, const uint32_t* qs, const int* lane_slot,
                        int L, int nst, int nlanes, uint32_t* xch, int spin_limit, const DevTables* T, hipStream_t s,
-                       bool quad) {
+                       bool quad, bool cleared) {
     const int nchains = nst * nlanes;
-    launch_fill_u32(xch, 64 + chain_pair_xbuf_words(nchains), 0, s);
+    if (!cleared) launch_fill_u32(xch, 64 + chain_pair_xbuf_words(nchains), 0, s);
     uint32_t* xbuf = xch + 64;
     unsigned* err = reinterpret_cast<unsigned*>(xch);
     if (quad)
@@ -2604,10 +2619,12 @@ void launch_diff(const uint32_t* a, const uint32_t* b, size_t n, unsigned* flag,
 }
 
 void launch_chain_out(const uint32_t* a_out, const uint32_t* c_out, int nst, int nlanes, uint64_t* a_coef, uint64_t* b_coef,
-                      uint32_t* a17, uint32_t* b12, const DevTables* T, hipStream_t s) {
+                      uint32_t* a17, uint32_t* b12, const DevTables* T, hipStream_t s, const unsigned* flag_src,
+                      unsigned* flag_dst) {
     // one block per lane and ring element (a, then c), one inverse transform
     // per modulus of 256 threads each
-    hipLaunchKernelGGL(chain_out_kernel, dim3(nlanes, 2), dim3(512), 0, s, a_out, c_out, nst, nlanes, a_coef, b_coef, a17, b12, T);
+    hipLaunchKernelGGL(chain_out_kernel, dim3(nlanes, 2), dim3(512), 0, s, a_out, c_out, nst, nlanes, a_coef, b_coef, a17, b12,
+                       flag_src, flag_dst, T);
 }
 
 void launch_qprep(const uint64_t* qcoef, uint32_t* qs, uint32_t* qp, int nq, const DevTables* T, hipStream_t s) {

@@ -13,9 +13,12 @@ void launch_fill_u64(unsigned long long* p, size_t n, unsigned long long v, hipS
 void launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 // K5: entries [l][8][N] u16 -> te_tmp [st][lane_idx][row][r][N]; M scratch [nlanes][l][N]
 // entry (column j, lane index li) at entries + ent_off[li] + j * col_stride;
-// ncols_nonzero[li]: columns of that lane that may be nonzero (device arrays)
+// ncols_nonzero[li]: columns of that lane that may be nonzero (device arrays);
+// clear (optional): clear_words words zeroed on the way (a multi-CU chain's
+// exchange buffer, so that its launch needs no fill of its own)
 void launch_k5(const uint16_t* entries, const long long* ent_off, long long col_stride, uint32_t* M, uint32_t* te_tmp, int l,
-               const int* ncols_nonzero, int nlanes, const DevTables* T, hipStream_t s);
+               const int* ncols_nonzero, int nlanes, const DevTables* T, hipStream_t s, uint32_t* clear = nullptr,
+               size_t clear_words = 0);
 void launch_chain_pre(const uint32_t* te_tmp, uint32_t* ip_tmp, uint32_t* a_out, int L, int nst, int qa_base,
                       const DevTables* T, hipStream_t s);
 void launch_chain_sec(const uint32_t* te_tmp, uint32_t* a_out, uint32_t* c_out, const uint32_t* qs, const int* lane_slot,
@@ -24,14 +27,15 @@ void launch_chain_sec(const uint32_t* te_tmp, uint32_t* a_out, uint32_t* c_out, 
 // through xch: word 0 the timeout flag, then 63 words unused, the exchange
 // words (chain_pair_xbuf_words) from word 64; all cleared here. Every wait
 // ends after spin_limit polls (<= 0: at once); xch[0] != 0 after the launch
-// means a wait timed out (the outputs are then invalid)
+// means a wait timed out (the outputs are then invalid). cleared: xch was
+// zeroed already on this stream (launch_k5's clear), no fill here
 // quad: four CUs per chain (modulus x half of the slots, chain_quad_kernel)
 // instead of two; the same xbuf size. chain_multi_blocks: the launch's grid
 size_t chain_pair_xbuf_words(int nchains);
 int chain_multi_blocks(int nchains, bool quad);
 void launch_chain_pair(const uint32_t* te_tmp, uint32_t* a_out, uint32_t* c_out, const uint32_t* qs, const int* lane_slot,
                        int L, int nst, int nlanes, uint32_t* xch, int spin_limit, const DevTables* T, hipStream_t s,
-                       bool quad = false);
+                       bool quad = false, bool cleared = false);
 // rows_tmp [st][lane][row][arr][r][gpos] (written by launch_chain_pre) -> slab;
 // L = N/2 gives the limb-block layout read by launch_k1b, else the rotated one (launch_k1)
 void launch_relayout(const uint32_t* rows_tmp, uint32_t* slab, int L, int nst, hipStream_t s);
@@ -67,8 +71,11 @@ void launch_fill_hash(uint32_t* p, size_t n, uint32_t seed, hipStream_t s);
 void launch_pack_buckets(const uint8_t* buckets, size_t n_valid, size_t n_total, uint16_t* coeffs, hipStream_t s);
 // *flag |= 1 where a and b differ
 void launch_diff(const uint32_t* a, const uint32_t* b, size_t n, unsigned* flag, hipStream_t s);
+// flag_dst (optional): receives *flag_src (a multi-CU chain's timeout flag),
+// written to pinned host memory by the kernel
 void launch_chain_out(const uint32_t* a_out, const uint32_t* c_out, int nst, int nlanes, uint64_t* a_coef, uint64_t* b_coef,
-                      uint32_t* a17, uint32_t* b12, const DevTables* T, hipStream_t s);
+                      uint32_t* a17, uint32_t* b12, const DevTables* T, hipStream_t s, const unsigned* flag_src = nullptr,
+                      unsigned* flag_dst = nullptr);
 void launch_qprep(const uint64_t* qcoef, uint32_t* qs, uint32_t* qp, int nq, const DevTables* T, hipStream_t s);
 // per-query Galois-key cache copies (src[q] -> key rows, else key rows -> dst[q]);
 // src, dst: device arrays of nq entry pointers (null = none)

@@ -112,6 +112,11 @@ int build_tables(DevTables* T) {
         uint64_t ni = dnh::powmod(N, p - 2, p);
         T->ninv[r] = (uint32_t)ni;
         T->ninvq[r] = shoup(ni, p);
+        // (q/p_r)^-1 mod p_r = (the other modulus)^-1 mod p_r
+        const uint64_t qi = dnh::powmod(mod_p(1 - r) % p, p - 2, p);
+        T->crt_k[0][r] = (uint32_t)(ni * qi % p);
+        T->crt_k[1][r] = (uint32_t)((uint64_t)T->ipsi[r][1] * T->crt_k[0][r] % p);
+        for (int m = 0; m < 2; ++m) T->crt_kq[m][r] = shoup(T->crt_k[m][r], p);
         uint64_t x = 1;
         for (int k = 0; k < M2N; ++k) {
             T->psipow[r][k] = (uint32_t)x;

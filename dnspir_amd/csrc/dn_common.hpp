@@ -41,6 +41,10 @@ struct DevTables {
     // each with its Shoup companion floor(w * 2^32 / p)
     uint32_t psi[2][N], psiq[2][N], ipsi[2][N], ipsiq[2][N];
     uint32_t ninv[2], ninvq[2];
+    // CRT factors of the chains' digit phase with their Shoup companions:
+    // crt_k[0][r] = N^-1 (q/p_r)^-1 mod p_r, crt_k[1][r] = psi_r^-1 crt_k[0][r]
+    // (the last inverse stage's twiddle folded in)
+    uint32_t crt_k[2][2], crt_kq[2][2];
     // psi_r^k for k < 2N (twists of the subgroup DFTs), with Shoup companions
     uint32_t psipow[2][M2N], psipowq[2][M2N];
     // 5^i and 5^-i mod 2N for i < N/2

@@ -662,6 +662,7 @@ __global__ __launch_bounds__(1024) void chain_kernel(const uint32_t* __restrict_
     sig_upd[0] = T->sigma5[tid];
     sig_upd[1] = T->sigma5[tid + 1024];
     int gpos_upd[2] = {T->slot_gpos[tid], T->slot_gpos[tid + 1024]};
+
     // phase A: 512 threads per modulus rA, radix-4 group gA first (stages 10, 9)
     const int rA = tid >> 9, gA = tid & 511;
     int sigA[4];
@@ -703,17 +704,24 @@ __global__ __launch_bounds__(1024) void chain_kernel(const uint32_t* __restrict_
             lds_barrier();
             inv_pass4gi<true>(xb[rA], tw2[1][rA], mod_p(rA), 1, gAx);
             lds_barrier();
-            inv_pass2_lasti<true>(xb[rA], tw2[1][rA], mod_p(rA), gAx);
-            lds_barrier();
         }
-        // ---- B: CRT, centered lift, digit D1 (base_pir.rs:544-570) ----
+        // ---- B: the last inverse stage, CRT, centered lift, digit D1
+        // (base_pir.rs:544-570) for the coefficient pair (tid, tid + 1024). The
+        // stage's butterfly (x, y) -> (x + y, (x - y) psi^-1) and the CRT factors
+        // N^-1 (q/p_r)^-1 fold into one Shoup product per coefficient. ----
+        uint32_t ecrt[2][2];  // [k: lo, hi][r]
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            const uint32_t p = mod_p(r);
+            const uint32_t x = xb[r][lx<true>(tid)], y = xb[r][lx<true>(tid + 1024)];  // [0, 2p)
+            ecrt[0][r] = mul_shoup(x + y, T->crt_k[0][r], T->crt_kq[0][r], p);
+            ecrt[1][r] = mul_shoup(x - y + 2 * p, T->crt_k[1][r], T->crt_kq[1][r], p);
+        }
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
             const int j = tid + 1024 * k;
             const int xj = lx<true>(j);
-            const uint32_t x1 = xb[0][xj], x2 = xb[1][xj];  // [0, 2p)
-            const uint32_t e1 = mul_shoup(x1, NINV_Q2INV_P1, NINV_Q2INV_P1_SH, P1);
-            const uint32_t e2 = mul_shoup(x2, NINV_Q1INV_P2, NINV_Q1INV_P2_SH, P2);
+            const uint32_t e1 = ecrt[k][0], e2 = ecrt[k][1];
             uint64_t v = (uint64_t)e1 * P2 + (uint64_t)e2 * P1;
             v = v >= Q ? v - Q : v;
             const int64_t x = center_q(v);
@@ -1926,14 +1934,8 @@ __global__ __launch_bounds__(QUAD_NT) void chain_quad_kernel(const uint32_t* __r
     // The inverse stage's butterfly (x, y) -> (x + y, (x - y) psi^-1) and the
     // CRT factors N^-1 (q/p_r)^-1 fold into one Shoup product per output:
     // e_lo = (x + y) k_r, e_hi = (x - y) psi_r^-1 k_r (mod p_r).
-    const uint32_t kc[2] = {NINV_Q2INV_P1, NINV_Q1INV_P2}, kcq[2] = {NINV_Q2INV_P1_SH, NINV_Q1INV_P2_SH};
-    uint32_t wk[2], wkq[2];
-#pragma unroll
-    for (int m = 0; m < 2; ++m) {
-        const uint32_t pm = mod_p(m);
-        wk[m] = (uint32_t)((uint64_t)T->ipsi[m][1] * kc[m] % pm);
-        wkq[m] = (uint32_t)(((uint64_t)wk[m] << 32) / pm);
-    }
+    const uint32_t kc[2] = {T->crt_k[0][0], T->crt_k[0][1]}, kcq[2] = {T->crt_kq[0][0], T->crt_kq[0][1]};
+    const uint32_t wk[2] = {T->crt_k[1][0], T->crt_k[1][1]}, wkq[2] = {T->crt_kq[1][0], T->crt_kq[1][1]};
     const uint32_t wf = T->psi[r][1], wfq = T->psiq[r][1];
     // per-thread constants of the c-update: local slots tid, tid + 512
     uint32_t cb0[2], cb1[2], cqb[2];

@@ -222,63 +222,13 @@ __device__ __forceinline__ void inv_pass4(uint32_t* buf, const uint32_t* tw, con
         buf[b4] = v0; buf[b4 + 1] = v1; buf[b4 + 2] = v2; buf[b4 + 3] = v3;
     }
 }
-// Radix-4 passes for 512 threads per transform (g < 512): stages t, t+1
-// (forward) or t+1, t (inverse) with 2^t groups at stage t; element stride
-// q = N / 2^(t+2) inside a group of 4.
-template <bool PAD = false>
-__device__ __forceinline__ void fwd_pass4g(uint32_t* buf, const uint32_t* tw, const uint32_t* twq, uint32_t p, int t, int g) {
-    const int m = 1 << t, lq = LOGN - t - 2, q = 1 << lq;
-    const int i = g >> lq, j0 = (i << (lq + 2)) + (g & (q - 1));
-    uint32_t v0 = buf[lx<PAD>(j0)], v1 = buf[lx<PAD>(j0 + q)], v2 = buf[lx<PAD>(j0 + 2 * q)], v3 = buf[lx<PAD>(j0 + 3 * q)];
-    const uint32_t w = tw[m + i], wq = twq[m + i];
-    bf_ct(v0, v2, w, wq, p);
-    bf_ct(v1, v3, w, wq, p);
-    bf_ct(v0, v1, tw[2 * m + 2 * i], twq[2 * m + 2 * i], p);
-    bf_ct(v2, v3, tw[2 * m + 2 * i + 1], twq[2 * m + 2 * i + 1], p);
-    buf[lx<PAD>(j0)] = v0; buf[lx<PAD>(j0 + q)] = v1; buf[lx<PAD>(j0 + 2 * q)] = v2; buf[lx<PAD>(j0 + 3 * q)] = v3;
-}
-// last forward stage (10) by 512 threads: pairs (2i, 2i + 1), i = g, g + 512
-template <bool PAD = false>
-__device__ __forceinline__ void fwd_pass2_last(uint32_t* buf, const uint32_t* tw, const uint32_t* twq, uint32_t p, int g) {
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-        const int i = g + 512 * k;
-        uint32_t x = buf[lx<PAD>(2 * i)], y = buf[lx<PAD>(2 * i + 1)];
-        bf_ct(x, y, tw[1024 + i], twq[1024 + i], p);
-        buf[lx<PAD>(2 * i)] = x;
-        buf[lx<PAD>(2 * i + 1)] = y;
-    }
-}
-template <bool PAD = false>
-__device__ __forceinline__ void inv_pass4g(uint32_t* buf, const uint32_t* tw, const uint32_t* twq, uint32_t p, int t, int g) {
-    const int m = 1 << t, lq = LOGN - t - 2, q = 1 << lq;
-    const int i = g >> lq, j0 = (i << (lq + 2)) + (g & (q - 1));
-    uint32_t v0 = buf[lx<PAD>(j0)], v1 = buf[lx<PAD>(j0 + q)], v2 = buf[lx<PAD>(j0 + 2 * q)], v3 = buf[lx<PAD>(j0 + 3 * q)];
-    bf_gs(v0, v1, tw[2 * m + 2 * i], twq[2 * m + 2 * i], p);
-    bf_gs(v2, v3, tw[2 * m + 2 * i + 1], twq[2 * m + 2 * i + 1], p);
-    const uint32_t w = tw[m + i], wq = twq[m + i];
-    bf_gs(v0, v2, w, wq, p);
-    bf_gs(v1, v3, w, wq, p);
-    buf[lx<PAD>(j0)] = v0; buf[lx<PAD>(j0 + q)] = v1; buf[lx<PAD>(j0 + 2 * q)] = v2; buf[lx<PAD>(j0 + 3 * q)] = v3;
-}
-// last inverse stage (0) by 512 threads: pairs (j, j + N/2), j = g, g + 512
-template <bool PAD = false>
-__device__ __forceinline__ void inv_pass2_last(uint32_t* buf, const uint32_t* tw, const uint32_t* twq, uint32_t p, int g) {
-    const uint32_t w = tw[1], wq = twq[1];
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-        const int j = g + 512 * k;
-        uint32_t x = buf[lx<PAD>(j)], y = buf[lx<PAD>(j + N / 2)];
-        bf_gs(x, y, w, wq, p);
-        buf[lx<PAD>(j)] = x;
-        buf[lx<PAD>(j + N / 2)] = y;
-    }
-}
 
-// The same radix-4 passes with each twiddle stored beside its Shoup companion
-// (tw2[k] = (w_k, wq_k), 16-byte aligned): one 8-byte LDS read for the first
-// stage's twiddle, one 16-byte read for the second stage's adjacent pair, so
-// a group issues 2 twiddle reads instead of 6.
+// Radix-4 passes for 512 threads per transform (group g < 512): stages t, t+1
+// (forward) or t+1, t (inverse) with 2^t groups at stage t; element stride
+// q = N / 2^(t+2) inside a group of 4. Each twiddle sits beside its Shoup
+// companion (tw2[k] = (w_k, wq_k), 16-byte aligned): one 8-byte LDS read for
+// the first stage's twiddle, one 16-byte read for the second stage's adjacent
+// pair, so a group issues 2 twiddle reads instead of 6.
 template <bool PAD = false>
 __device__ __forceinline__ void fwd_pass4gi(uint32_t* buf, const uint2* tw2, uint32_t p, int t, int g) {
     const int m = 1 << t, lq = LOGN - t - 2, q = 1 << lq;
@@ -349,18 +299,6 @@ __device__ __forceinline__ void fwd_pass4i(uint32_t* buf, const uint2* tw2, uint
         buf[b4] = v0; buf[b4 + 1] = v1; buf[b4 + 2] = v2; buf[b4 + 3] = v3;
     }
 }
-template <bool PAD = false>
-__device__ __forceinline__ void fwd_pass2_lasti(uint32_t* buf, const uint2* tw2, uint32_t p, int g) {
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-        const int i = g + 512 * k;
-        uint32_t x = buf[lx<PAD>(2 * i)], y = buf[lx<PAD>(2 * i + 1)];
-        const uint2 w = tw2[1024 + i];
-        bf_ct(x, y, w.x, w.y, p);
-        buf[lx<PAD>(2 * i)] = x;
-        buf[lx<PAD>(2 * i + 1)] = y;
-    }
-}
 // fwd_pass4i with the groups of wave w (g = thread index of 256) in elements
 // [512w, 512w + 512), where the radix-8 passes at strides 32 and 4 leave them
 template <bool PAD = false>
@@ -394,19 +332,6 @@ __device__ __forceinline__ void fwd_pass2_lastw(uint32_t* buf, const uint2* tw2,
         buf[lx<PAD>(2 * i + 1)] = y;
     }
 }
-template <bool PAD = false>
-__device__ __forceinline__ void inv_pass2_lasti(uint32_t* buf, const uint2* tw2, uint32_t p, int g) {
-    const uint2 w = tw2[1];
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-        const int j = g + 512 * k;
-        uint32_t x = buf[lx<PAD>(j)], y = buf[lx<PAD>(j + N / 2)];
-        bf_gs(x, y, w.x, w.y, p);
-        buf[lx<PAD>(j)] = x;
-        buf[lx<PAD>(j + N / 2)] = y;
-    }
-}
-
 // whole-block transforms by 256 threads (g = thread index) on an LDS array;
 // inputs in [0, 2p), outputs in [0, p) (forward) / [0, 2p) (inverse, which
 // leaves the N factor in)
@@ -1661,8 +1586,9 @@ __device__ __forceinline__ void c_update(const uint32_t* __restrict__ c_in, uint
 
 // The secondary chain on two CUs per chain (latency form for a few queries):
 // block (chain, r) keeps a and c mod p_r. Each step it inverse-transforms
-// sigma_5(a) mod p_r, swaps the 2048 coefficients with its partner block (the
-// other modulus) through xbuf, computes the top gadget digit D1 of every
+// sigma_5(a) mod p_r down to stage 1, swaps those 2048 words with its partner
+// block (the other modulus) through xbuf, applies the last inverse stage of
+// both moduli inside the CRT, computes the top gadget digit D1 of every
 // coefficient, forward-transforms it mod p_r (NTT(D0) = sigma_5(a) - 2^26
 // NTT(D1) needs no transform) and updates a and c mod p_r: half the
 // transforms of chain_kernel<true> per block, plus one exchange. Partners are blocks b and b ^ 8 (one XCD under round-robin
@@ -1761,8 +1687,7 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
         lds_barrier();
         if (tid < 512) inv_pass4gi<true>(xa, tw2[1], p, 1, gA);
         lds_barrier();
-        if (tid < 512) inv_pass2_lasti<true>(xa, tw2[1], p, gA);
-        lds_barrier();
+        // (the last inverse stage, pairs (j, j + 1024), folds into phase B)
         // ---- exchange with the partner (other modulus) ----
         const uint32_t tag = (uint32_t)(i % 31 + 1) << 27;
         // words j = tid and tid + 1024 side by side: one 8-byte store and load per thread
@@ -1807,14 +1732,24 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
         }
         oth[0] = (uint32_t)u2 & 0x07FFFFFFu;
         oth[1] = (uint32_t)(u2 >> 32) & 0x07FFFFFFu;
-        // ---- B: CRT, centered lift, digits (base_pir.rs:544-570): D1 mod p_r ----
+        // ---- B: the last inverse stage of both moduli for the pair (tid,
+        // tid + 1024), CRT, centered lift, digits (base_pir.rs:544-570): D1 mod
+        // p_r. The stage's butterfly and the CRT factors fold into one Shoup
+        // product per coefficient (as in chain_kernel). ----
+        uint32_t ecrt[2][2];  // [k: lo, hi][modulus]
+        {
+            const uint32_t x1 = r == 0 ? own[0] : oth[0], y1 = r == 0 ? own[1] : oth[1];  // [0, 2p1)
+            const uint32_t x2 = r == 0 ? oth[0] : own[0], y2 = r == 0 ? oth[1] : own[1];  // [0, 2p2)
+            ecrt[0][0] = mul_shoup(x1 + y1, T->crt_k[0][0], T->crt_kq[0][0], P1);
+            ecrt[1][0] = mul_shoup(x1 - y1 + 2 * P1, T->crt_k[1][0], T->crt_kq[1][0], P1);
+            ecrt[0][1] = mul_shoup(x2 + y2, T->crt_k[0][1], T->crt_kq[0][1], P2);
+            ecrt[1][1] = mul_shoup(x2 - y2 + 2 * P2, T->crt_k[1][1], T->crt_kq[1][1], P2);
+        }
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
             const int j = tid + 1024 * k;
             const int xj = lx<true>(j);
-            const uint32_t x1 = r == 0 ? own[k] : oth[k], x2 = r == 0 ? oth[k] : own[k];  // [0, 2p)
-            const uint32_t e1 = mul_shoup(x1, NINV_Q2INV_P1, NINV_Q2INV_P1_SH, P1);
-            const uint32_t e2 = mul_shoup(x2, NINV_Q1INV_P2, NINV_Q1INV_P2_SH, P2);
+            const uint32_t e1 = ecrt[k][0], e2 = ecrt[k][1];
             uint64_t v = (uint64_t)e1 * P2 + (uint64_t)e2 * P1;
             v = v >= Q ? v - Q : v;
             const int64_t x = center_q(v);

@@ -64,21 +64,25 @@ struct DevTables {
 };
 
 // ---------------------------------------------------------------------------
-// exact modular arithmetic
+// exact modular arithmetic. A conditional subtraction x >= p ? x - p : x for
+// x < 2p < 2^31 is written as the unsigned minimum of x and x - p (x - p wraps
+// above x when x < p): one v_min_u32 instead of a compare, a select and an add.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t csub(uint32_t x, uint32_t p) {
+    return __builtin_elementwise_min(x, x - p);
+}
 __device__ __forceinline__ uint32_t add_mod(uint32_t a, uint32_t b, uint32_t p) {
-    uint32_t s = a + b;
-    return s >= p ? s - p : s;
+    return csub(a + b, p);
 }
 __device__ __forceinline__ uint32_t sub_mod(uint32_t a, uint32_t b, uint32_t p) {
-    uint32_t d = a - b;
-    return a >= b ? d : d + p;
+    const uint32_t d = a - b;  // wraps when a < b, and d + p then wraps back below p
+    return __builtin_elementwise_min(d, d + p);
 }
 // a * w mod p for a < 2^32, w < p, wq = floor(w * 2^32 / p) (Shoup)
 __device__ __forceinline__ uint32_t mul_shoup(uint32_t a, uint32_t w, uint32_t wq, uint32_t p) {
     uint32_t qt = __umulhi(a, wq);
     uint32_t r = a * w - qt * p;  // in [0, 2p)
-    return r >= p ? r - p : r;
+    return csub(r, p);
 }
 // x < 2^52 (a product of two residues) -> x mod p. Two folds of
 // 2^K = 4095 (mod p) leave z < 2^24 + 2^K < 2p.

@@ -80,8 +80,7 @@ __device__ __forceinline__ int brv_bits(int v, int bits) {
 // array; Harvey lazy values ([0, 4p) forward, [0, 2p) inverse), Shoup products by twiddles
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t red2p(uint32_t a, uint32_t p2) {  // [0, 4p) -> [0, 2p)
-    const uint32_t b = a - p2;
-    return a < p2 ? a : b;
+    return csub(a, p2);
 }
 __device__ __forceinline__ uint32_t shoup_lazy(uint32_t a, uint32_t w, uint32_t wq, uint32_t p) {  // -> [0, 2p)
     return a * w - __umulhi(a, wq) * p;

@@ -346,7 +346,7 @@ __device__ __forceinline__ void ntt256_fwd(uint32_t* buf, const uint32_t* tw, co
     for (int j = g; j < N; j += 256) {
         const int x = lx<PAD>(j);
         uint32_t v = red2p(buf[x], 2 * p);
-        buf[x] = v >= p ? v - p : v;
+        buf[x] = csub(v, p);
     }
     __syncthreads();
 }
@@ -470,7 +470,7 @@ __global__ __launch_bounds__(K5R_NT) void k5_rows_kernel(uint32_t* __restrict__ 
         else y = ((M2N - z) * T->pow5[i]) & (M2N - 1);
         const int u = (((M2N - y) & (M2N - 1)) - coset) >> lstep;  // natural DFT index
         const uint32_t v = tile[brv_bits(u, logl) * TS + d];
-        base[(size_t)i * N + s] = v >= P1 ? v - P1 : v;
+        base[(size_t)i * N + s] = csub(v, P1);
     }
 }
 
@@ -495,7 +495,7 @@ __global__ __launch_bounds__(256) void k5_cols_final_kernel(const uint32_t* __re
     for (int s = tid; s < N; s += nt) {
         const int xs = lx<true>(s);
         uint32_t u = a[0][xs];
-        u = u >= P1 ? u - P1 : u;
+        u = csub(u, P1);
         int32_t x = (int32_t)mul_shoup(u, T->ninv[0], T->ninvq[0], P1);
         if (x > (int32_t)(P1 / 2)) x -= (int32_t)P1;
         int32_t xt = x % (int32_t)T_PLAIN;
@@ -652,9 +652,9 @@ __global__ __launch_bounds__(1024) void chain_kernel(const uint32_t* __restrict_
             // |x| < q/2 < 2^50: the second balanced digit is hi itself (the third is 0)
             const int32_t d1 = (int32_t)((x + (1ll << (LOG2_B - 1))) >> LOG2_B);  // |d1| <= 2^24
             uint32_t t = (uint32_t)(d1 + (int32_t)P1);
-            xb[0][xj] = t >= P1 ? t - P1 : t;
+            xb[0][xj] = csub(t, P1);
             t = (uint32_t)(d1 + (int32_t)P2);
-            xb[1][xj] = t >= P2 ? t - P2 : t;
+            xb[1][xj] = csub(t, P2);
         }
         lds_barrier();
         // sigma_5(a) of this step (sa is rewritten in D)
@@ -702,7 +702,7 @@ __global__ __launch_bounds__(1024) void chain_kernel(const uint32_t* __restrict_
                 const int gp = gpos_upd[k >> 1];
                 dst[(0 * 2 + r) * N + gp] = te[k];
                 uint32_t e1 = red2p(d1, 2 * p);
-                e1 = e1 >= p ? e1 - p : e1;
+                e1 = csub(e1, p);
                 const uint32_t m = mul_mod(e1, (uint32_t)((1ull << LOG2_B) % p), p, K);
                 dst[(1 * 2 + r) * N + gp] = sga[k] >= m ? sga[k] - m : sga[k] + p - m;
                 dst[(2 * 2 + r) * N + gp] = e1;
@@ -1489,8 +1489,8 @@ __global__ __launch_bounds__(512) void epi_primary_kernel(EpiArgs A, const DevTa
     for (int s = tid; s < N; s += 512) {
         const int x = lx<true>(s);
         uint32_t x1 = w[0][x], x2 = w[1][x];
-        x1 = x1 >= P1 ? x1 - P1 : x1;
-        x2 = x2 >= P2 ? x2 - P2 : x2;
+        x1 = csub(x1, P1);
+        x2 = csub(x2, P2);
         const uint32_t e1 = mul_mod(x1, NINV_Q2INV_P1, P1, 26);
         const uint32_t e2 = mul_mod(x2, NINV_Q1INV_P2, P2, 25);
         uint64_t b = (uint64_t)e1 * P2 + (uint64_t)e2 * P1;
@@ -1755,7 +1755,7 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
             // |x| < q/2 < 2^50: the second balanced digit is hi itself (the third is 0)
             const int32_t d1 = (int32_t)((x + (1ll << (LOG2_B - 1))) >> LOG2_B);  // |d1| <= 2^24
             const uint32_t t = (uint32_t)(d1 + (int32_t)p);
-            xd[0][xj] = t >= p ? t - p : t;
+            xd[0][xj] = csub(t, p);
         }
         lds_barrier();
         // ---- C: forward NTT of D1 mod p_r (threads 0..511, radix 4) ----
@@ -1834,7 +1834,7 @@ __device__ __forceinline__ uint32_t digit1_mod(uint32_t e1, uint32_t e2, uint32_
     // |x| < q/2 < 2^50: the second balanced digit is hi itself (the third is 0)
     const int32_t d1 = (int32_t)((x + (1ll << (LOG2_B - 1))) >> LOG2_B);  // |d1| <= 2^24
     const uint32_t t = (uint32_t)(d1 + (int32_t)p);
-    return t >= p ? t - p : t;
+    return csub(t, p);
 }
 
 __global__ __launch_bounds__(QUAD_NT) void chain_quad_kernel(const uint32_t* __restrict__ te_tmp, uint32_t* __restrict__ a_out,

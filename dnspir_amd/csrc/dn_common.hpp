@@ -64,9 +64,10 @@ struct DevTables {
 };
 
 // ---------------------------------------------------------------------------
-// exact modular arithmetic. A conditional subtraction x >= p ? x - p : x for
-// x < 2p < 2^31 is written as the unsigned minimum of x and x - p (x - p wraps
-// above x when x < p): one v_min_u32 instead of a compare, a select and an add.
+// exact modular arithmetic. A conditional subtraction x >= p ? x - p : x is
+// written as the unsigned minimum of x and x - p (x - p wraps above x when
+// x < p; the same result for every 32-bit x): one v_min_u32 instead of a
+// compare, a select and an add.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t csub(uint32_t x, uint32_t p) {
     return __builtin_elementwise_min(x, x - p);
@@ -98,6 +99,8 @@ __device__ __forceinline__ uint32_t red64(uint64_t x, uint32_t p, int K) {
     uint64_t y = (x >> K) * 4095ull + (x & mask);   // < 2^51 + 2^K
     y = (y >> K) * 4095ull + (y & mask);            // < 2^38 + 2^K
     uint32_t z = (uint32_t)((y >> K) * 4095ull + (y & mask));  // < 2^26 + 2^K
+    // (kept as compares: k1w_kernel's schedule, which this epilogue shares,
+    // measured 4% slower with the minimum form)
     z = z >= p ? z - p : z;
     return z >= p ? z - p : z;
 }

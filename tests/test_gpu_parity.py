@@ -290,6 +290,17 @@ def test_secondary_chain_on_one_two_four_cus(pir, nsh, two):
         assert st["pair_retries"] == 0, st
 
 
+def test_chain_cus_setting_is_checked(pir):
+    from dnspir_amd import DnspirError
+    ctx = pir.Context()
+    for bad in (3, 5, 8, -1):
+        with pytest.raises(DnspirError) as ei:
+            ctx.set_chain_cus(bad)
+        assert ei.value.code == -5
+    for good in (0, 1, 2, 4):
+        ctx.set_chain_cus(good)
+
+
 def test_batch_on_four_cu_chains_matches_alone(pir):
     """A batch of five queries (15 chains: four-CU blocks for 16, the last
     chain's blocks idle) answers every query as process_queries does alone."""

@@ -199,3 +199,11 @@ def test_every_error_code_has_a_text():
     assert _lib.lib().dnspir_strerror(0).decode() == "ok"
     assert _lib.lib().dnspir_strerror(-999).decode() == "unknown error"
     assert set(_lib.ERRORS) == set(codes)
+
+
+def test_chain_cus_setter_needs_a_context():
+    """dnspir_ctx_set_chain_cus without a context is DNSPIR_E_ARG (no device
+    needed to refuse); the values it refuses on a context are tested on the GPU."""
+    L = _lib.lib()
+    for cus in (0, 1, 2, 4, 3):
+        assert L.dnspir_ctx_set_chain_cus(None, cus) == -5

@@ -1648,6 +1648,30 @@ int dnspir_autotune_batch(dnspir_ctx* c, const dnspir_shard* const* fleet, size_
     return 0;
 }
 
+int dnspir_probe_mfma_rate(dnspir_ctx* c, int zero_operands, int reps, double* tops) {
+    if (!tops || reps < 1 || (zero_operands != 0 && zero_operands != 1)) return DNSPIR_E_ARG;
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (!c) c = default_ctx();
+    std::lock_guard<std::mutex> lk(c->mu);
+    const hipStream_t s = c->stream;
+    int* out = nullptr;
+    HIP_OK(hipMalloc(&out, mfma_rate_out_words() * sizeof(int)));
+    hipEvent_t e0 = c->ev[3], e1 = c->ev[4];
+    launch_mfma_rate(zero_operands, out, 1u, s);  // warm-up
+    hipError_t e = hipEventRecord(e0, s);
+    for (int k = 0; k < reps && e == hipSuccess; ++k) launch_mfma_rate(zero_operands, out, 2u + k, s);
+    if (e == hipSuccess) e = hipEventRecord(e1, s);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    (void)hipFree(out);
+    HIP_OK(e);
+    *tops = ms > 0 ? mfma_rate_ops() * reps / (ms * 1e-3) / 1e12 : 0;
+    return 0;
+}
+
 // ---- client ------------------------------------------------------------------
 struct dnspir_rng {
     dnh::ChaCha12 r;

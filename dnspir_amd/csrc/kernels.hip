@@ -904,7 +904,8 @@ __global__ __launch_bounds__(256) void qcorr_kernel(const uint32_t* __restrict__
 // or 1 mod p2), the query into 4 signed digits; the 16 digit products are summed in int32 per weight
 // w = u + v (|sum| <= 4 * 6144 * 2^14 < 2^29), and
 // OUT = sum_w 2^(8w) ACC_w + corr_b mod p. One wave per a-group (32 output
-// rows), CT column tiles of 32 queries. Writes outb[q][shard][r][gpos][lane].
+// rows), CT column tiles of 32 queries. Writes outb[q][shard][r][lane][gpos]
+// (lane-major, so the epilogue of one (query, shard, lane) reads 8 KiB rows).
 // ---------------------------------------------------------------------------
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
@@ -1007,7 +1008,7 @@ __device__ __forceinline__ void k1b_store(const v16i (&acc)[7][CT], const void* 
             for (int w = 0; w < 7; ++w) t += (int64_t)acc[w][ct][e] * cw[w];
             const int row = 8 * (e >> 2) + 4 * (l >> 5) + (e & 3);
             const int a = g * 4 + (row >> 3), lane8 = row & 7;
-            o[(half * HALFN + a) * LANES + lane8] = from_i64(t, p, K);
+            o[(size_t)lane8 * N + half * HALFN + a] = from_i64(t, p, K);
         }
     }
 }
@@ -1228,6 +1229,72 @@ __global__ __launch_bounds__(256) void k1b_direct_kernel(const uint32_t* const* 
         }
     }
     k1b_store<CT>(acc, ql, outb, S, r, half, sh, g, l, nq, q_stride);
+}
+
+// Diagnostic: the batched inner product's MFMA instruction mix with the
+// operands held in registers (no memory traffic): per wave 7 x 2 accumulator
+// tiles, per K-step 32 v_mfma_i32_32x32x32_i8 in k1b's (u, v, column tile)
+// order, 96 K-steps, one 256-thread block per (a-group quad, half, shard,
+// modulus) of a 32-shard half-size fleet. ZERO = 0 draws every operand digit
+// from a hash and changes it each step (what shards and queries look like to
+// the matrix pipe); ZERO = 1 keeps every digit zero. The clock this GPU holds
+// depends on how much the operand bits switch, so the two bound what the k1b
+// kernels can reach on real data.
+template <int ZERO>
+__global__ __launch_bounds__(256) void mfma_rate_kernel(int* __restrict__ out, int nk, uint32_t seed) {
+    uint32_t h = (blockIdx.x * 256u + threadIdx.x) * 2654435761u ^ seed;
+    v4i a[4], b[2][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        h = h * 1664525u + 1013904223u;
+        a[i] = ZERO ? (v4i){0, 0, 0, 0} : (v4i){(int)h, (int)(h * 3u), (int)(h * 5u), (int)(h * 7u)};
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            h = h * 1664525u + 1013904223u;
+            b[c][i] = ZERO ? (v4i){0, 0, 0, 0} : (v4i){(int)h, (int)(h * 11u), (int)(h * 13u), (int)(h * 17u)};
+        }
+    }
+    v16i acc[7][2];
+#pragma unroll
+    for (int w = 0; w < 7; ++w)
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int e = 0; e < 16; ++e) acc[w][c][e] = 0;
+    for (int k = 0; k < nk; ++k) {
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    acc[u + v][ct] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a[u], b[ct][v], acc[u + v][ct], 0, 0, 0);
+        if (!ZERO) {
+            const v4i kv = {k, k, k, k};
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                a[i] ^= kv;
+                b[0][i] += kv;
+                b[1][i] ^= kv;
+            }
+        }
+    }
+    int sum = 0;
+#pragma unroll
+    for (int w = 0; w < 7; ++w)
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int e = 0; e < 16; ++e) sum += acc[w][c][e];
+    out[blockIdx.x * 256 + threadIdx.x] = sum;
+}
+
+constexpr int MFMA_RATE_BLOCKS = 8192, MFMA_RATE_STEPS = 96;
+size_t mfma_rate_out_words() { return (size_t)MFMA_RATE_BLOCKS * 256; }
+double mfma_rate_ops() { return (double)MFMA_RATE_BLOCKS * 4 * MFMA_RATE_STEPS * 32 * 65536; }
+void launch_mfma_rate(int zero, int* out, uint32_t seed, hipStream_t s) {
+    if (zero) hipLaunchKernelGGL(mfma_rate_kernel<1>, dim3(MFMA_RATE_BLOCKS), dim3(256), 0, s, out, MFMA_RATE_STEPS, seed);
+    else hipLaunchKernelGGL(mfma_rate_kernel<0>, dim3(MFMA_RATE_BLOCKS), dim3(256), 0, s, out, MFMA_RATE_STEPS, seed);
 }
 
 // Candidates for the batched inner product, per column-tile count. All give
@@ -1453,7 +1520,7 @@ __global__ __launch_bounds__(256) void k1_split_kernel(const uint32_t* const* __
 // (double_pir.rs:157-177, 70-81): column shard*8 + lane, entry lane
 // lane0_mult*q + part, at se + q*se_qstride + (col*lpr + lane)*N.
 struct EpiArgs {
-    const uint32_t* outb;             // [q][shard][r][gpos][lane] reduced (k1b), or null
+    const uint32_t* outb;             // [q][shard][r][lane][gpos] reduced (k1b, k1d, k1w), or null
     const unsigned long long* acc;    // [shard][r][gpos][lane] partial sums (k1), or null
     size_t q_stride;                  // elements per query in outb
     const uint32_t* const* a17s;      // per shard: [lane][N]
@@ -1473,8 +1540,8 @@ __global__ __launch_bounds__(512) void epi_primary_kernel(EpiArgs A, const DevTa
         const int x = lx<true>(s);
         if (A.outb) {
             const uint32_t* o = A.outb + (size_t)q * A.q_stride + (size_t)sh * 2 * N * LANES;
-            w[0][x] = o[(size_t)g * LANES + lane];
-            w[1][x] = o[((size_t)N + g) * LANES + lane];
+            w[0][x] = o[(size_t)lane * N + g];
+            w[1][x] = o[((size_t)LANES + lane) * N + g];
         } else {
             w[0][x] = red64(A.acc[(((size_t)sh * 2 + 0) * N + g) * LANES + lane], P1, 26);
             w[1][x] = red64(A.acc[(((size_t)sh * 2 + 1) * N + g) * LANES + lane], P2, 25);
@@ -2175,7 +2242,7 @@ __global__ __launch_bounds__(256) void k1d_kernel(const uint32_t* const* __restr
             const int tot = acc[q][w] + __shfl_xor(acc[q][w], 32);
             t += (int64_t)tot * cw[w];
         }
-        if (h == 0) outb[(size_t)q * q_stride + (((size_t)sh * 2 + r) * N + half * HALFN + a) * LANES + lane8] = from_i64(t, p, K);
+        if (h == 0) outb[(size_t)q * q_stride + (((size_t)sh * 2 + r) * LANES + lane8) * N + half * HALFN + a] = from_i64(t, p, K);
     }
 }
 
@@ -2421,7 +2488,7 @@ __device__ __forceinline__ void k1w_body(const uint32_t* const* __restrict__ sla
             t += (int64_t)tot * cw[w];
         }
         if (h == 0)
-            outb[(size_t)q * q_stride + (((size_t)sh * 2 + R) * N + half * HALFN + a) * LANES + lane8] = from_i64(t, p, K);
+            outb[(size_t)q * q_stride + (((size_t)sh * 2 + R) * LANES + lane8) * N + half * HALFN + a] = from_i64(t, p, K);
     }
 }
 

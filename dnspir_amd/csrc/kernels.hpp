@@ -42,7 +42,7 @@ void launch_relayout(const uint32_t* rows_tmp, uint32_t* slab, int L, int nst, h
 // bytes of the query-limb scratch `ql` for nq queries
 size_t qlimb_bytes(int nq, int nst);
 // K1 on limb-block shards for nq <= 64 queries (qp [q][4][2][N] Galois order):
-// outb [q][shard][r][gpos][lane] reduced residues
+// outb [q][shard][r][lane][gpos] reduced residues
 // main_start (optional) is recorded right before the inner-product launch
 void launch_k1b(const uint32_t* const* slabs, int nshards, const uint32_t* qp, int nq, uint32_t* ql, uint32_t* outb, int nst,
                 hipStream_t s, hipEvent_t main_start = nullptr);
@@ -66,6 +66,12 @@ int k1b_get_choice(int ct, int nst);
 void launch_k1b_variant(int i, const uint32_t* const* slabs, int nshards, const uint32_t* ql, uint32_t* outb, int nst, int nq,
                         hipStream_t s);
 void launch_fill_hash(uint32_t* p, size_t n, uint32_t seed, hipStream_t s);
+// the batched inner product's MFMA mix on register operands (zero = 1: all-zero
+// digits, else hashed digits changing every step): out needs mfma_rate_out_words()
+// words; one launch performs mfma_rate_ops() int8 operations
+size_t mfma_rate_out_words();
+double mfma_rate_ops();
+void launch_mfma_rate(int zero, int* out, uint32_t seed, hipStream_t s);
 // n_total entries [n_total][N] u16 from n_valid buckets of 2560 bytes (the
 // rest zero): pack_bytes_into_coeffs, dns/bucket.rs:289-303
 void launch_pack_buckets(const uint8_t* buckets, size_t n_valid, size_t n_total, uint16_t* coeffs, hipStream_t s);

@@ -322,6 +322,16 @@ def autotune_batch(fleet, nq, ctx=None):
              "chosen": bool(e.chosen)} for e in ents[:cnt.value]]
 
 
+def probe_mfma_rate(zero_operands=False, reps=5, ctx=None):
+    """dnspir_probe_mfma_rate: int8 TOPS of the batched inner product's MFMA
+    mix on register operands (hashed digits, or all zero)."""
+    init()
+    t = ctypes.c_double()
+    check(lib().dnspir_probe_mfma_rate(ctx.h if ctx else None, 1 if zero_operands else 0, reps, ctypes.byref(t)),
+          "probe_mfma_rate")
+    return t.value
+
+
 def get_database_shape(num_entries):
     """pir_wrapper.rs:219-231; DnspirError(DNSPIR_E_SHAPE) where the reference
     asserts (0 entries, more than N^2)."""

@@ -243,6 +243,18 @@ typedef struct {
 } dnspir_tune_entry;
 int dnspir_autotune_batch(dnspir_ctx* ctx, const dnspir_shard* const* fleet, size_t len, int nq, dnspir_tune_entry* entries,
                           int max_entries, int* n_entries);
+/* Diagnostic for the batched inner product's roofline (no counterpart in the
+ * reference): runs its int8 MFMA instruction mix (14 accumulator tiles per
+ * wave, 32 v_mfma_i32_32x32x32_i8 per K-step, 8192 blocks of 4 waves, 96
+ * K-steps) with operands held in registers, so no memory traffic, on the
+ * context's stream, and returns the rate in int8 TOPS (the mean of `reps`
+ * launches after one warm-up). zero_operands = 0 uses hashed digits that change
+ * every step, as real shards and queries do; 1 keeps every digit zero. The
+ * clock the GPU holds under int8 MFMA load depends on how much the operand
+ * bits switch, so the first is the ceiling the batched kernels face on real
+ * data and the second the switching-free rate. Allocates 8 MiB of device
+ * memory for the call. */
+int dnspir_probe_mfma_rate(dnspir_ctx* ctx, int zero_operands, int reps, double* tops);
 /* get_database_shape (pir_wrapper.rs:219-231): index-group orders.
  * DNSPIR_E_SHAPE (outputs untouched) where the reference asserts: num_entries
  * of 0 or above N^2 (pir_wrapper.rs:221-222). */

@@ -207,3 +207,15 @@ def test_one_query_batches_two_stream_graph_replays(pir):
             assert got[k * rb:(k + 1) * rb] == want[0], (rep, k)
     assert ctx.set_graphs(True) is False
 
+
+
+def test_mfma_rate_probe(pir):
+    """The register-operand MFMA probe behind the bench's k1b ceiling: both
+    operand kinds give a rate within the nominal 5.0 POPS int8 peak."""
+    rnd = pir.probe_mfma_rate(False, reps=2)
+    zero = pir.probe_mfma_rate(True, reps=2)
+    _say(f"[batch] MFMA mix on register operands: {rnd:.0f} TOPS hashed, {zero:.0f} TOPS zero")
+    assert 100 < rnd < 5100 and 100 < zero < 5100
+    from dnspir_amd import DnspirError
+    with pytest.raises(DnspirError):
+        pir.probe_mfma_rate(False, reps=0)

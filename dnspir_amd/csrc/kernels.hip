@@ -10,6 +10,7 @@
 //                (the Galois scatter of base_pir.rs:1085-1247 made static)
 //   lane-major temporaries [stream][lane][row][...][r][slot] during encoding
 #include <atomic>
+#include <type_traits>
 
 #include "dn_common.hpp"
 #include "kernels.hpp"
@@ -84,6 +85,14 @@ __device__ __forceinline__ uint32_t red2p(uint32_t a, uint32_t p2) {  // [0, 4p)
 }
 __device__ __forceinline__ uint32_t shoup_lazy(uint32_t a, uint32_t w, uint32_t wq, uint32_t p) {  // -> [0, 2p)
     return a * w - __umulhi(a, wq) * p;
+}
+// Shoup companion floor(w 2^32 / p) of a constant w < p
+__device__ __forceinline__ uint32_t shoup_companion(uint32_t w, uint32_t p) {
+    return (uint32_t)(((uint64_t)w << 32) / p);
+}
+// [0, 8p) -> [0, p) (p < 2^26): three conditional subtractions
+__device__ __forceinline__ uint32_t red8p(uint32_t x, uint32_t p) {
+    return csub(csub(csub(x, 4 * p), 2 * p), p);
 }
 // forward CT butterfly (Harvey): inputs and outputs in [0, 4p)
 __device__ __forceinline__ void bf_ct(uint32_t& x, uint32_t& y, uint32_t w, uint32_t wq, uint32_t p) {
@@ -474,6 +483,72 @@ __global__ __launch_bounds__(K5R_NT) void k5_rows_kernel(uint32_t* __restrict__ 
     }
 }
 
+// k5_rows_kernel for l <= 256 with one wave per 64-slot row of the tile: the
+// DIF groups of a wave share their row indices, so every twiddle is
+// wave-uniform and comes from the scalar cache (psipow and its Shoup
+// companion), leaving LDS to the data. Same arithmetic as k5_rows_kernel.
+__global__ __launch_bounds__(K5R_NT) void k5_rows_w_kernel(uint32_t* __restrict__ M, int l, int logl, int kq, int conj,
+                                                           const DevTables* __restrict__ T) {
+    constexpr int TS = 64, NW = K5R_NT / 64;
+    extern __shared__ uint32_t tile[];  // [l][64]
+    const int s0 = blockIdx.x * TS, li = blockIdx.y;
+    uint32_t* base = M + (size_t)li * l * N;
+    const int tid = threadIdx.x, d = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lstep = LOGN + 1 - logl;  // 2N / l = 2^lstep
+    const int s = s0 + d;
+    const int z = (T->slot_exp[s] * kq) & (M2N - 1);
+    const int coset = z & ((1 << lstep) - 1);
+    for (int j = wave; j < l; j += NW) {
+        const int e = (M2N - ((coset * j) & (M2N - 1))) & (M2N - 1);
+        tile[j * TS + d] = mul_shoup(base[(size_t)j * N + s], T->psipow[0][e], T->psipowq[0][e], P1);
+    }
+    __syncthreads();
+    // DIF stages top .. top-R+1 for the groups of 2^R rows this wave owns
+    auto pass = [&](auto RR, int top) {
+        constexpr int R = decltype(RR)::value, G = 1 << R;
+        const int sh = top - R + 1;
+        for (int gi = wave; gi < (l >> R); gi += NW) {
+            const int jlow = gi & ((1 << sh) - 1), blk = gi >> sh;
+            const int b0 = (blk << (top + 1)) + jlow;
+            uint32_t v[G];
+#pragma unroll
+            for (int k = 0; k < G; ++k) v[k] = tile[(b0 + (k << sh)) * TS + d];
+#pragma unroll
+            for (int t = 0; t < R; ++t) {
+                const int lh = top - t, hk = G >> (t + 1);
+#pragma unroll
+                for (int k = 0; k < G; ++k) {
+                    if (k & hk) continue;
+                    // twiddle psi^-(2N / 2^(lh+1) * position) (the table dtw of k5_rows_kernel)
+                    const int pos = (b0 + (k << sh)) & ((1 << lh) - 1);
+                    const int e = (M2N - (((M2N >> (lh + 1)) * pos) & (M2N - 1))) & (M2N - 1);
+                    bf_gs(v[k], v[k + hk], T->psipow[0][e], T->psipowq[0][e], P1);
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < G; ++k) tile[(b0 + (k << sh)) * TS + d] = v[k];
+        }
+    };
+    int top = logl - 1;
+    while (top >= 2) {
+        pass(std::integral_constant<int, 3>(), top);
+        __syncthreads();
+        top -= 3;
+    }
+    if (top == 1) pass(std::integral_constant<int, 2>(), top);
+    else if (top == 0) pass(std::integral_constant<int, 1>(), top);
+    if (top >= 0) __syncthreads();
+    for (int i = wave; i < l; i += NW) {
+        int y;
+        if (conj && i >= N / 2) y = (z * T->pow5[i - N / 2]) & (M2N - 1);
+        else y = ((M2N - z) * T->pow5[i]) & (M2N - 1);
+        const int u = (((M2N - y) & (M2N - 1)) - coset) >> lstep;  // natural DFT index
+        const uint32_t v = tile[brv_bits(u, logl) * TS + d];
+        base[(size_t)i * N + s] = csub(v, P1);
+    }
+}
+
 // steps 4-6: final column c takes column j = rev(c) (step 5), gathers its
 // slots by sigma_{5^-j} (step 4), then inverse NTT, centered lift, times
 // l^-1 mod t, centered lift, embedding into p1 and p2 and forward NTTs.
@@ -583,6 +658,20 @@ __global__ __launch_bounds__(1024) void chain_kernel(const uint32_t* __restrict_
         cg1[k] = (uint32_t)(((uint64_t)cg1[k] + p - t26 * cg0[k] % p) % p);
         if (SEC) cb1[k] = (uint32_t)(((uint64_t)cb1[k] + p - t26 * cb0[k] % p) % p);
     }
+    // the update multiplies by these constants every step: Shoup companions
+    uint32_t cg0q[4], cg1q[4], cqaq[4], cb0q[4], cb1q[4], cqbq[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t p = mod_p(k & 1);
+        cg0q[k] = shoup_companion(cg0[k], p);
+        cg1q[k] = shoup_companion(cg1[k], p);
+        cqaq[k] = shoup_companion(cqa[k], p);
+        if (SEC) {
+            cb0q[k] = shoup_companion(cb0[k], p);
+            cb1q[k] = shoup_companion(cb1[k], p);
+            cqbq[k] = shoup_companion(cqb[k], p);
+        }
+    }
     sig_upd[0] = T->sigma5[tid];
     sig_upd[1] = T->sigma5[tid + 1024];
     int gpos_upd[2] = {T->slot_gpos[tid], T->slot_gpos[tid + 1024]};
@@ -690,11 +779,14 @@ __global__ __launch_bounds__(1024) void chain_kernel(const uint32_t* __restrict_
             const uint32_t p = mod_p(r);
             const int K = mod_k(r);
             const uint32_t d1 = xb[r][lx<true>(s)];  // NTT(D1), [0, 4p)
-            sa[r][s] = red64((uint64_t)sga[k] * cg0[k] + (uint64_t)d1 * cg1[k] + (uint64_t)te[k] * cqa[k], p, K);
+            // three Shoup products in [0, 2p) each
+            sa[r][s] = red8p(shoup_lazy(sga[k], cg0[k], cg0q[k], p) + shoup_lazy(d1, cg1[k], cg1q[k], p) +
+                                 shoup_lazy(te[k], cqa[k], cqaq[k], p),
+                             p);
             if (SEC) {
-                const uint64_t cv = (uint64_t)sc[cur][r][sig_upd[k >> 1]] + (uint64_t)sga[k] * cb0[k] +
-                                    (uint64_t)d1 * cb1[k] + (uint64_t)te[k] * cqb[k];
-                sc[cur ^ 1][r][s] = red64(cv, p, K);
+                const uint32_t cv = sc[cur][r][sig_upd[k >> 1]] + shoup_lazy(sga[k], cb0[k], cb0q[k], p) +
+                                    shoup_lazy(d1, cb1[k], cb1q[k], p) + shoup_lazy(te[k], cqb[k], cqbq[k], p);
+                sc[cur ^ 1][r][s] = red8p(cv, p);  // [0, 7p) -> [0, p)
             } else {
                 // IPGK rows and the TE row, Galois order: [arr][r][gpos];
                 // NTT(D0) = sigma_5(a) - 2^26 NTT(D1)
@@ -1613,7 +1705,9 @@ void launch_k5(const uint16_t* entries, const long long* ent_off, long long col_
     hipLaunchKernelGGL(k5_col_ntt_kernel, dim3(l, nlanes), dim3(256), 0, s, entries, ent_off, col_stride, M, l, ncols_nonzero, T,
                        clear, clear_per_block, clear_words);
     // tiles of 64 KiB: TS slots x l rows, TS * 4 bytes contiguous per row
-    if (l <= 512)
+    if (l <= 256)
+        hipLaunchKernelGGL(k5_rows_w_kernel, dim3(N / 64, nlanes), dim3(K5R_NT), (size_t)l * 64 * 4, s, M, l, logl, kq, conj, T);
+    else if (l <= 512)
         hipLaunchKernelGGL(k5_rows_kernel<32>, dim3(N / 32, nlanes), dim3(K5R_NT), (size_t)l * 32 * 4, s, M, l, logl, kq, conj, T);
     else if (l == 1024)
         hipLaunchKernelGGL(k5_rows_kernel<16>, dim3(N / 16, nlanes), dim3(K5R_NT), (size_t)l * 16 * 4, s, M, l, logl, kq, conj, T);
@@ -1637,16 +1731,18 @@ void launch_chain_sec(const uint32_t* te_tmp, uint32_t* a_out, uint32_t* c_out, 
 // c <- sigma_5(c) + sigma_5(a) cb0 + NTT(D1) cb1' + TE cqb mod p for slots
 // tid, tid + 1024 (the b-recurrence of the fused secondary inner product,
 // with NTT(D0) = sigma_5(a) - 2^26 NTT(D1) folded into cb1' = cb1 - 2^26 cb0)
+// (cq: the Shoup companions of the three constants [cb0, cb1, cqb][k])
 __device__ __forceinline__ void c_update(const uint32_t* __restrict__ c_in, uint32_t* __restrict__ c_out,
                                          const uint32_t* d1t, const uint32_t sga[2], const uint32_t te[2],
-                                         const uint32_t cb0[2], const uint32_t cb1[2], const uint32_t cqb[2], const int sig[2],
-                                         uint32_t p, int K, int tid) {
+                                         const uint32_t cb0[2], const uint32_t cb1[2], const uint32_t cqb[2],
+                                         const uint32_t cq[3][2], const int sig[2], uint32_t p, int tid) {
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
         const int s = tid + 1024 * k;
         const uint32_t d1 = d1t[lx<true>(s)];  // [0, 4p)
-        const uint64_t cv = (uint64_t)c_in[sig[k]] + (uint64_t)sga[k] * cb0[k] + (uint64_t)d1 * cb1[k] + (uint64_t)te[k] * cqb[k];
-        c_out[s] = red64(cv, p, K);
+        const uint32_t cv = c_in[sig[k]] + shoup_lazy(sga[k], cb0[k], cq[0][k], p) + shoup_lazy(d1, cb1[k], cq[1][k], p) +
+                            shoup_lazy(te[k], cqb[k], cq[2][k], p);
+        c_out[s] = red8p(cv, p);  // [0, 7p) -> [0, p)
     }
 }
 
@@ -1679,7 +1775,6 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
     const int st = chain / nlanes, li = chain % nlanes;
     const int tid = threadIdx.x;
     const uint32_t p = mod_p(r);
-    const int K = mod_k(r);
     for (int k = tid; k < N; k += 1024) {
         tw2[0][k] = make_uint2(T->psi[r][k], T->psiq[r][k]);
         tw2[1][k] = make_uint2(T->ipsi[r][k], T->ipsiq[r][k]);
@@ -1706,6 +1801,17 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
         const uint64_t t26 = (1ull << LOG2_B) % p;
         cg1[k] = (uint32_t)(((uint64_t)cg1[k] + p - t26 * cg0[k] % p) % p);
         cb1[k] = (uint32_t)(((uint64_t)cb1[k] + p - t26 * cb0[k] % p) % p);
+    }
+    // Shoup companions of the update constants
+    uint32_t cg0q[2], cg1q[2], cqaq[2], cq[3][2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        cg0q[k] = shoup_companion(cg0[k], p);
+        cg1q[k] = shoup_companion(cg1[k], p);
+        cqaq[k] = shoup_companion(cqa[k], p);
+        cq[0][k] = shoup_companion(cb0[k], p);
+        cq[1][k] = shoup_companion(cb1[k], p);
+        cq[2][k] = shoup_companion(cqb[k], p);
     }
     // phase A: threads 0..511, radix-4 group g = tid first (stages 10, 9)
     const int gA = tid & 511;
@@ -1767,17 +1873,17 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
         // while the partner's words travel: the c-update of the previous step
         // (its digit transforms are still in xd, its TE row in te)
         if (i > 0) {
-            c_update(sc[cur], sc[cur ^ 1], xd[0], sga, te, cb0, cb1, cqb, sig_upd, p, K, tid);
+            c_update(sc[cur], sc[cur ^ 1], xd[0], sga, te, cb0, cb1, cqb, cq, sig_upd, p, tid);
             cur ^= 1;
         }
         // this step's sigma_5(a) (sa is rewritten in D) and TE row, and the two
         // products of the a-update that do not need the digits
-        uint64_t pa[2];
+        uint32_t pa[2];  // [0, 4p)
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
             te[k] = tn[k];
             sga[k] = sa[sig_upd[k]];
-            pa[k] = (uint64_t)sga[k] * cg0[k] + (uint64_t)te[k] * cqa[k];
+            pa[k] = shoup_lazy(sga[k], cg0[k], cg0q[k], p) + shoup_lazy(te[k], cqa[k], cqaq[k], p);
         }
         // both words in one round trip, polled again until both carry this step's tag
         const uint64_t tag2 = (uint64_t)tag | ((uint64_t)tag << 32);
@@ -1845,11 +1951,11 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
         for (int k = 0; k < 2; ++k) {
             const int s = tid + 1024 * k;
             const uint32_t d1 = xd[0][lx<true>(s)];  // NTT(D1), [0, 4p)
-            sa[s] = red64(pa[k] + (uint64_t)d1 * cg1[k], p, K);
+            sa[s] = red8p(pa[k] + shoup_lazy(d1, cg1[k], cg1q[k], p), p);  // [0, 6p) -> [0, p)
         }
         lds_barrier();
     }
-    c_update(sc[cur], sc[cur ^ 1], xd[0], sga, te, cb0, cb1, cqb, sig_upd, p, K, tid);
+    c_update(sc[cur], sc[cur ^ 1], xd[0], sga, te, cb0, cb1, cqb, cq, sig_upd, p, tid);
     cur ^= 1;
     lds_barrier();
     uint32_t* ao = a_out + ((size_t)chain * 2 + r) * N;
@@ -1882,13 +1988,14 @@ constexpr int QUAD_NT = 512;
 __device__ __forceinline__ void c_update_half(const uint32_t* __restrict__ c_in, uint32_t* __restrict__ c_out,
                                               const uint32_t* d1t, int off, const uint32_t sga[2], const uint32_t te[2],
                                               const uint32_t cb0[2], const uint32_t cb1[2], const uint32_t cqb[2],
-                                              const int sig[2], uint32_t p, int K, int tid) {
+                                              const uint32_t cq[3][2], const int sig[2], uint32_t p, int tid) {
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
         const int sl = tid + QUAD_NT * k;
         const uint32_t d1 = d1t[lx<true>(off + sl)];  // [0, 4p)
-        const uint64_t cv = (uint64_t)c_in[sig[k]] + (uint64_t)sga[k] * cb0[k] + (uint64_t)d1 * cb1[k] + (uint64_t)te[k] * cqb[k];
-        c_out[sl] = red64(cv, p, K);
+        const uint32_t cv = c_in[sig[k]] + shoup_lazy(sga[k], cb0[k], cq[0][k], p) + shoup_lazy(d1, cb1[k], cq[1][k], p) +
+                            shoup_lazy(te[k], cqb[k], cq[2][k], p);
+        c_out[sl] = red8p(cv, p);  // [0, 7p) -> [0, p)
     }
 }
 
@@ -1921,7 +2028,6 @@ __global__ __launch_bounds__(QUAD_NT) void chain_quad_kernel(const uint32_t* __r
     const int st = chain / nlanes, li = chain % nlanes;
     const int tid = threadIdx.x;
     const uint32_t p = mod_p(r);
-    const int K = mod_k(r);
     const int off = h * HALFN;  // first slot (and transform element) of the half
     for (int k = tid; k < N; k += QUAD_NT) {
         tw2[0][k] = make_uint2(T->psi[r][k], T->psiq[r][k]);
@@ -1953,13 +2059,20 @@ __global__ __launch_bounds__(QUAD_NT) void chain_quad_kernel(const uint32_t* __r
         sig_upd[k] = T->sigma5[s] - off;  // within the half
         cb1[k] = (uint32_t)(((uint64_t)cb1[k] + p - t26 * cb0[k] % p) % p);
     }
+    uint32_t cq[3][2];  // Shoup companions of cb0, cb1, cqb
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        cq[0][k] = shoup_companion(cb0[k], p);
+        cq[1][k] = shoup_companion(cb1[k], p);
+        cq[2][k] = shoup_companion(cqb[k], p);
+    }
     // the transform passes: threads 0..255, radix-4 group g = 256 h + tid of the
     // full-length passes (the groups of half h)
     const int gq = off / 4 + (tid & 255);
     // the a-update runs inside the last forward pass: thread tid < 256 updates
     // local slots 4 tid .. 4 tid + 3, the elements of its last radix-4 group
     const int sd0 = 4 * (tid & 255);
-    uint32_t cg0[4], cg1[4], cqa[4];
+    uint32_t cg0[4], cg1[4], cqa[4], cg0q[4], cg1q[4], cqaq[4];
     int sig_d[4];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
@@ -1968,6 +2081,9 @@ __global__ __launch_bounds__(QUAD_NT) void chain_quad_kernel(const uint32_t* __r
         cg1[c] = (uint32_t)(((uint64_t)T->gka[1][r][s] + p - t26 * cg0[c] % p) % p);
         cqa[c] = qa[r * N + s];
         sig_d[c] = T->sigma5[s] - off;
+        cg0q[c] = shoup_companion(cg0[c], p);
+        cg1q[c] = shoup_companion(cg1[c], p);
+        cqaq[c] = shoup_companion(cqa[c], p);
     }
     int sigA[4];
 #pragma unroll
@@ -2038,7 +2154,7 @@ __global__ __launch_bounds__(QUAD_NT) void chain_quad_kernel(const uint32_t* __r
 #pragma unroll
         for (int k = 0; k < 2; ++k) own[k] = xa[lx<true>(off + tid + QUAD_NT * k)];
         if (i > 0) {
-            c_update_half(sc[cur], sc[cur ^ 1], xd[0], off, sga, te, cb0, cb1, cqb, sig_upd, p, K, tid);
+            c_update_half(sc[cur], sc[cur ^ 1], xd[0], off, sga, te, cb0, cb1, cqb, cq, sig_upd, p, tid);
             cur ^= 1;
         }
 #pragma unroll
@@ -2047,11 +2163,12 @@ __global__ __launch_bounds__(QUAD_NT) void chain_quad_kernel(const uint32_t* __r
             sga[k] = sa[sig_upd[k]];
         }
         // the two products of the a-update that do not need the digits
-        uint64_t pa[4];
+        uint32_t pa[4];  // [0, 4p)
         {
             const uint32_t tv[4] = {ted.x, ted.y, ted.z, ted.w};
 #pragma unroll
-            for (int c = 0; c < 4; ++c) pa[c] = (uint64_t)sa[sig_d[c]] * cg0[c] + (uint64_t)tv[c] * cqa[c];
+            for (int c = 0; c < 4; ++c)
+                pa[c] = shoup_lazy(sa[sig_d[c]], cg0[c], cg0q[c], p) + shoup_lazy(tv[c], cqa[c], cqaq[c], p);
         }
         // the other three roles' words for pairs j = tid, tid + 512 (one 8-byte
         // load per role): every poll reloads all pairs not yet tagged in one round trip
@@ -2128,14 +2245,15 @@ __global__ __launch_bounds__(QUAD_NT) void chain_quad_kernel(const uint32_t* __r
             bf_ct(v0, v1, wb.x, wb.y, p);
             bf_ct(v2, v3, wb.z, wb.w, p);
             xd[0][b4] = v0; xd[0][b4 + 1] = v1; xd[0][b4 + 2] = v2; xd[0][b4 + 3] = v3;  // NTT(D1), [0, 4p)
-            sa[sd0] = red64(pa[0] + (uint64_t)v0 * cg1[0], p, K);
-            sa[sd0 + 1] = red64(pa[1] + (uint64_t)v1 * cg1[1], p, K);
-            sa[sd0 + 2] = red64(pa[2] + (uint64_t)v2 * cg1[2], p, K);
-            sa[sd0 + 3] = red64(pa[3] + (uint64_t)v3 * cg1[3], p, K);
+            // [0, 6p) -> [0, p)
+            sa[sd0] = red8p(pa[0] + shoup_lazy(v0, cg1[0], cg1q[0], p), p);
+            sa[sd0 + 1] = red8p(pa[1] + shoup_lazy(v1, cg1[1], cg1q[1], p), p);
+            sa[sd0 + 2] = red8p(pa[2] + shoup_lazy(v2, cg1[2], cg1q[2], p), p);
+            sa[sd0 + 3] = red8p(pa[3] + shoup_lazy(v3, cg1[3], cg1q[3], p), p);
         }
         lds_barrier();
     }
-    c_update_half(sc[cur], sc[cur ^ 1], xd[0], off, sga, te, cb0, cb1, cqb, sig_upd, p, K, tid);
+    c_update_half(sc[cur], sc[cur ^ 1], xd[0], off, sga, te, cb0, cb1, cqb, cq, sig_upd, p, tid);
     cur ^= 1;
     lds_barrier();
     uint32_t* ao = a_out + ((size_t)chain * 2 + r) * N + off;

@@ -131,6 +131,15 @@ __device__ __forceinline__ int64_t center_q(uint64_t v) {
 // mod switch to `bits` bits, bfv.rs:51-58: (round(lift_c(x) 2^B / q) + 2^B) mod 2^B
 __device__ __forceinline__ uint32_t mod_switch_enc(uint64_t x, int bits) {
     int64_t c = center_q(x);
+    if (bits <= 12) {
+        // |c| 2^bits < 2^62 and |r| <= 2^(bits-1) + 1, so |r q| < 2^63: the
+        // exact remainder fits a 64-bit integer
+        const int64_t r0 = (int64_t)floor((double)c * ((double)(1ull << bits) / (double)Q) + 0.5);
+        const int64_t rem = c * ((int64_t)1 << bits) - r0 * (int64_t)Q;
+        const int64_t r = r0 + (2 * rem > (int64_t)Q ? 1 : 0) - (2 * rem < -(int64_t)Q ? 1 : 0);
+        const int64_t m = (int64_t)1 << bits;
+        return (uint32_t)((r + m) & (m - 1));
+    }
     double est = (double)c * ((double)(1ull << bits) / (double)Q);
     int64_t r = (int64_t)floor(est + 0.5);
     __int128 num = (__int128)c * ((__int128)1 << bits);

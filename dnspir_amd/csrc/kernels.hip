@@ -1647,11 +1647,9 @@ __global__ __launch_bounds__(512) void epi_primary_kernel(EpiArgs A, const DevTa
     }
     for (int s = tid; s < N; s += 512) {
         const int x = lx<true>(s);
-        uint32_t x1 = w[0][x], x2 = w[1][x];
-        x1 = csub(x1, P1);
-        x2 = csub(x2, P2);
-        const uint32_t e1 = mul_mod(x1, NINV_Q2INV_P1, P1, 26);
-        const uint32_t e2 = mul_mod(x2, NINV_Q1INV_P2, P2, 25);
+        // [0, 2p) -> CRT terms in [0, p) (Shoup products by the constants)
+        const uint32_t e1 = mul_shoup(w[0][x], NINV_Q2INV_P1, NINV_Q2INV_P1_SH, P1);
+        const uint32_t e2 = mul_shoup(w[1][x], NINV_Q1INV_P2, NINV_Q1INV_P2_SH, P2);
         uint64_t b = (uint64_t)e1 * P2 + (uint64_t)e2 * P1;
         b = b >= Q ? b - Q : b;
         if (A.bcoef) A.bcoef[((size_t)sh * LANES + lane) * N + s] = b;

@@ -703,6 +703,11 @@ def main():
 
     k1_ms = float(np.mean(k1))
     achieved = bytes_streamed / (k1_ms * 1e-3) / 1e9
+    # a plain streaming read of the same resident slabs on this GPU: what the
+    # HBM delivers to loads of this kind, beside the nominal 8 TB/s
+    read_gbs = pir.probe_read_rate(shards[:resident], reps=3, ctx=ctx)
+    log(f"[bench] plain streaming read of the {resident} resident slabs: {read_gbs:.0f} GB/s; "
+        f"inner product {achieved:.0f} GB/s")
     tag = "2p22" if args.entries == 1 << 22 else f"e{args.entries}"
     traffic = read_traffic(tag)
 
@@ -798,6 +803,10 @@ def main():
                            "ref_bytes_per_launch"),
             "ref_bytes_per_launch": ref_bytes,
             "ref_equivalent_gbs": round(ref_bytes / (k1_ms * 1e-3) / 1e9, 1),
+            "streaming_read_gbs": round(read_gbs, 1),
+            "frac_of_streaming_read": round(achieved / read_gbs, 4),
+            "streaming_read_note": ("a plain read of the resident shard slabs with the same 16-byte nontemporal "
+                                    "loads, timed on this GPU in the same run (dnspir_probe_read_rate)"),
         },
         "cpu_baseline": cpu,
         "small_configurations": small,

@@ -152,6 +152,7 @@ SIGNATURES = {
     "dnspir_key_bytes": (sz, []),
     "dnspir_autotune_batch": (ci, [vp, vp, sz, ci, ctypes.POINTER(TuneEntry), ci, ip]),
     "dnspir_probe_mfma_rate": (ci, [vp, ci, ci, ctypes.POINTER(ctypes.c_double)]),
+    "dnspir_probe_read_rate": (ci, [vp, vp, sz, ci, ctypes.POINTER(ctypes.c_double)]),
     "dnspir_batch_scratch_bytes": (sz, [ci, sz, ci]),
     "dnspir_server_create": (ci, [ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]),
     "dnspir_server_add_database": (ci, [vp, vp, sz, ip]),

@@ -1672,6 +1672,46 @@ int dnspir_probe_mfma_rate(dnspir_ctx* c, int zero_operands, int reps, double* t
     return 0;
 }
 
+int dnspir_probe_read_rate(dnspir_ctx* c, const dnspir_shard* const* fleet, size_t len, int reps, double* gbs) {
+    if (!gbs || reps < 1 || !fleet || len == 0 || len > 65535) return DNSPIR_E_ARG;
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (!c) c = default_ctx();
+    for (size_t k = 0; k < len; ++k)
+        if (!fleet[k] || !fleet[k]->has_db) return DNSPIR_E_ARG;
+    std::vector<const char*> hp(len);
+    std::vector<size_t> hb(len);
+    double total = 0;
+    for (size_t k = 0; k < len; ++k) {
+        hp[k] = reinterpret_cast<const char*>(fleet[k]->slab);
+        hb[k] = slab_bytes(fleet[k]->l);
+        total += (double)hb[k];
+    }
+    std::lock_guard<std::mutex> lk(c->mu);
+    const hipStream_t s = c->stream;
+    char* dev = nullptr;
+    const size_t outw = read_rate_out_words((int)len);
+    HIP_OK(hipMalloc(&dev, len * (sizeof(char*) + sizeof(size_t)) + outw * 4));
+    const char** dptr = reinterpret_cast<const char**>(dev);
+    size_t* dbytes = reinterpret_cast<size_t*>(dev + len * sizeof(char*));
+    uint32_t* out = reinterpret_cast<uint32_t*>(dev + len * (sizeof(char*) + sizeof(size_t)));
+    hipEvent_t e0 = c->ev[3], e1 = c->ev[4];
+    hipError_t e = hipMemcpyAsync(dptr, hp.data(), len * sizeof(char*), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(dbytes, hb.data(), len * sizeof(size_t), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) launch_read_rate(dptr, dbytes, (int)len, out, s);  // warm-up
+    if (e == hipSuccess) e = hipEventRecord(e0, s);
+    for (int k = 0; k < reps && e == hipSuccess; ++k) launch_read_rate(dptr, dbytes, (int)len, out, s);
+    if (e == hipSuccess) e = hipEventRecord(e1, s);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    (void)hipFree(dev);
+    HIP_OK(e);
+    *gbs = ms > 0 ? total * reps / (ms * 1e-3) / 1e9 : 0;
+    return 0;
+}
+
 // ---- client ------------------------------------------------------------------
 struct dnspir_rng {
     dnh::ChaCha12 r;

@@ -1381,6 +1381,32 @@ __global__ __launch_bounds__(256) void mfma_rate_kernel(int* __restrict__ out, i
     out[blockIdx.x * 256 + threadIdx.x] = sum;
 }
 
+// Diagnostic: a plain streaming read of device ranges (16-byte nontemporal
+// loads, as the K1 kernels issue them), for the HBM rate the inner products
+// are compared with. Block (x, y) reads chunk x of range y; each thread folds
+// what it reads into one word so the loads are kept.
+__global__ __launch_bounds__(256) void read_rate_kernel(const char* const* __restrict__ ptrs, const size_t* __restrict__ bytes,
+                                                        uint32_t* __restrict__ out) {
+    const size_t n16 = bytes[blockIdx.y] / 16;
+    const gv4i* src = (const gv4i*)ptrs[blockIdx.y];
+    const size_t per = (n16 + gridDim.x - 1) / gridDim.x;
+    const size_t b = (size_t)blockIdx.x * per, e = min(n16, b + per);
+    v4i acc = {0, 0, 0, 0};
+    size_t i = b + threadIdx.x;
+    for (; i + 3 * 256 < e; i += 4 * 256) {
+        const v4i x0 = __builtin_nontemporal_load(src + i), x1 = __builtin_nontemporal_load(src + i + 256);
+        const v4i x2 = __builtin_nontemporal_load(src + i + 512), x3 = __builtin_nontemporal_load(src + i + 768);
+        acc ^= x0 ^ x1 ^ x2 ^ x3;
+    }
+    for (; i < e; i += 256) acc ^= __builtin_nontemporal_load(src + i);
+    out[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * 256 + threadIdx.x] = (uint32_t)(acc.x ^ acc.y ^ acc.z ^ acc.w);
+}
+constexpr int READ_RATE_CHUNKS = 512;
+size_t read_rate_out_words(int nranges) { return (size_t)nranges * READ_RATE_CHUNKS * 256; }
+void launch_read_rate(const char* const* ptrs, const size_t* bytes, int nranges, uint32_t* out, hipStream_t s) {
+    hipLaunchKernelGGL(read_rate_kernel, dim3(READ_RATE_CHUNKS, nranges), dim3(256), 0, s, ptrs, bytes, out);
+}
+
 constexpr int MFMA_RATE_BLOCKS = 8192, MFMA_RATE_STEPS = 96;
 size_t mfma_rate_out_words() { return (size_t)MFMA_RATE_BLOCKS * 256; }
 double mfma_rate_ops() { return (double)MFMA_RATE_BLOCKS * 4 * MFMA_RATE_STEPS * 32 * 65536; }

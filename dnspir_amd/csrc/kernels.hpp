@@ -72,6 +72,10 @@ void launch_fill_hash(uint32_t* p, size_t n, uint32_t seed, hipStream_t s);
 size_t mfma_rate_out_words();
 double mfma_rate_ops();
 void launch_mfma_rate(int zero, int* out, uint32_t seed, hipStream_t s);
+// a streaming read of nranges device ranges (ptrs, bytes: device arrays of
+// nranges entries; bytes multiples of 16); out needs read_rate_out_words(nranges) words
+size_t read_rate_out_words(int nranges);
+void launch_read_rate(const char* const* ptrs, const size_t* bytes, int nranges, uint32_t* out, hipStream_t s);
 // n_total entries [n_total][N] u16 from n_valid buckets of 2560 bytes (the
 // rest zero): pack_bytes_into_coeffs, dns/bucket.rs:289-303
 void launch_pack_buckets(const uint8_t* buckets, size_t n_valid, size_t n_total, uint16_t* coeffs, hipStream_t s);

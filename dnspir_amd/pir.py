@@ -332,6 +332,16 @@ def probe_mfma_rate(zero_operands=False, reps=5, ctx=None):
     return t.value
 
 
+def probe_read_rate(fleet, reps=3, ctx=None):
+    """dnspir_probe_read_rate: GB/s of a plain streaming read of the fleet's
+    shard slabs (the same 16-byte loads as the inner products)."""
+    init()
+    f, n = _fleet(fleet)
+    t = ctypes.c_double()
+    check(lib().dnspir_probe_read_rate(ctx.h if ctx else None, f, n, reps, ctypes.byref(t)), "probe_read_rate")
+    return t.value
+
+
 def get_database_shape(num_entries):
     """pir_wrapper.rs:219-231; DnspirError(DNSPIR_E_SHAPE) where the reference
     asserts (0 entries, more than N^2)."""

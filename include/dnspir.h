@@ -255,6 +255,13 @@ int dnspir_autotune_batch(dnspir_ctx* ctx, const dnspir_shard* const* fleet, siz
  * data and the second the switching-free rate. Allocates 8 MiB of device
  * memory for the call. */
 int dnspir_probe_mfma_rate(dnspir_ctx* ctx, int zero_operands, int reps, double* tops);
+/* Diagnostic for the single-query inner product's roofline (no counterpart in
+ * the reference): a plain streaming read, with the 16-byte loads the K1
+ * kernels issue, of every listed shard's slab (dnspir_required_bytes less the
+ * cached a-part), `reps` times after one warm-up, on the context's stream;
+ * returns GB/s. Shards must hold a database; a repeated shard is read again.
+ * Allocates a few MiB of device memory for the call. */
+int dnspir_probe_read_rate(dnspir_ctx* ctx, const dnspir_shard* const* fleet, size_t len, int reps, double* gbs);
 /* get_database_shape (pir_wrapper.rs:219-231): index-group orders.
  * DNSPIR_E_SHAPE (outputs untouched) where the reference asserts: num_entries
  * of 0 or above N^2 (pir_wrapper.rs:221-222). */

@@ -219,3 +219,15 @@ def test_mfma_rate_probe(pir):
     from dnspir_amd import DnspirError
     with pytest.raises(DnspirError):
         pir.probe_mfma_rate(False, reps=0)
+
+
+def test_read_rate_probe(pir):
+    """The streaming-read probe behind the bench's HBM ceiling: a rate within
+    the nominal 8 TB/s; fleets without a database are refused."""
+    g, _, _ = _fleet(pir, 1024, 3)
+    gbs = pir.probe_read_rate(g, reps=2)
+    _say(f"[batch] streaming read of 3 half slabs: {gbs:.0f} GB/s")
+    assert 100 < gbs < 8200
+    from dnspir_amd import DnspirError
+    with pytest.raises(DnspirError):
+        pir.probe_read_rate([pir.PIRDatabase(1024)], reps=1)

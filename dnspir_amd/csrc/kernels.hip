@@ -94,6 +94,16 @@ __device__ __forceinline__ uint32_t shoup_companion(uint32_t w, uint32_t p) {
 __device__ __forceinline__ uint32_t red8p(uint32_t x, uint32_t p) {
     return csub(csub(csub(x, 4 * p), 2 * p), p);
 }
+// The top balanced base-2^26 digit D1 of the centered lift of the coefficient
+// whose CRT terms are e1 = c (q/p1)^-1 mod p1 and e2 = c (q/p2)^-1 mod p2 (both
+// reduced), base_pir.rs:544-570: x = e1 p2 + e2 p1 mod q, centered; |x| < q/2 <
+// 2^50, so the second digit is hi = (x + 2^25) >> 26 itself (the third is 0).
+__device__ __forceinline__ int32_t crt_digit1(uint32_t e1, uint32_t e2) {
+    uint64_t v = (uint64_t)e1 * P2 + (uint64_t)e2 * P1;
+    v = v >= Q ? v - Q : v;
+    const int64_t x = center_q(v);
+    return (int32_t)((x + (1ll << (LOG2_B - 1))) >> LOG2_B);  // |d1| <= 2^24
+}
 // forward CT butterfly (Harvey): inputs and outputs in [0, 4p)
 __device__ __forceinline__ void bf_ct(uint32_t& x, uint32_t& y, uint32_t w, uint32_t wq, uint32_t p) {
     const uint32_t p2 = 2 * p;
@@ -734,12 +744,7 @@ __global__ __launch_bounds__(1024) void chain_kernel(const uint32_t* __restrict_
         for (int k = 0; k < 2; ++k) {
             const int j = tid + 1024 * k;
             const int xj = lx<true>(j);
-            const uint32_t e1 = ecrt[k][0], e2 = ecrt[k][1];
-            uint64_t v = (uint64_t)e1 * P2 + (uint64_t)e2 * P1;
-            v = v >= Q ? v - Q : v;
-            const int64_t x = center_q(v);
-            // |x| < q/2 < 2^50: the second balanced digit is hi itself (the third is 0)
-            const int32_t d1 = (int32_t)((x + (1ll << (LOG2_B - 1))) >> LOG2_B);  // |d1| <= 2^24
+            const int32_t d1 = crt_digit1(ecrt[k][0], ecrt[k][1]);
             uint32_t t = (uint32_t)(d1 + (int32_t)P1);
             xb[0][xj] = csub(t, P1);
             t = (uint32_t)(d1 + (int32_t)P2);
@@ -1945,12 +1950,7 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
         for (int k = 0; k < 2; ++k) {
             const int j = tid + 1024 * k;
             const int xj = lx<true>(j);
-            const uint32_t e1 = ecrt[k][0], e2 = ecrt[k][1];
-            uint64_t v = (uint64_t)e1 * P2 + (uint64_t)e2 * P1;
-            v = v >= Q ? v - Q : v;
-            const int64_t x = center_q(v);
-            // |x| < q/2 < 2^50: the second balanced digit is hi itself (the third is 0)
-            const int32_t d1 = (int32_t)((x + (1ll << (LOG2_B - 1))) >> LOG2_B);  // |d1| <= 2^24
+            const int32_t d1 = crt_digit1(ecrt[k][0], ecrt[k][1]);
             const uint32_t t = (uint32_t)(d1 + (int32_t)p);
             xd[0][xj] = csub(t, p);
         }
@@ -2023,15 +2023,9 @@ __device__ __forceinline__ void c_update_half(const uint32_t* __restrict__ c_in,
     }
 }
 
-// D1 mod p of the coefficient c whose CRT terms are e1 = c (q/p1)^-1 mod p1
-// and e2 = c (q/p2)^-1 mod p2 (both reduced): base_pir.rs:544-570
+// crt_digit1 mod p
 __device__ __forceinline__ uint32_t digit1_mod(uint32_t e1, uint32_t e2, uint32_t p) {
-    uint64_t v = (uint64_t)e1 * P2 + (uint64_t)e2 * P1;
-    v = v >= Q ? v - Q : v;
-    const int64_t x = center_q(v);
-    // |x| < q/2 < 2^50: the second balanced digit is hi itself (the third is 0)
-    const int32_t d1 = (int32_t)((x + (1ll << (LOG2_B - 1))) >> LOG2_B);  // |d1| <= 2^24
-    const uint32_t t = (uint32_t)(d1 + (int32_t)p);
+    const uint32_t t = (uint32_t)(crt_digit1(e1, e2) + (int32_t)p);
     return csub(t, p);
 }
 

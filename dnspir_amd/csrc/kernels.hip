@@ -350,11 +350,11 @@ __device__ __forceinline__ void fwd_pass2_lastw(uint32_t* buf, const uint2* tw2,
         buf[lx<PAD>(2 * i + 1)] = y;
     }
 }
-// whole-block transforms by 256 threads (g = thread index) on an LDS array;
-// inputs in [0, 2p), outputs in [0, p) (forward) / [0, 2p) (inverse, which
-// leaves the N factor in)
+// whole-block forward transform by 256 threads (g = thread index) on an LDS
+// array, inputs in [0, 2p), outputs left in [0, 4p) (callers reduce them as
+// they write them out); the inverse leaves the N factor in, outputs in [0, 2p)
 template <bool PAD = false>
-__device__ __forceinline__ void ntt256_fwd(uint32_t* buf, const uint32_t* tw, const uint32_t* twq, uint32_t p, int g) {
+__device__ __forceinline__ void ntt256_fwd_lazy(uint32_t* buf, const uint32_t* tw, const uint32_t* twq, uint32_t p, int g) {
 #pragma unroll
     for (int P = 0; P < 3; ++P) {
         fwd_pass8<PAD>(buf, tw, twq, p, P, g);
@@ -362,13 +362,9 @@ __device__ __forceinline__ void ntt256_fwd(uint32_t* buf, const uint32_t* tw, co
     }
     fwd_pass4<PAD>(buf, tw, twq, p, g);
     __syncthreads();
-    for (int j = g; j < N; j += 256) {
-        const int x = lx<PAD>(j);
-        uint32_t v = red2p(buf[x], 2 * p);
-        buf[x] = csub(v, p);
-    }
-    __syncthreads();
 }
+// [0, 4p) -> [0, p)
+__device__ __forceinline__ uint32_t red4p(uint32_t x, uint32_t p) { return csub(csub(x, 2 * p), p); }
 template <bool PAD = false>
 __device__ __forceinline__ void ntt256_inv_unscaled(uint32_t* buf, const uint32_t* tw, const uint32_t* twq, uint32_t p, int g) {
     inv_pass4<PAD>(buf, tw, twq, p, g);
@@ -408,8 +404,8 @@ __global__ __launch_bounds__(256) void k5_col_ntt_kernel(const uint16_t* __restr
     const uint16_t* src = entries + ent_off[li] + (long long)j * col_stride;
     for (int s = threadIdx.x; s < N; s += blockDim.x) a[lx<true>(s)] = src[s];
     __syncthreads();
-    ntt256_fwd<true>(a, T->psi[0], T->psiq[0], P1, threadIdx.x);
-    for (int s = threadIdx.x; s < N; s += blockDim.x) out[s] = a[lx<true>(s)];
+    ntt256_fwd_lazy<true>(a, T->psi[0], T->psiq[0], P1, threadIdx.x);
+    for (int s = threadIdx.x; s < N; s += blockDim.x) out[s] = red4p(a[lx<true>(s)], P1);
 }
 
 // step 3 (coset_ntt, base_pir.rs:397-444) for a tile of TS slots of one lane,
@@ -591,14 +587,14 @@ __global__ __launch_bounds__(256) void k5_cols_final_kernel(const uint32_t* __re
         a[1][xs] = y < 0 ? (uint32_t)(y + (int32_t)P2) : (uint32_t)y;
     }
     __syncthreads();
-    ntt256_fwd<true>(a[0], T->psi[0], T->psiq[0], P1, tid);
-    ntt256_fwd<true>(a[1], T->psi[1], T->psiq[1], P2, tid);
+    ntt256_fwd_lazy<true>(a[0], T->psi[0], T->psiq[0], P1, tid);
+    ntt256_fwd_lazy<true>(a[1], T->psi[1], T->psiq[1], P2, tid);
     const int st = c < L ? 0 : 1;
     const int row = c < L ? c : c - L;
     uint32_t* dst = te_tmp + (((size_t)(st * nlanes + li) * L + row) * 2) * N;
-    for (int s = tid; s < N; s += nt) {
-        dst[s] = a[0][lx<true>(s)];
-        dst[N + s] = a[1][lx<true>(s)];
+    for (int s = tid; s < N; s += nt) {  // [0, 4p) -> [0, p)
+        dst[s] = red4p(a[0][lx<true>(s)], P1);
+        dst[N + s] = red4p(a[1][lx<true>(s)], P2);
     }
 }
 

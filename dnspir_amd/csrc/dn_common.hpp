@@ -114,16 +114,7 @@ __device__ __forceinline__ uint32_t from_i64(int64_t v, uint32_t p, int K) {
     return r == 0 ? 0 : p - r;
 }
 
-constexpr uint32_t Q2INV_P1 = 33560578u;  // p2^-1 mod p1
-constexpr uint32_t Q1INV_P2 = 16771072u;  // p1^-1 mod p2
 
-// CRT of (x1 mod p1, x2 mod p2) to [0, q), base_pir.rs:917-933 / 544-551
-__device__ __forceinline__ uint64_t crt(uint32_t x1, uint32_t x2) {
-    uint32_t e1 = mul_mod(x1, Q2INV_P1, P1, 26);
-    uint32_t e2 = mul_mod(x2, Q1INV_P2, P2, 25);
-    uint64_t v = (uint64_t)e1 * P2 + (uint64_t)e2 * P1;
-    return v >= Q ? v - Q : v;
-}
 // centered lift of a residue mod q: [-(q-1)/2, (q-1)/2]
 __device__ __forceinline__ int64_t center_q(uint64_t v) {
     return v > Q / 2 ? (int64_t)v - (int64_t)Q : (int64_t)v;

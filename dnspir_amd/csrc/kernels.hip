@@ -47,31 +47,6 @@ __device__ void ntt_fwd(uint32_t* a, const DevTables* __restrict__ T, int r, int
     }
 }
 
-// inverse negacyclic NTT (Gentleman-Sande): slot order -> coefficients
-__device__ void ntt_inv(uint32_t* a, const DevTables* __restrict__ T, int r, int tid, int nt) {
-    const uint32_t p = mod_p(r);
-    const uint32_t* ipsi = T->ipsi[r];
-    const uint32_t* ipsiq = T->ipsiq[r];
-#pragma unroll 1
-    for (int s = LOGN - 1; s >= 0; --s) {
-        const int m = 1 << s;
-        const int lt = LOGN - 1 - s;
-        const int t = 1 << lt;
-        for (int b = tid; b < N / 2; b += nt) {
-            const int i = b >> lt;
-            const int j = (i << (lt + 1)) + (b & (t - 1));
-            const uint32_t U = a[j];
-            const uint32_t V = a[j + t];
-            a[j] = add_mod(U, V, p);
-            a[j + t] = mul_shoup(sub_mod(U, V, p), ipsi[m + i], ipsiq[m + i], p);
-        }
-        __syncthreads();
-    }
-    const uint32_t ni = T->ninv[r], niq = T->ninvq[r];
-    for (int j = tid; j < N; j += nt) a[j] = mul_shoup(a[j], ni, niq, p);
-    __syncthreads();
-}
-
 __device__ __forceinline__ int brv_bits(int v, int bits) {
     return bits == 0 ? 0 : (int)(__brev((unsigned)v) >> (32 - bits));
 }
@@ -300,22 +275,6 @@ __device__ __forceinline__ void fwd_pass8i(uint32_t* buf, const uint2* tw2, uint
     bf_ct(v[6], v[7], w3b.z, w3b.w, p);
 #pragma unroll
     for (int c = 0; c < 8; ++c) buf[lx<PAD>(j0 + c * q)] = v[c];
-}
-template <bool PAD = false>
-__device__ __forceinline__ void fwd_pass4i(uint32_t* buf, const uint2* tw2, uint32_t p, int g) {
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-        const int i = g + 256 * k;
-        const int b4 = lx<PAD>(4 * i);
-        uint32_t v0 = buf[b4], v1 = buf[b4 + 1], v2 = buf[b4 + 2], v3 = buf[b4 + 3];
-        const uint2 wa = tw2[512 + i];
-        bf_ct(v0, v2, wa.x, wa.y, p);
-        bf_ct(v1, v3, wa.x, wa.y, p);
-        const uint4 wb = *reinterpret_cast<const uint4*>(tw2 + 1024 + 2 * i);
-        bf_ct(v0, v1, wb.x, wb.y, p);
-        bf_ct(v2, v3, wb.z, wb.w, p);
-        buf[b4] = v0; buf[b4 + 1] = v1; buf[b4 + 2] = v2; buf[b4 + 3] = v3;
-    }
 }
 // fwd_pass4i with the groups of wave w (g = thread index of 256) in elements
 // [512w, 512w + 512), where the radix-8 passes at strides 32 and 4 leave them

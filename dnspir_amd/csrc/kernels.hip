@@ -2337,6 +2337,102 @@ __global__ __launch_bounds__(256) void k1d_kernel(const uint32_t* const* __restr
     }
 }
 
+// k1d for a fleet of one shard: one block per (a-group, r, half) whose four
+// waves each take a quarter of the K-steps, summed through LDS at the end.
+// Four times the waves of k1d_kernel (4096 instead of 1024), so enough loads
+// are in flight to stream at HBM rate; the integer sums are the same, so are
+// the results.
+template <int NQ, int PD>
+__global__ __launch_bounds__(256) void k1d_split_kernel(const uint32_t* const* __restrict__ slabs, const uint32_t* __restrict__ ql,
+                                                        uint32_t* __restrict__ outb, int nst, size_t q_stride) {
+    __shared__ u32x4 qs[6 * 32][NQ][4][2];
+    const int S = 3 * nst;
+    const int half = blockIdx.y;
+    const int r = blockIdx.z & 1, sh = blockIdx.z >> 1;
+    const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63, h = l >> 5;
+    const int CT = 1;
+    const u32x4* Q = reinterpret_cast<const u32x4*>(ql) + ((size_t)(r * 2 + half) * S) * 32 * CT * 256;
+    for (int it = tid; it < S * 32 * NQ * 4 * 2; it += 256) {
+        const int hh = it & 1, v = (it >> 1) & 3, q = (it >> 3) % NQ, kc = it / (8 * NQ);
+        qs[kc][q][v][hh] = Q[(size_t)kc * CT * 256 + v * 64 + hh * 32 + q];
+    }
+    __syncthreads();
+    const int g = blockIdx.x;
+    const ARow arow = a_row(slabs[sh], r, half, g, S);
+    const int nk = S * 32, kq = nk / 4;  // S * 32 is a multiple of 4
+    const int k0 = wave * kq, k1 = k0 + kq;
+    int acc[NQ][7];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+        for (int w = 0; w < 7; ++w) acc[q][w] = 0;
+    ATile abuf[PD];
+#pragma unroll
+    for (int d = 0; d < PD; ++d) load_a(abuf[d], arow, r, l, k0 + d);
+    for (int kk0 = k0; kk0 < k1; kk0 += PD) {
+#pragma unroll
+        for (int d = 0; d < PD; ++d) {
+            const int kk = kk0 + d;
+            v4i a[4];
+            a[0] = abuf[d].d[0];
+            a[1] = abuf[d].d[1];
+            a[2] = abuf[d].d[2];
+            a[3] = r == 0 ? top_digits(abuf[d].top) : top_digits1(abuf[d].top);
+            if (kk + PD < k1) load_a(abuf[d], arow, r, l, kk + PD);
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                u32x4 b[4];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) b[v] = qs[kk][q][v][h];
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        int t = acc[q][u + v];
+                        t = __builtin_amdgcn_sdot4(a[u].x, (int)b[v].x, t, false);
+                        t = __builtin_amdgcn_sdot4(a[u].y, (int)b[v].y, t, false);
+                        t = __builtin_amdgcn_sdot4(a[u].z, (int)b[v].z, t, false);
+                        t = __builtin_amdgcn_sdot4(a[u].w, (int)b[v].w, t, false);
+                        acc[q][u + v] = t;
+                    }
+            }
+        }
+    }
+    // waves 1..3 hand their partial sums to wave 0 (|sum| < 2^29 in total)
+    __syncthreads();  // every wave is done reading qs
+    int* part = reinterpret_cast<int*>(&qs[0][0][0][0]);  // [3][NQ][7][64]
+    if (wave > 0) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+#pragma unroll
+            for (int w = 0; w < 7; ++w) part[(((wave - 1) * NQ + q) * 7 + w) * 64 + l] = acc[q][w];
+    }
+    __syncthreads();
+    if (wave > 0) return;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q)
+#pragma unroll
+        for (int w = 0; w < 7; ++w)
+            acc[q][w] += part[(q * 7 + w) * 64 + l] + part[((NQ + q) * 7 + w) * 64 + l] + part[((2 * NQ + q) * 7 + w) * 64 + l];
+    const uint32_t p = mod_p(r);
+    const int K = mod_k(r);
+    uint32_t cw[7];
+    weight_consts(p, cw);
+    const uint32_t* corr = qcorr_ptr(ql, S, CT) + (size_t)(r * 2 + half) * CT * 32;
+    const int row = l & 31;
+    const int a = g * 4 + (row >> 3), lane8 = row & 7;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        int64_t t = corr[q];
+#pragma unroll
+        for (int w = 0; w < 7; ++w) {
+            const int tot = acc[q][w] + __shfl_xor(acc[q][w], 32);
+            t += (int64_t)tot * cw[w];
+        }
+        if (h == 0) outb[(size_t)q * q_stride + (((size_t)sh * 2 + r) * LANES + lane8) * N + half * HALFN + a] = from_i64(t, p, K);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Two-stream K1 for one or two queries (limb-block shards). The key-switch
 // digits satisfy NTT(D0_i) = sigma(a_i) - 2^26 NTT(D1_i) mod p_r
@@ -2638,6 +2734,10 @@ size_t qlimb_bytes(int nq, int nst) {
     return (size_t)2 * 2 * 3 * nst * 32 * CT * 4 * 1024 + (size_t)4 * CT * 32 * 4;
 }
 
+// fleets of up to this many shards take k1d_split_kernel for one or two queries
+// (one full shard: 105 instead of 160 us; from two shards on k1d_kernel has
+// enough waves and is as fast or faster)
+constexpr int K1D_SPLIT_MAX_SHARDS = 1;
 void launch_k1b(const uint32_t* const* slabs, int nshards, const uint32_t* qp, int nq, uint32_t* ql, uint32_t* outb, int nst,
                 hipStream_t s, hipEvent_t main_start) {
     const int CT = nq > 32 ? 2 : 1;
@@ -2646,7 +2746,12 @@ void launch_k1b(const uint32_t* const* slabs, int nshards, const uint32_t* qp, i
     hipLaunchKernelGGL(qcorr_kernel, dim3(4, CT * 32), dim3(256), 0, s, qp, ql + (size_t)4 * S * 32 * CT * 1024, nq, nst, CT * 32);
     if (main_start) (void)hipEventRecord(main_start, s);
     const size_t qstride = (size_t)nshards * 2 * N * LANES;
-    if (nq == 1)
+    const bool split = nshards <= K1D_SPLIT_MAX_SHARDS;
+    if (nq == 1 && split)
+        hipLaunchKernelGGL((k1d_split_kernel<1, 2>), dim3(HALFN / 4, 2, nshards * 2), dim3(256), 0, s, slabs, ql, outb, nst, qstride);
+    else if (nq == 2 && split)
+        hipLaunchKernelGGL((k1d_split_kernel<2, 2>), dim3(HALFN / 4, 2, nshards * 2), dim3(256), 0, s, slabs, ql, outb, nst, qstride);
+    else if (nq == 1)
         hipLaunchKernelGGL((k1d_kernel<1, 2>), dim3(HALFN / 16, 2, nshards * 2), dim3(256), 0, s, slabs, ql, outb, nst, qstride);
     else if (nq == 2)
         hipLaunchKernelGGL((k1d_kernel<2, 2>), dim3(HALFN / 16, 2, nshards * 2), dim3(256), 0, s, slabs, ql, outb, nst, qstride);

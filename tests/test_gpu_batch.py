@@ -231,3 +231,18 @@ def test_read_rate_probe(pir):
     from dnspir_amd import DnspirError
     with pytest.raises(DnspirError):
         pir.probe_read_rate([pir.PIRDatabase(1024)], reps=1)
+
+
+def test_two_query_batch_on_one_shard(pir):
+    """A two-query batch on a fleet of one shard (the split-K inner product,
+    two queries in one pass) equals each query answered alone and the oracle."""
+    g, o, e = _fleet(pir, 1024, 1, with_oracle=True)
+    qs = _queries(pir, 2, 1024, 1, 23)
+    _say("[batch] two queries, one shard")
+    reps, _ = pir.process_batch(g, [q for _, _, q, _ in qs])
+    for k, (pi, si, q, sk) in enumerate(qs):
+        single, _ = pir.process_queries(g, None, q, None)
+        assert reps[k] == single[0], k
+        oreps, _, _ = orc.process_queries(o, None, q, None)
+        assert reps[k] == oreps[0], k
+        assert np.array_equal(pir.process_reply(reps[k], sk), e[si][pi]), k

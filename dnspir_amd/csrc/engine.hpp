@@ -138,6 +138,11 @@ struct dnspir_ctx {
     int k1_mode = 0;  // 0: two streams from K1W_MIN_SHARDS shards on, 2: always, 3: never
     int chain_cus = 0;  // most CUs per secondary chain (0: by chain count; 1, 2 or 4)
     unsigned* h_err = nullptr;  // pinned copy of the chain timeout flag
+    // pinned staging of dnspir_process_queries (host buffers): two queries of
+    // up to HOST_QCAP bytes, two replies
+    static constexpr size_t HOST_QCAP = 80000;
+    uint8_t* h_q = nullptr;
+    uint8_t* h_r = nullptr;
     int exchange_polls = 1 << 21;  // bound of each exchange wait of the two-CU chain
     bool pair_pending = false;  // h_err is to be checked after the next sync
     int pair_held = 0;          // multi-CU chain blocks this context has reserved on the device
@@ -178,6 +183,8 @@ struct dnspir_ctx {
     ~dnspir_ctx() {
         drop_graphs();
         if (h_err) (void)hipHostFree(h_err);
+        if (h_q) (void)hipHostFree(h_q);
+        if (h_r) (void)hipHostFree(h_r);
     }
 
     dne::DevBuf* scratch_bufs[19] = {&se,    &M,     &te,    &aout, &cout, &qcoef,  &qs,     &qp,    &acc,  &outb,

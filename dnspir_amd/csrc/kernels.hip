@@ -53,7 +53,8 @@ __device__ __forceinline__ int brv_bits(int v, int bits) {
 
 // ---------------------------------------------------------------------------
 // register-radix NTT passes (radix 8, radix 4 for stages 9-10) over an LDS
-// array; Harvey lazy values ([0, 4p) forward, [0, 2p) inverse), Shoup products by twiddles
+// array; lazy values (forward: growing by 2p per stage, below 24p after a
+// transform; inverse: Harvey's [0, 2p)), Shoup products by twiddles
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t red2p(uint32_t a, uint32_t p2) {  // [0, 4p) -> [0, 2p)
     return csub(a, p2);
@@ -79,14 +80,21 @@ __device__ __forceinline__ int32_t crt_digit1(uint32_t e1, uint32_t e2) {
     const int64_t x = center_q(v);
     return (int32_t)((x + (1ll << (LOG2_B - 1))) >> LOG2_B);  // |d1| <= 2^24
 }
-// forward CT butterfly (Harvey): inputs and outputs in [0, 4p)
+// forward CT butterfly without reducing x: the Shoup product takes any 32-bit
+// y to [0, 2p), so inputs below B give outputs below B + 2p. An 11-stage
+// transform of inputs below 2p ends below 24p < 2^31 (p < 2^26); consumers
+// either feed the outputs to Shoup products (any 32-bit input) or reduce
+// them with red32.
 __device__ __forceinline__ void bf_ct(uint32_t& x, uint32_t& y, uint32_t w, uint32_t wq, uint32_t p) {
-    const uint32_t p2 = 2 * p;
-    const uint32_t u = red2p(x, p2);
+    const uint32_t u = x;
     const uint32_t t = shoup_lazy(y, w, wq, p);
     x = u + t;
-    y = u - t + p2;
+    y = u - t + 2 * p;
 }
+// any 32-bit x -> [0, p): a Shoup product by 1 (companion m = floor(2^32 / p))
+// leaves [0, 2p), one conditional subtraction
+constexpr uint32_t M32_P1 = (uint32_t)((1ull << 32) / P1), M32_P2 = (uint32_t)((1ull << 32) / P2);
+__device__ __forceinline__ uint32_t red32(uint32_t x, uint32_t p, uint32_t m) { return csub(x - __umulhi(x, m) * p, p); }
 // inverse GS butterfly, inputs/outputs in [0, 2p)
 __device__ __forceinline__ void bf_gs(uint32_t& x, uint32_t& y, uint32_t w, uint32_t wq, uint32_t p) {
     const uint32_t p2 = 2 * p;
@@ -310,7 +318,7 @@ __device__ __forceinline__ void fwd_pass2_lastw(uint32_t* buf, const uint2* tw2,
     }
 }
 // whole-block forward transform by 256 threads (g = thread index) on an LDS
-// array, inputs in [0, 2p), outputs left in [0, 4p) (callers reduce them as
+// array, inputs in [0, 2p), outputs left below 24p (callers reduce them as
 // they write them out); the inverse leaves the N factor in, outputs in [0, 2p)
 template <bool PAD = false>
 __device__ __forceinline__ void ntt256_fwd_lazy(uint32_t* buf, const uint32_t* tw, const uint32_t* twq, uint32_t p, int g) {
@@ -322,8 +330,6 @@ __device__ __forceinline__ void ntt256_fwd_lazy(uint32_t* buf, const uint32_t* t
     fwd_pass4<PAD>(buf, tw, twq, p, g);
     __syncthreads();
 }
-// [0, 4p) -> [0, p)
-__device__ __forceinline__ uint32_t red4p(uint32_t x, uint32_t p) { return csub(csub(x, 2 * p), p); }
 template <bool PAD = false>
 __device__ __forceinline__ void ntt256_inv_unscaled(uint32_t* buf, const uint32_t* tw, const uint32_t* twq, uint32_t p, int g) {
     inv_pass4<PAD>(buf, tw, twq, p, g);
@@ -364,7 +370,7 @@ __global__ __launch_bounds__(256) void k5_col_ntt_kernel(const uint16_t* __restr
     for (int s = threadIdx.x; s < N; s += blockDim.x) a[lx<true>(s)] = src[s];
     __syncthreads();
     ntt256_fwd_lazy<true>(a, T->psi[0], T->psiq[0], P1, threadIdx.x);
-    for (int s = threadIdx.x; s < N; s += blockDim.x) out[s] = red4p(a[lx<true>(s)], P1);
+    for (int s = threadIdx.x; s < N; s += blockDim.x) out[s] = red32(a[lx<true>(s)], P1, M32_P1);
 }
 
 // step 3 (coset_ntt, base_pir.rs:397-444) for a tile of TS slots of one lane,
@@ -551,9 +557,9 @@ __global__ __launch_bounds__(256) void k5_cols_final_kernel(const uint32_t* __re
     const int st = c < L ? 0 : 1;
     const int row = c < L ? c : c - L;
     uint32_t* dst = te_tmp + (((size_t)(st * nlanes + li) * L + row) * 2) * N;
-    for (int s = tid; s < N; s += nt) {  // [0, 4p) -> [0, p)
-        dst[s] = red4p(a[0][lx<true>(s)], P1);
-        dst[N + s] = red4p(a[1][lx<true>(s)], P2);
+    for (int s = tid; s < N; s += nt) {  // below 24p -> [0, p)
+        dst[s] = red32(a[0][lx<true>(s)], P1, M32_P1);
+        dst[N + s] = red32(a[1][lx<true>(s)], P2, M32_P2);
     }
 }
 
@@ -570,8 +576,9 @@ __global__ __launch_bounds__(256) void k5_cols_final_kernel(const uint32_t* __re
 // base_pir.rs:1085-1247), so the digits never leave the chip; PRE stores the
 // digits as the shard's IPGK rows.
 // NTT passes are radix-4 (inverse) and radix-8 (forward) over registers with
-// LDS exchanges; values stay in [0, 2p) / [0, 4p) between butterflies (Harvey's
-// lazy reduction with Shoup multiplications); twiddles live in LDS. Passes
+// LDS exchanges; values stay in [0, 2p) between inverse butterflies (Harvey's
+// lazy reduction) and grow below 24p through a forward transform, with Shoup
+// multiplications by twiddles; twiddles live in LDS. Passes
 // whose data stays within one wave order LDS per wave, not per block.
 // ---------------------------------------------------------------------------
 constexpr uint32_t NINV_Q2INV_P1 = 67055624u;  // N^-1 p2^-1 mod p1
@@ -700,10 +707,8 @@ __global__ __launch_bounds__(1024) void chain_kernel(const uint32_t* __restrict_
             const int j = tid + 1024 * k;
             const int xj = lx<true>(j);
             const int32_t d1 = crt_digit1(ecrt[k][0], ecrt[k][1]);
-            uint32_t t = (uint32_t)(d1 + (int32_t)P1);
-            xb[0][xj] = csub(t, P1);
-            t = (uint32_t)(d1 + (int32_t)P2);
-            xb[1][xj] = csub(t, P2);
+            xb[0][xj] = (uint32_t)(d1 + (int32_t)P1);  // [0, 2p): |d1| <= 2^24 < p
+            xb[1][xj] = (uint32_t)(d1 + (int32_t)P2);
         }
         lds_barrier();
         // sigma_5(a) of this step (sa is rewritten in D)
@@ -738,7 +743,7 @@ __global__ __launch_bounds__(1024) void chain_kernel(const uint32_t* __restrict_
             const int s = tid + 1024 * (k >> 1), r = k & 1;
             const uint32_t p = mod_p(r);
             const int K = mod_k(r);
-            const uint32_t d1 = xb[r][lx<true>(s)];  // NTT(D1), [0, 4p)
+            const uint32_t d1 = xb[r][lx<true>(s)];  // NTT(D1), below 24p
             // three Shoup products in [0, 2p) each
             sa[r][s] = red8p(shoup_lazy(sga[k], cg0[k], cg0q[k], p) + shoup_lazy(d1, cg1[k], cg1q[k], p) +
                                  shoup_lazy(te[k], cqa[k], cqaq[k], p),
@@ -753,8 +758,7 @@ __global__ __launch_bounds__(1024) void chain_kernel(const uint32_t* __restrict_
                 uint32_t* dst = ip_tmp + (((size_t)(st * nlanes + li) * L + i) * 6) * N;
                 const int gp = gpos_upd[k >> 1];
                 dst[(0 * 2 + r) * N + gp] = te[k];
-                uint32_t e1 = red2p(d1, 2 * p);
-                e1 = csub(e1, p);
+                const uint32_t e1 = red32(d1, p, r == 0 ? M32_P1 : M32_P2);
                 const uint32_t m = mul_mod(e1, (uint32_t)((1ull << LOG2_B) % p), p, K);
                 dst[(1 * 2 + r) * N + gp] = sga[k] >= m ? sga[k] - m : sga[k] + p - m;
                 dst[(2 * 2 + r) * N + gp] = e1;
@@ -1723,7 +1727,7 @@ __device__ __forceinline__ void c_update(const uint32_t* __restrict__ c_in, uint
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
         const int s = tid + 1024 * k;
-        const uint32_t d1 = d1t[lx<true>(s)];  // [0, 4p)
+        const uint32_t d1 = d1t[lx<true>(s)];  // below 24p
         const uint32_t cv = c_in[sig[k]] + shoup_lazy(sga[k], cb0[k], cq[0][k], p) + shoup_lazy(d1, cb1[k], cq[1][k], p) +
                             shoup_lazy(te[k], cqb[k], cq[2][k], p);
         c_out[s] = red8p(cv, p);  // [0, 7p) -> [0, p)
@@ -1906,8 +1910,7 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
             const int j = tid + 1024 * k;
             const int xj = lx<true>(j);
             const int32_t d1 = crt_digit1(ecrt[k][0], ecrt[k][1]);
-            const uint32_t t = (uint32_t)(d1 + (int32_t)p);
-            xd[0][xj] = csub(t, p);
+            xd[0][xj] = (uint32_t)(d1 + (int32_t)p);  // [0, 2p): |d1| <= 2^24 < p
         }
         lds_barrier();
         // ---- C: forward NTT of D1 mod p_r (threads 0..511, radix 4) ----
@@ -1929,7 +1932,7 @@ __global__ __launch_bounds__(1024) void chain_pair_kernel(const uint32_t* __rest
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
             const int s = tid + 1024 * k;
-            const uint32_t d1 = xd[0][lx<true>(s)];  // NTT(D1), [0, 4p)
+            const uint32_t d1 = xd[0][lx<true>(s)];  // NTT(D1), below 24p
             sa[s] = red8p(pa[k] + shoup_lazy(d1, cg1[k], cg1q[k], p), p);  // [0, 6p) -> [0, p)
         }
         lds_barrier();
@@ -1971,17 +1974,16 @@ __device__ __forceinline__ void c_update_half(const uint32_t* __restrict__ c_in,
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
         const int sl = tid + QUAD_NT * k;
-        const uint32_t d1 = d1t[lx<true>(off + sl)];  // [0, 4p)
+        const uint32_t d1 = d1t[lx<true>(off + sl)];  // below 24p
         const uint32_t cv = c_in[sig[k]] + shoup_lazy(sga[k], cb0[k], cq[0][k], p) + shoup_lazy(d1, cb1[k], cq[1][k], p) +
                             shoup_lazy(te[k], cqb[k], cq[2][k], p);
         c_out[sl] = red8p(cv, p);  // [0, 7p) -> [0, p)
     }
 }
 
-// crt_digit1 mod p
+// crt_digit1 mod p, in [0, 2p) (|d1| <= 2^24 < p)
 __device__ __forceinline__ uint32_t digit1_mod(uint32_t e1, uint32_t e2, uint32_t p) {
-    const uint32_t t = (uint32_t)(crt_digit1(e1, e2) + (int32_t)p);
-    return csub(t, p);
+    return (uint32_t)(crt_digit1(e1, e2) + (int32_t)p);
 }
 
 __global__ __launch_bounds__(QUAD_NT) void chain_quad_kernel(const uint32_t* __restrict__ te_tmp, uint32_t* __restrict__ a_out,
@@ -2217,7 +2219,7 @@ __global__ __launch_bounds__(QUAD_NT) void chain_quad_kernel(const uint32_t* __r
             const uint4 wb = *reinterpret_cast<const uint4*>(&tw2[0][1024 + 2 * gq]);
             bf_ct(v0, v1, wb.x, wb.y, p);
             bf_ct(v2, v3, wb.z, wb.w, p);
-            xd[0][b4] = v0; xd[0][b4 + 1] = v1; xd[0][b4 + 2] = v2; xd[0][b4 + 3] = v3;  // NTT(D1), [0, 4p)
+            xd[0][b4] = v0; xd[0][b4 + 1] = v1; xd[0][b4 + 2] = v2; xd[0][b4 + 3] = v3;  // NTT(D1), below 24p
             // [0, 6p) -> [0, p)
             sa[sd0] = red8p(pa[0] + shoup_lazy(v0, cg1[0], cg1q[0], p), p);
             sa[sd0 + 1] = red8p(pa[1] + shoup_lazy(v1, cg1[1], cg1q[1], p), p);

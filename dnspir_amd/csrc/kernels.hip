@@ -2597,27 +2597,33 @@ __global__ __launch_bounds__(K1W_TILES_NT) void k1w_tiles_kernel(const uint32_t*
 // shard group fastest, which keeps pages of all 256 shards open at once).
 template <int R, int PD, int U, int NQ, int SPB>
 __device__ __forceinline__ void k1w_body(const uint32_t* const* __restrict__ slabs, int nshards, const uint32_t* __restrict__ W,
-                                         uint32_t* __restrict__ outb, int nst, size_t q_stride, const u32x4* wl, int g, int sg,
-                                         int half) {
+                                         uint32_t* __restrict__ outb, int nst, size_t q_stride, const u32x4* wl, int g, int sg0,
+                                         int sg_step, int half) {
     const int S = 3 * nst, nsp = 2 * nst;
-    const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63, h = l >> 5;
+    // the wave index as a wave-uniform value (shard, rotation and row pointers in SGPRs)
+    const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), l = tid & 63, h = l >> 5;
     const int ai = (l & 31) >> 3;
     const int nk = nsp * 32;
-    const int sh = sg * SPB + wave;
+    // this wave's shards: wave w of shard groups sg0, sg0 + sg_step, ...
+    int sh = sg0 * SPB + wave;
     if (sh >= nshards) return;
-    const ARow arow = a_row(slabs[sh], R, half, g, S);
-    int acc[NQ][7];
+    const int a = g * 4 + ((l & 31) >> 3);
+    // the top-digit correction of this lane's output position, per query (the
+    // same for every shard)
+    uint32_t corr[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q)
-#pragma unroll
-        for (int w = 0; w < 7; ++w) acc[q][w] = 0;
+        corr[q] = W[q * (k1w_tile_bytes(nst) / 4) + (size_t)4 * (HALFN / 4) * nsp * 32 * 128 + (R * 2 + half) * HALFN + a];
     // K-step q: s' = q / 32 -> slab stream s = 3 st + (0: TE, 2: IPGK1), chunk
     // q % 32, i.e. slab K-step q + 32 * ((s' + 1) / 2). Each shard walks the
     // K-steps from its own starting point (the integer sums do not depend on
     // the order), so the 16 shards of a block, which stream the same a-group,
-    // read at different offsets. Loads are unconditional (index clamped to the
-    // last step).
-    const int rot = (sh * 37) & (nk - 1);
+    // read at different offsets. The last PD steps of a shard prefetch the
+    // first PD steps of the wave's next shard, so the stream does not drain
+    // at the shard's epilogue; after the wave's last shard they reload its last
+    // step.
+    ARow arow = a_row(slabs[sh], R, half, g, S);
+    int rot = (sh * 37) & (nk - 1);
     ATile abuf[PD];
 #pragma unroll
     for (int d = 0; d < PD; ++d) {
@@ -2625,59 +2631,74 @@ __device__ __forceinline__ void k1w_body(const uint32_t* const* __restrict__ sla
         load_a(abuf[d], arow, R, l, q + 32 * (((q >> 5) + 1) >> 1));
     }
     const int lb = ai * 2 + h;
-    // U steps per loop iteration (U a multiple of PD dividing nk): the body is
-    // long enough that the compiler keeps the PD-step prefetch within it
-    for (int kk0 = 0; kk0 < nk; kk0 += U) {
+    for (;;) {
+        const int shn = sh + sg_step * SPB;
+        const bool more = shn < nshards;
+        const ARow nrow = a_row(slabs[more ? shn : sh], R, half, g, S);
+        const int nrot = (shn * 37) & (nk - 1);
+        int acc[NQ][7];
 #pragma unroll
-        for (int dd = 0; dd < U; ++dd) {
-            const int d = dd % PD;
-            const int kk = kk0 + dd;
-            v4i a[4];
-            a[0] = abuf[d].d[0];
-            a[1] = abuf[d].d[1];
-            a[2] = abuf[d].d[2];
-            a[3] = R == 0 ? top_digits(abuf[d].top) : top_digits1(abuf[d].top);
-            const int qn = (min(kk + PD, nk - 1) + rot) & (nk - 1);
-            load_a(abuf[d], arow, R, l, qn + 32 * (((qn >> 5) + 1) >> 1));
-            // keep the prefetch here: without this the scheduler moves each
-            // load down to its use and only one step stays in flight
-            __builtin_amdgcn_sched_barrier(0);
+        for (int q = 0; q < NQ; ++q)
 #pragma unroll
-            for (int q = 0; q < NQ; ++q) {
-                // one weight digit vector at a time (4 registers live, not 16)
+            for (int w = 0; w < 7; ++w) acc[q][w] = 0;
+        // U steps per loop iteration (U a multiple of PD dividing nk): the body is
+        // long enough that the compiler keeps the PD-step prefetch within it
+        for (int kk0 = 0; kk0 < nk; kk0 += U) {
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const u32x4 b = wl[(q * nk + ((kk + rot) & (nk - 1))) * 32 + v * 8 + lb];
+            for (int dd = 0; dd < U; ++dd) {
+                const int d = dd % PD;
+                const int kk = kk0 + dd;
+                v4i av[4];
+                av[0] = abuf[d].d[0];
+                av[1] = abuf[d].d[1];
+                av[2] = abuf[d].d[2];
+                av[3] = R == 0 ? top_digits(abuf[d].top) : top_digits1(abuf[d].top);
+                const int kn = kk + PD;
+                const bool next = kn >= nk && more;
+                const int qn = kn < nk ? (kn + rot) & (nk - 1) : next ? (kn - nk + nrot) & (nk - 1) : (nk - 1 + rot) & (nk - 1);
+                load_a(abuf[d], next ? nrow : arow, R, l, qn + 32 * (((qn >> 5) + 1) >> 1));
+                // keep the prefetch here: without this the scheduler moves each
+                // load down to its use and only one step stays in flight
+                __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        int t = acc[q][u + v];
-                        t = __builtin_amdgcn_sdot4(a[u].x, (int)b.x, t, false);
-                        t = __builtin_amdgcn_sdot4(a[u].y, (int)b.y, t, false);
-                        t = __builtin_amdgcn_sdot4(a[u].z, (int)b.z, t, false);
-                        t = __builtin_amdgcn_sdot4(a[u].w, (int)b.w, t, false);
-                        acc[q][u + v] = t;
+                for (int q = 0; q < NQ; ++q) {
+                    // one weight digit vector at a time (4 registers live, not 16)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const u32x4 b = wl[(q * nk + ((kk + rot) & (nk - 1))) * 32 + v * 8 + lb];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            int t = acc[q][u + v];
+                            t = __builtin_amdgcn_sdot4(av[u].x, (int)b.x, t, false);
+                            t = __builtin_amdgcn_sdot4(av[u].y, (int)b.y, t, false);
+                            t = __builtin_amdgcn_sdot4(av[u].z, (int)b.z, t, false);
+                            t = __builtin_amdgcn_sdot4(av[u].w, (int)b.w, t, false);
+                            acc[q][u + v] = t;
+                        }
                     }
                 }
             }
         }
-    }
-    const uint32_t p = mod_p(R);
-    const int K = mod_k(R);
-    uint32_t cw[7];
-    weight_consts(p, cw);
-    const int row = l & 31;
-    const int a = g * 4 + (row >> 3), lane8 = row & 7;
+        const uint32_t p = mod_p(R);
+        const int K = mod_k(R);
+        uint32_t cw[7];
+        weight_consts(p, cw);
+        const int lane8 = l & 7;
 #pragma unroll
-    for (int q = 0; q < NQ; ++q) {
-        const uint32_t* corr = W + q * (k1w_tile_bytes(nst) / 4) + (size_t)4 * (HALFN / 4) * nsp * 32 * 128;
-        int64_t t = corr[(R * 2 + half) * HALFN + a];
+        for (int q = 0; q < NQ; ++q) {
+            int64_t t = corr[q];
 #pragma unroll
-        for (int w = 0; w < 7; ++w) {
-            const int tot = acc[q][w] + __shfl_xor(acc[q][w], 32);
-            t += (int64_t)tot * cw[w];
+            for (int w = 0; w < 7; ++w) {
+                const int tot = acc[q][w] + __shfl_xor(acc[q][w], 32);
+                t += (int64_t)tot * cw[w];
+            }
+            if (h == 0)
+                outb[(size_t)q * q_stride + (((size_t)sh * 2 + R) * LANES + lane8) * N + half * HALFN + a] = from_i64(t, p, K);
         }
-        if (h == 0)
-            outb[(size_t)q * q_stride + (((size_t)sh * 2 + R) * LANES + lane8) * N + half * HALFN + a] = from_i64(t, p, K);
+        if (!more) break;
+        sh = shn;
+        arow = nrow;
+        rot = nrot;
     }
 }
 
@@ -2697,11 +2718,8 @@ __global__ __launch_bounds__(64 * SPB) void k1w_kernel(const uint32_t* const* __
     }
     __syncthreads();
     // shard groups sg, sg + gridDim.y, ...: the weights stay in LDS across them
-    const int nsg = (nshards + SPB - 1) / SPB;
-    for (int sgi = sg; sgi < nsg; sgi += gridDim.y) {
-        if (r == 0) k1w_body<0, PD, U, NQ, SPB>(slabs, nshards, W, outb, nst, q_stride, wl, g, sgi, half);
-        else k1w_body<1, PD, U, NQ, SPB>(slabs, nshards, W, outb, nst, q_stride, wl, g, sgi, half);
-    }
+    if (r == 0) k1w_body<0, PD, U, NQ, SPB>(slabs, nshards, W, outb, nst, q_stride, wl, g, sg, gridDim.y, half);
+    else k1w_body<1, PD, U, NQ, SPB>(slabs, nshards, W, outb, nst, q_stride, wl, g, sg, gridDim.y, half);
 }
 
 // blocks per (a-group, r, half): one block fills the a-group's weights into

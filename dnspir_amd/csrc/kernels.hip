@@ -2137,9 +2137,10 @@ __global__ __launch_bounds__(QUAD_NT) void chain_quad_kernel(const uint32_t* __r
             te[k] = tn[k];
             sga[k] = sa[sig_upd[k]];
         }
-        // the two products of the a-update that do not need the digits
-        uint32_t pa[4];  // [0, 4p)
-        {
+        // the two products of the a-update that do not need the digits (only
+        // the transform threads update a)
+        uint32_t pa[4] = {0, 0, 0, 0};  // [0, 4p)
+        if (tid < 256) {
             const uint32_t tv[4] = {ted.x, ted.y, ted.z, ted.w};
 #pragma unroll
             for (int c = 0; c < 4; ++c)

@@ -23,6 +23,7 @@
 #include <unistd.h>
 
 #include <atomic>
+#include <chrono>
 #include <list>
 #include <memory>
 #include <mutex>
@@ -65,7 +66,7 @@ int write_full(int fd, const uint8_t* p, size_t n) {
             continue;
         }
         if (r > 0) put += (size_t)r;
-        else if (r < 0 && errno != EINTR) return DNSPIR_E_IO;
+        else if (r == 0 || errno != EINTR) return DNSPIR_E_IO;
     }
     return 0;
 }
@@ -182,13 +183,20 @@ struct dnspir_frontend {
     void accept_loop() {
         for (;;) {
             const int fd = ::accept(lfd, nullptr, nullptr);
-            std::lock_guard<std::mutex> lk(mu);
+            const int err = errno;
+            std::unique_lock<std::mutex> lk(mu);
             if (stopping) {
                 if (fd >= 0) ::close(fd);
                 return;
             }
             if (fd < 0) {
-                if (errno == EINTR || errno == ECONNABORTED || errno == EMFILE || errno == ENFILE) continue;
+                if (err == EMFILE || err == ENFILE) {
+                    // out of descriptors: retry once connections have closed
+                    lk.unlock();
+                    std::this_thread::sleep_for(std::chrono::milliseconds(10));
+                    continue;
+                }
+                if (err == EINTR || err == ECONNABORTED) continue;
                 return;
             }
             const int one = 1;

@@ -385,7 +385,8 @@ int dnspir_server_destroy(dnspir_server* s);
 
 typedef struct dnspir_frontend dnspir_frontend;
 typedef struct {
-    dnspir_server* server;   /* answers CMD_QUERY; NULL = no PIR (RESP_UNAVAILABLE) */
+    dnspir_server* server;   /* answers CMD_QUERY; NULL = no PIR (RESP_UNAVAILABLE); borrowed:
+                                destroy the front end before this server */
     int db;                  /* its database (dnspir_server_add_database) */
     size_t num_entries;      /* entries advertised by INFO; get_database_shape of it must
                                 give that database's group order and shard count */
